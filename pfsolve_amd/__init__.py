@@ -1,0 +1,154 @@
+"""pfsolve_amd -- MI355X (gfx950) Jones-Worland step behind PfSolve's plan API.
+
+The compute path is libpfsolve_gfx950.so (hand-written HIP kernels behind a C
+ABI, include/pfSolve.h).  This module binds that library with ctypes and
+mirrors the reference's plan interface: a PfSolveConfiguration describes the
+plan, initializePfSolve creates it, PfSolveAppend enqueues one step with a
+PfSolveLaunchParams, deletePfSolve frees it (reference:
+PfSolve/pfSolve_AppManagement/pfSolve_InitializeApp.h:1480,
+PfSolve/pfSolve_AppManagement/pfSolve_RunApp.h:80,
+PfSolve/pfSolve_AppManagement/pfSolve_DeleteApp.h:28).
+
+There is no CPU fallback: if the library is missing, importing the product
+API raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import _abi
+from ._abi import *  # noqa: F401,F403  (structs and macros)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpfsolve_gfx950.so")
+_lib: C.CDLL | None = None
+
+
+class PfSolveError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        self.code = code
+        name = "?"
+        if _lib is not None:
+            name = _lib.getPfSolveErrorString(code).decode()
+            detail = _lib.pfsolve_last_error()
+            if detail:
+                name += " (" + detail.decode() + ")"
+        super().__init__(f"{where}: {name} [{code}]")
+
+
+def lib() -> C.CDLL:
+    """The product library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m pfsolve_amd.build` "
+                "(there is no CPU fallback for the JW step)")
+        handle = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        _abi.declare(handle)
+        _abi.declare_ext(handle)
+        _lib = handle
+    return _lib
+
+
+def check(code: int, where: str) -> None:
+    if code != 0:
+        raise PfSolveError(code, where)
+
+
+def initializePfSolve(app: _abi.PfSolveApplication, cfg: _abi.PfSolveConfiguration) -> int:
+    return lib().initializePfSolve(C.byref(app), cfg)
+
+
+def PfSolveAppend(app: _abi.PfSolveApplication, inverse: int, lp: _abi.PfSolveLaunchParams | None) -> int:
+    return lib().PfSolveAppend(C.byref(app), inverse, C.byref(lp) if lp is not None else None)
+
+
+def deletePfSolve(app: _abi.PfSolveApplication) -> None:
+    lib().deletePfSolve(C.byref(app))
+
+
+def PfSolveSync(app: _abi.PfSolveApplication) -> int:
+    return lib().PfSolveSync(C.byref(app))
+
+
+def getPfSolveErrorString(code: int) -> str:
+    return lib().getPfSolveErrorString(code).decode()
+
+
+def PfSolveGetVersion() -> int:
+    return lib().PfSolveGetVersion()
+
+
+class Plan:
+    """One PfSolve application with its pointer arrays kept alive.
+
+    ``fields`` are PfSolveConfiguration fields (size=[N, B, 1] etc.);
+    ``matrices`` and ``rhs`` are device pointers (ints) of the plan's
+    `buffer` and `outputBuffer`; ``stream`` an optional hipStream_t (int).
+    """
+
+    def __init__(self, matrices: int, rhs: int, stream: int | None = None, **fields):
+        self._keep = []
+        cfg = _abi.PfSolveConfiguration()
+        for k, v in fields.items():
+            cur = getattr(cfg, k)
+            if isinstance(cur, C.Array):
+                arr = getattr(cfg, k)
+                for i, x in enumerate(v):
+                    arr[i] = x
+            else:
+                setattr(cfg, k, v)
+        self._dev = C.c_int(0)
+        cfg.device = C.pointer(self._dev)
+        self._buf = (C.c_void_p * 1)(matrices)
+        self._out = (C.c_void_p * 1)(rhs)
+        cfg.buffer = C.cast(self._buf, _abi.vpp)
+        cfg.outputBuffer = C.cast(self._out, _abi.vpp)
+        if stream is not None:
+            self._stream = (C.c_void_p * 1)(stream)
+            cfg.stream = C.cast(self._stream, _abi.vpp)
+            cfg.num_streams = 1
+        self.app = _abi.PfSolveApplication()
+        check(initializePfSolve(self.app, cfg), "initializePfSolve")
+        self.alive = True
+
+    def append(self, **launch) -> None:
+        lp = _abi.PfSolveLaunchParams()
+        for k, v in launch.items():
+            if k in ("buffer", "outputBuffer"):
+                arr = (C.c_void_p * 1)(v)
+                self._keep.append(arr)
+                setattr(lp, k, C.cast(arr, _abi.vpp))
+                continue
+            cur = getattr(lp, k)
+            if isinstance(cur, C.Array):
+                for i, x in enumerate(v):
+                    cur[i] = x
+            else:
+                setattr(lp, k, v)
+        check(PfSolveAppend(self.app, 0, lp), "PfSolveAppend")
+        del self._keep[:-4]
+
+    def geometry(self) -> _abi.PfSolveGeometry:
+        g = _abi.PfSolveGeometry()
+        check(lib().pfsolve_plan_geometry(C.byref(self.app), C.byref(g)), "pfsolve_plan_geometry")
+        return g
+
+    def close(self) -> None:
+        if self.alive:
+            deletePfSolve(self.app)
+            self.alive = False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

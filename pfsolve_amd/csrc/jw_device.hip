@@ -1,0 +1,931 @@
+// jw_device.hip -- gfx950 (CDNA4, wave64) kernels of the Jones-Worland step.
+//
+// Kernels
+//   jw_prologue_kernel  one wavefront per matrix set: reads the caller's
+//                       matrices in the reference storage (any JW mode or the
+//                       fused V/T/M step), factors the unit-diagonal solve
+//                       (LU of I+L+U, pivots by a Moebius-map scan across
+//                       lanes), and writes the per-row coefficient table and
+//                       the per-lane scan multipliers.
+//   jw_main_kernel<T,R,L>
+//                       persistent: each workgroup stages one table in LDS,
+//                       each wavefront owns 64/L groups of L lanes, each group
+//                       owns a pair of systems (the re/im halves of a complex
+//                       system in XXII layout), each lane owns R consecutive
+//                       rows.  Per pair: banded matvec (neighbour rows by
+//                       ds_bpermute), one forward and two backward first-order
+//                       recurrences, each as lane-local sweep + log2(L)-step
+//                       affine scan across lanes + lane-local redo, row-0
+//                       scaleC, store.  FP64 or double-double.
+//   jw_block_kernel     block plan: layout conversion + scale operator.
+//   jw_gbmv_kernel      dgbmv plan: general band matvec, x staged in LDS.
+//
+// Reference semantics restated (paths relative to the reference root):
+//   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel2/pfSolve_JonesWorland.h:38-408
+//   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_MatVecMul.h:28-133,383-446
+//   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_TridiagonalSolve.h:1578-1929
+//   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_ReadWrite.h:1561-1733,1906-2012,2275-2371
+//   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_RegistersScaleC.h:27-201
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "jw_internal.h"
+
+// ------------------------------------------------------------------------
+// double-double arithmetic (Dekker 1971; Knuth TwoSum; Joldes, Muller,
+// Popescu, ACM TOMS 44(2) 2017, Alg. 6 (add), 12 (mul), 17 (div)).
+// This file is compiled with -ffp-contract=off so that the error-free
+// transforms stay exact; FMAs are written explicitly.
+// ------------------------------------------------------------------------
+struct dd {
+    double hi, lo;
+};
+
+__device__ __forceinline__ dd dd_make(double h, double l) { dd r; r.hi = h; r.lo = l; return r; }
+
+__device__ __forceinline__ dd two_sum(double a, double b)
+{
+    double s = a + b;
+    double bb = s - a;
+    return dd_make(s, (a - (s - bb)) + (b - bb));
+}
+
+__device__ __forceinline__ dd fast_two_sum(double a, double b)
+{
+    double s = a + b;
+    return dd_make(s, b - (s - a));
+}
+
+__device__ __forceinline__ dd two_prod(double a, double b)
+{
+    double p = a * b;
+    return dd_make(p, __builtin_fma(a, b, -p));
+}
+
+__device__ __forceinline__ dd dd_add(dd x, dd y)
+{
+    dd s = two_sum(x.hi, y.hi);
+    dd t = two_sum(x.lo, y.lo);
+    double c = s.lo + t.hi;
+    dd v = fast_two_sum(s.hi, c);
+    double w = t.lo + v.lo;
+    return fast_two_sum(v.hi, w);
+}
+
+__device__ __forceinline__ dd dd_neg(dd x) { return dd_make(-x.hi, -x.lo); }
+__device__ __forceinline__ dd dd_sub(dd x, dd y) { return dd_add(x, dd_neg(y)); }
+
+__device__ __forceinline__ dd dd_mul(dd x, dd y)
+{
+    dd c = two_prod(x.hi, y.hi);
+    double tl0 = x.lo * y.lo;
+    double tl1 = __builtin_fma(x.hi, y.lo, tl0);
+    double cl2 = __builtin_fma(x.lo, y.hi, tl1);
+    double cl3 = c.lo + cl2;
+    return fast_two_sum(c.hi, cl3);
+}
+
+__device__ __forceinline__ dd dd_div(dd x, dd y)
+{
+    double th = x.hi / y.hi;
+    dd c = two_prod(y.hi, th);
+    double cl = __builtin_fma(y.lo, th, c.lo);
+    dd r = fast_two_sum(c.hi, cl);
+    double pih = x.hi - r.hi;
+    double dl = x.lo - r.lo;
+    double d = pih + dl;
+    double tl = d / y.hi;
+    return fast_two_sum(th, tl);
+}
+
+// ------------------------------------------------------------------------
+// cross-lane moves (ds_bpermute: any lane of the wavefront, 32 bits per op)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double lane_fetch(double v, int src_lane)
+{
+    int lo = __double2loint(v);
+    int hi = __double2hiint(v);
+    lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, lo);
+    hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, hi);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ dd lane_fetch(dd v, int src_lane)
+{
+    return dd_make(lane_fetch(v.hi, src_lane), lane_fetch(v.lo, src_lane));
+}
+
+// ------------------------------------------------------------------------
+// arithmetic traits: one kernel source for FP64 and double-double
+// ------------------------------------------------------------------------
+template <typename T> struct Ar;
+
+template <> struct Ar<double> {
+    __device__ static __forceinline__ double zero() { return 0.0; }
+    __device__ static __forceinline__ double one() { return 1.0; }
+    __device__ static __forceinline__ double from(double h, double) { return h; }
+    __device__ static __forceinline__ double add(double a, double b) { return a + b; }
+    __device__ static __forceinline__ double sub(double a, double b) { return a - b; }
+    __device__ static __forceinline__ double mul(double a, double b) { return a * b; }
+    __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+    __device__ static __forceinline__ double neg(double a) { return -a; }
+    // a + b*c and a - b*c as single roundings
+    __device__ static __forceinline__ double madd(double a, double b, double c) { return __builtin_fma(b, c, a); }
+    __device__ static __forceinline__ double msub(double a, double b, double c) { return __builtin_fma(-b, c, a); }
+    __device__ static __forceinline__ double hi(double a) { return a; }
+    __device__ static __forceinline__ double ldexp_(double a, int e) { return ldexp(a, e); }
+    __device__ static __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
+};
+
+template <> struct Ar<dd> {
+    __device__ static __forceinline__ dd zero() { return dd_make(0.0, 0.0); }
+    __device__ static __forceinline__ dd one() { return dd_make(1.0, 0.0); }
+    __device__ static __forceinline__ dd from(double h, double l) { return dd_make(h, l); }
+    __device__ static __forceinline__ dd add(dd a, dd b) { return dd_add(a, b); }
+    __device__ static __forceinline__ dd sub(dd a, dd b) { return dd_sub(a, b); }
+    __device__ static __forceinline__ dd mul(dd a, dd b) { return dd_mul(a, b); }
+    __device__ static __forceinline__ dd div(dd a, dd b) { return dd_div(a, b); }
+    __device__ static __forceinline__ dd neg(dd a) { return dd_neg(a); }
+    __device__ static __forceinline__ dd madd(dd a, dd b, dd c) { return dd_add(a, dd_mul(b, c)); }
+    __device__ static __forceinline__ dd msub(dd a, dd b, dd c) { return dd_sub(a, dd_mul(b, c)); }
+    __device__ static __forceinline__ double hi(dd a) { return a.hi; }
+    __device__ static __forceinline__ dd ldexp_(dd a, int e) { return dd_make(ldexp(a.hi, e), ldexp(a.lo, e)); }
+    __device__ static __forceinline__ dd sel(bool c, dd a, dd b) { return c ? a : b; }
+};
+
+// element access of caller memory: T storage, or FP64 storage promoted to T
+template <typename T> __device__ __forceinline__ T mem_load(const void* base, int64_t idx, int mem_f64)
+{
+    if (mem_f64) return Ar<T>::from(reinterpret_cast<const double*>(base)[idx], 0.0);
+    return reinterpret_cast<const T*>(base)[idx];
+}
+template <> __device__ __forceinline__ double mem_load<double>(const void* base, int64_t idx, int)
+{
+    return reinterpret_cast<const double*>(base)[idx];
+}
+template <typename T> __device__ __forceinline__ void mem_store(void* base, int64_t idx, T v, int mem_f64)
+{
+    if (mem_f64) {
+        // round DD to FP64 storage
+        reinterpret_cast<double*>(base)[idx] = Ar<T>::hi(Ar<T>::add(v, Ar<T>::zero()));
+        return;
+    }
+    reinterpret_cast<T*>(base)[idx] = v;
+}
+template <> __device__ __forceinline__ void mem_store<double>(void* base, int64_t idx, double v, int)
+{
+    reinterpret_cast<double*>(base)[idx] = v;
+}
+
+constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v / 2); }
+
+// reorder_i0 (storage position of logical row i), single-upload case N <= 3840:
+// PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:66-142
+__device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N)
+{
+    int64_t ws = ((N + 1023) / 1024) * 64;
+    if (N < 512) ws = 64;
+    int64_t R = (N + ws - 1) / ws;
+    if ((R & 1) == 0) R++;
+    int64_t ceilq = (N + R - 1) / R;
+    int64_t k = i % R;
+    int64_t rem = N % R;
+    int64_t ret = i / R;
+    if (k < rem)
+        ret += k * ceilq;
+    else
+        ret += rem * ceilq + (k - rem) * (N / R);
+    return ret;
+}
+
+// ------------------------------------------------------------------------
+// table layout (elements of T):
+//   coefficient pair p (0..4) of row-in-lane i, lane t:  ((p*R + i)*L + t)*2 + {0,1}
+//     p0 = (vm2, vm1)  p1 = (v0, vp1)  p2 = (vp2, l)  p3 = (g, h)  p4 = (D, E)
+//   scan multipliers of level k, lane t: SC + (k*L + t)*4 + {0: F, 1: B1, 2: B2}
+// ------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int64_t tab_coef(int p, int i, int t, int c, int R, int L)
+{
+    return ((int64_t)(p * R + i) * L + t) * 2 + c;
+}
+__host__ __device__ __forceinline__ int64_t tab_scan(int k, int t, int j, int R, int L)
+{
+    return (int64_t)JW_NCOEF * R * L + ((int64_t)k * L + t) * 4 + j;
+}
+__host__ __device__ __forceinline__ int ilog2_rt(int v)
+{
+    int r = 0;
+    while ((1 << r) < v) r++;
+    return r;
+}
+
+// ------------------------------------------------------------------------
+// prologue: one wavefront (64 lanes) per matrix set
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const void* mat, T* table, int L, int R)
+{
+    using A = Ar<T>;
+    extern __shared__ __align__(16) unsigned char pro_smem[];
+    const int RL = R * L;
+    T* sld = reinterpret_cast<T*>(pro_smem);
+    T* sud = sld + RL;
+    T* sE = sud + RL;
+    T* sl = sE + RL;
+    T* sh = sl + RL;
+    const int lane = threadIdx.x;
+    const int64_t N = d.N;
+    const int log2L = ilog2_rt(L);
+    const int mf = d.mem_f64;
+
+    // phase 1: raw coefficients per row, zeros outside the matrix
+    for (int r = lane; r < RL; r += 64) {
+        T v[5];
+        for (int k = 0; k < 5; k++) v[k] = A::zero();
+        T ldv = A::zero(), udv = A::zero(), Dv = A::zero(), Ev = A::zero();
+        if (r < N) {
+            const int64_t P = d.permuted ? reorder_pos(r, N) : (int64_t)r;
+            switch (d.band_kind) {
+            case JW_BAND_DIAG:
+                v[2] = mem_load<T>(mat, d.mdA + P, mf);
+                break;
+            case JW_BAND_UPPER:
+                v[2] = mem_load<T>(mat, d.mdA + P, mf);
+                if (r + 1 < N) v[3] = mem_load<T>(mat, d.udA + P, mf);
+                break;
+            case JW_BAND_LOWER:
+                v[2] = mem_load<T>(mat, d.mdA + P, mf);
+                if (r >= 1) v[1] = mem_load<T>(mat, d.ldA + P, mf);
+                break;
+            case JW_BAND_GB:
+                for (int off = -d.KL; off <= d.KU; off++) {
+                    int64_t col = r + off;
+                    if (col < 0 || col >= N || off < -2 || off > 2) continue;
+                    v[2 + off] = mem_load<T>(mat, d.offV + (int64_t)(d.KU - off) * N + col, mf);
+                }
+                break;
+            default:
+                v[2] = A::one();
+                break;
+            }
+            if (d.mdS >= 0) {
+                T s = mem_load<T>(mat, d.mdS + P, mf);
+                for (int k = 0; k < 5; k++) v[k] = A::mul(v[k], s);
+            }
+            if ((d.solve_kind == JW_SOLVE_LOWER || d.solve_kind == JW_SOLVE_TRI) && r >= 1)
+                ldv = mem_load<T>(mat, d.ldS + P, mf);
+            if ((d.solve_kind == JW_SOLVE_UPPER || d.solve_kind == JW_SOLVE_TRI) && r + 1 < N)
+                udv = mem_load<T>(mat, d.udS + P, mf);
+            if (d.has_m) {
+                Dv = mem_load<T>(mat, d.mdM + P, mf);
+                if (r + 1 < N) Ev = mem_load<T>(mat, d.udM + P, mf);
+            }
+        }
+        const int t = r / R, i = r % R;
+        table[tab_coef(0, i, t, 0, R, L)] = v[0];
+        table[tab_coef(0, i, t, 1, R, L)] = v[1];
+        table[tab_coef(1, i, t, 0, R, L)] = v[2];
+        table[tab_coef(1, i, t, 1, R, L)] = v[3];
+        table[tab_coef(2, i, t, 0, R, L)] = v[4];
+        table[tab_coef(4, i, t, 0, R, L)] = Dv;
+        table[tab_coef(4, i, t, 1, R, L)] = Ev;
+        sld[r] = ldv;
+        sud[r] = udv;
+        sE[r] = Ev;
+    }
+    __syncthreads();
+
+    // phase 2: factors of the unit-diagonal solve, lane t owns rows t*R .. t*R+R-1
+    const int t = lane;
+    const bool active = t < L;
+    if (d.solve_kind == JW_SOLVE_TRI) {
+        // pivots p_r = 1 - m_r / p_{r-1}, m_r = ld_r ud_{r-1}: the Moebius map
+        // p -> (p - m)/p is the 2x2 matrix [[1,-m],[1,0]] on (num, den).
+        T a = A::one(), b = A::zero(), c = A::zero(), e = A::one();
+        if (active) {
+            for (int i = 0; i < R; i++) {
+                const int r = t * R + i;
+                T m = (r >= 1 && r < N) ? A::mul(sld[r], sud[r - 1]) : A::zero();
+                T na = A::msub(a, m, c), nb = A::msub(b, m, e);
+                c = a;
+                e = b;
+                a = na;
+                b = nb;
+                double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
+                if (mx > 0.0) {
+                    int ex;
+                    frexp(mx, &ex);
+                    a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
+                }
+            }
+        }
+        // inclusive scan over lanes: X_t <- X_t * X_{t-s}
+        for (int k = 0; k < log2L; k++) {
+            const int s = 1 << k;
+            const int src = (t >= s) ? t - s : t;
+            T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
+            if (active && t >= s) {
+                T na = A::madd(A::mul(a, pa), b, pc);
+                T nb = A::madd(A::mul(a, pb), b, pe);
+                T nc = A::madd(A::mul(c, pa), e, pc);
+                T ne = A::madd(A::mul(c, pb), e, pe);
+                a = na; b = nb; c = nc; e = ne;
+                double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
+                if (mx > 0.0) {
+                    int ex;
+                    frexp(mx, &ex);
+                    a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
+                }
+            }
+        }
+        // exclusive: composition of the maps of lanes 0..t-1 applied to (1,1)
+        {
+            const int src = (t >= 1) ? t - 1 : t;
+            T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
+            T pprev = A::one();
+            if (t >= 1) pprev = A::div(A::add(pa, pb), A::add(pc, pe));
+            if (active) {
+                for (int i = 0; i < R; i++) {
+                    const int r = t * R + i;
+                    T l = A::zero(), g = A::zero(), h = A::zero();
+                    if (r < N) {
+                        T m = (r >= 1) ? A::mul(sld[r], sud[r - 1]) : A::zero();
+                        l = (r >= 1) ? A::div(sld[r], pprev) : A::zero();
+                        T p = A::sub(A::one(), A::div(m, pprev));
+                        g = A::div(A::one(), p);
+                        h = A::mul(sud[r], g);
+                        pprev = p;
+                    }
+                    sl[r] = l;
+                    sh[r] = h;
+                    table[tab_coef(2, i, t, 1, R, L)] = l;
+                    table[tab_coef(3, i, t, 0, R, L)] = g;
+                    table[tab_coef(3, i, t, 1, R, L)] = h;
+                }
+            }
+        }
+    } else if (active) {
+        for (int i = 0; i < R; i++) {
+            const int r = t * R + i;
+            T l = A::zero(), g = A::zero(), h = A::zero();
+            if (r < N) {
+                if (d.solve_kind == JW_SOLVE_LOWER) l = sld[r];
+                if (d.solve_kind == JW_SOLVE_UPPER) { g = A::one(); h = sud[r]; }
+            }
+            sl[r] = l;
+            sh[r] = h;
+            table[tab_coef(2, i, t, 1, R, L)] = l;
+            table[tab_coef(3, i, t, 0, R, L)] = g;
+            table[tab_coef(3, i, t, 1, R, L)] = h;
+        }
+    }
+
+    // phase 3: per-lane products of the recurrence multipliers and the scan tables
+    T bf = A::one(), bb1 = A::one(), bb2 = A::one();
+    if (active) {
+        for (int i = 0; i < R; i++) {
+            const int r = t * R + i;
+            bf = A::mul(bf, A::neg(sl[r]));
+            bb1 = A::mul(bb1, A::neg(sh[r]));
+            bb2 = A::mul(bb2, A::neg(sE[r]));
+        }
+    }
+    for (int k = 0; k < log2L; k++) {
+        const int s = 1 << k;
+        if (active) {
+            table[tab_scan(k, t, 0, R, L)] = bf;
+            table[tab_scan(k, t, 1, R, L)] = bb1;
+            table[tab_scan(k, t, 2, R, L)] = bb2;
+            table[tab_scan(k, t, 3, R, L)] = A::zero();
+        }
+        const int su = (t >= s) ? t - s : t;
+        const int sdn = (t + s < L) ? t + s : t;
+        T ubf = lane_fetch(bf, su);
+        T ub1 = lane_fetch(bb1, sdn);
+        T ub2 = lane_fetch(bb2, sdn);
+        if (active && t >= s) bf = A::mul(bf, ubf);
+        if (active && t + s < L) {
+            bb1 = A::mul(bb1, ub1);
+            bb2 = A::mul(bb2, ub2);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// main kernel
+// ------------------------------------------------------------------------
+template <typename T> struct Pair { T a, b; };
+
+template <typename T, int R, int L>
+__device__ __forceinline__ Pair<T> lds_pair(const T* tab, int p, int i, int t)
+{
+    Pair<T> r;
+    const T* q = tab + tab_coef(p, i, t, 0, R, L);
+    r.a = q[0];
+    r.b = q[1];
+    return r;
+}
+
+template <typename T, int R, int L>
+__global__ __launch_bounds__(512) void jw_main_kernel(jw_run_desc d)
+{
+    using A = Ar<T>;
+    extern __shared__ __align__(16) unsigned char jw_smem[];
+    T* tab = reinterpret_cast<T*>(jw_smem);
+    constexpr int LOG2L = ilog2_c(L);
+    constexpr int NTAB = JW_NCOEF * R * L + 4 * LOG2L * L;
+    constexpr int G = 64 / L;
+
+    // stage the coefficient table in LDS (16-byte moves)
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(d.table);
+        uint4* dst = reinterpret_cast<uint4*>(jw_smem);
+        constexpr int n16 = (NTAB * (int)sizeof(T)) / 16;
+        for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int t = lane & (L - 1);
+    const int g = lane / L;
+    const int r0 = t * R;
+    const int mf = d.mem_f64;
+    const int64_t npass = (d.nsys + 2 * G - 1) / (2 * G);
+
+    for (int64_t q = (int64_t)blockIdx.x * nwaves + wave; q < npass; q += (int64_t)gridDim.x * nwaves) {
+        const int64_t sys0 = (q * G + g) * 2;
+        T x[2][R];
+
+        // ---- load (rows outside [in_lo, in_hi) and absent systems read as 0) ----
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int64_t sys = sys0 + s;
+            const bool ok = sys < d.nsys;
+            const int64_t base = sys * d.stride + d.offSol;
+            if constexpr (sizeof(T) == 8 && (R % 2) == 0) {
+                if (d.vec2) {
+#pragma unroll
+                    for (int i = 0; i < R; i += 2) {
+                        const int r = r0 + i;
+                        const bool k0 = ok && r >= d.in_lo && r < d.in_hi;
+                        const bool k1 = ok && r + 1 >= d.in_lo && r + 1 < d.in_hi;
+                        const double* p = reinterpret_cast<const double*>(d.rhs) + base + r;
+                        if (k0 && k1) {
+                            double2 v = *reinterpret_cast<const double2*>(p);
+                            x[s][i] = v.x;
+                            x[s][i + 1] = v.y;
+                        } else {
+                            x[s][i] = k0 ? p[0] : 0.0;
+                            x[s][i + 1] = k1 ? p[1] : 0.0;
+                        }
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const int r = r0 + i;
+                const bool k = ok && r >= d.in_lo && r < d.in_hi;
+                x[s][i] = k ? mem_load<T>(d.rhs, base + r, mf) : A::zero();
+            }
+        }
+
+        // ---- banded matvec u = V x (in place) ----
+        if (d.band) {
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                // x[r0-2], x[r0-1], x[r0+R], x[r0+R+1] from the neighbour lanes of the group
+                T xm2 = A::zero(), xm1 = A::zero(), xp1 = A::zero(), xp2 = A::zero();
+                if (d.kl >= 1) {
+                    T f = lane_fetch(x[s][R - 1], t >= 1 ? lane - 1 : lane);
+                    xm1 = A::sel(t >= 1, f, A::zero());
+                }
+                if (d.kl >= 2) {
+                    if constexpr (R >= 2) {
+                        T f = lane_fetch(x[s][R - 2], t >= 1 ? lane - 1 : lane);
+                        xm2 = A::sel(t >= 1, f, A::zero());
+                    } else {
+                        T f = lane_fetch(x[s][0], t >= 2 ? lane - 2 : lane);
+                        xm2 = A::sel(t >= 2, f, A::zero());
+                    }
+                }
+                if (d.ku >= 1) {
+                    T f = lane_fetch(x[s][0], t + 1 < L ? lane + 1 : lane);
+                    xp1 = A::sel(t + 1 < L, f, A::zero());
+                }
+                if (d.ku >= 2) {
+                    if constexpr (R >= 2) {
+                        T f = lane_fetch(x[s][1], t + 1 < L ? lane + 1 : lane);
+                        xp2 = A::sel(t + 1 < L, f, A::zero());
+                    } else {
+                        T f = lane_fetch(x[s][0], t + 2 < L ? lane + 2 : lane);
+                        xp2 = A::sel(t + 2 < L, f, A::zero());
+                    }
+                }
+                T pm2 = xm2, pm1 = xm1;
+#pragma unroll
+                for (int i = 0; i < R; i++) {
+                    Pair<T> c0 = lds_pair<T, R, L>(tab, 0, i, t);   // vm2, vm1
+                    Pair<T> c1 = lds_pair<T, R, L>(tab, 1, i, t);   // v0, vp1
+                    T vp2 = tab[tab_coef(2, i, t, 0, R, L)];
+                    const T xi = x[s][i];
+                    const T n1 = (i + 1 < R) ? x[s][(i + 1) % R] : xp1;
+                    const T n2 = (i + 2 < R) ? x[s][(i + 2) % R] : ((i + 2 == R) ? xp1 : xp2);
+                    T u = A::mul(c1.a, xi);
+                    u = A::madd(u, c0.b, pm1);
+                    u = A::madd(u, c0.a, pm2);
+                    u = A::madd(u, c1.b, n1);
+                    u = A::madd(u, vp2, n2);
+                    pm2 = pm1;
+                    pm1 = xi;
+                    x[s][i] = u;
+                }
+            }
+        }
+
+        // ---- forward recurrence w_r = u_r - l_r w_{r-1} ----
+        if (d.fwd) {
+            T lr[R];
+#pragma unroll
+            for (int i = 0; i < R; i++) lr[i] = tab[tab_coef(2, i, t, 1, R, L)];
+            T acc[2];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                T a = A::zero();
+#pragma unroll
+                for (int i = 0; i < R; i++) a = A::msub(x[s][i], lr[i], a);
+                acc[s] = a;
+            }
+#pragma unroll
+            for (int k = 0; k < LOG2L; k++) {
+                const int sft = 1 << k;
+                const T bk = tab[tab_scan(k, t, 0, R, L)];
+                const int src = t >= sft ? lane - sft : lane;
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    T ap = lane_fetch(acc[s], src);
+                    acc[s] = A::sel(t >= sft, A::madd(acc[s], bk, ap), acc[s]);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                T c = lane_fetch(acc[s], t >= 1 ? lane - 1 : lane);
+                T w = A::sel(t >= 1, c, A::zero());
+#pragma unroll
+                for (int i = 0; i < R; i++) {
+                    w = A::msub(x[s][i], lr[i], w);
+                    x[s][i] = w;
+                }
+            }
+        }
+
+        // ---- backward recurrences z_r = g_r w_r - h_r z_{r+1}, then y_r = D_r z_r - E_r y_{r+1} ----
+#pragma unroll
+        for (int stage = 0; stage < 2; stage++) {
+            if (stage == 0 && !d.b1) continue;
+            if (stage == 1 && !d.b2) continue;
+            const int p = stage == 0 ? 3 : 4;
+            T gg[R], hh[R];
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                Pair<T> c = lds_pair<T, R, L>(tab, p, i, t);
+                gg[i] = c.a;
+                hh[i] = c.b;
+            }
+            T acc[2];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                T a = A::zero();
+#pragma unroll
+                for (int i = R - 1; i >= 0; i--) {
+                    x[s][i] = A::mul(gg[i], x[s][i]);
+                    a = A::msub(x[s][i], hh[i], a);
+                }
+                acc[s] = a;
+            }
+#pragma unroll
+            for (int k = 0; k < LOG2L; k++) {
+                const int sft = 1 << k;
+                const T bk = tab[tab_scan(k, t, 1 + stage, R, L)];
+                const int src = t + sft < L ? lane + sft : lane;
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    T ap = lane_fetch(acc[s], src);
+                    acc[s] = A::sel(t + sft < L, A::madd(acc[s], bk, ap), acc[s]);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                T c = lane_fetch(acc[s], t + 1 < L ? lane + 1 : lane);
+                T z = A::sel(t + 1 < L, c, A::zero());
+#pragma unroll
+                for (int i = R - 1; i >= 0; i--) {
+                    z = A::msub(x[s][i], hh[i], z);
+                    x[s][i] = z;
+                }
+            }
+        }
+
+        // ---- row-0 scaleC (pfSolve_TridiagonalSolve.h:1918-1927) ----
+        if (d.apply_scale && t == 0) {
+            const T sc = A::from(d.scale_hi, d.scale_lo);
+#pragma unroll
+            for (int s = 0; s < 2; s++) x[s][0] = A::mul(x[s][0], sc);
+        }
+
+        // ---- store (rows in [out_lo, out_hi) only) ----
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int64_t sys = sys0 + s;
+            const bool ok = sys < d.nsys;
+            const int64_t base = sys * d.stride + d.offSol;
+            if constexpr (sizeof(T) == 8 && (R % 2) == 0) {
+                if (d.vec2) {
+#pragma unroll
+                    for (int i = 0; i < R; i += 2) {
+                        const int r = r0 + i;
+                        const bool k0 = ok && r >= d.out_lo && r < d.out_hi;
+                        const bool k1 = ok && r + 1 >= d.out_lo && r + 1 < d.out_hi;
+                        double* p = reinterpret_cast<double*>(d.rhs) + base + r;
+                        if (k0 && k1) {
+                            double2 v;
+                            v.x = x[s][i];
+                            v.y = x[s][i + 1];
+                            *reinterpret_cast<double2*>(p) = v;
+                        } else {
+                            if (k0) p[0] = x[s][i];
+                            if (k1) p[1] = x[s][i + 1];
+                        }
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const int r = r0 + i;
+                if (ok && r >= d.out_lo && r < d.out_hi) mem_store<T>(d.rhs, base + r, x[s][i], mf);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// block plan kernel (FP64): PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel2/pfSolve_block.h:37-115
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double block_scale(int op, int64_t n, int lshift, double s)
+{
+    const double x = (double)n, l = (double)lshift;
+    switch (op) {
+    case 1: return s;
+    case 2: return 2.0 * (x + l + 1.0) * sqrt((x + 1.0) / (x + s + l + 1.5));
+    case 3: return 4.0 * (x + l + 2.0) * (x + l + 3.0) *
+                   sqrt((x + 1.0) * (x + 2.0) / ((x + s + l + 2.5) * (x + s + l + 3.5)));
+    case 4: return (x + l + 1.0) * (4.0 * l + 6.0) * sqrt((x + 1.0) / (x + s + l + 1.5));
+    default: return 1.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void jw_block_kernel(jw_block_desc d)
+{
+    const double* in = reinterpret_cast<const double*>(d.in);
+    double* out = reinterpret_cast<double*>(d.out);
+    const int64_t total = d.N * d.B;
+    const int L = d.layout;
+    const bool in_complex = (L == 2 || L == 3 || L == 4);
+    const bool out_complex = (L == 5 || L == 6 || L == 7);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t xr = e % d.N, y = e / d.N;
+        double re = 0.0, im = 0.0;
+        if (!d.skip_read && xr >= d.in_lo && xr < d.in_hi) {
+            int64_t idx = y * d.inStride + d.offIn;
+            if (L == 7) idx *= 2;
+            idx += xr;
+            if (in_complex) {
+                double2 v = reinterpret_cast<const double2*>(in)[idx];
+                re = v.x;
+                im = v.y;
+            } else if (L == 6) {
+                im = in[idx];
+            } else if (L == 7) {
+                re = in[idx];
+                im = in[idx + d.inStride];
+            } else {
+                re = in[idx];
+            }
+        }
+        if (!d.skip_read && d.op >= 1 && d.op <= 4) {
+            const double f = block_scale(d.op, xr, d.lshift, d.scale);
+            re *= f;
+            im *= f;
+        }
+        if (!(xr >= d.out_lo && xr < d.out_hi)) continue;
+        int64_t o = y * d.outStride + d.offOut;
+        if (L == 4) o *= 2;
+        o += xr;
+        if (L == 2) {
+            if (d.accumulate) re += out[o];
+            out[o] = re;
+        } else if (L == 3) {
+            if (d.accumulate) im += out[o];
+            out[o] = im;
+        } else if (L == 4) {
+            if (d.accumulate) {
+                re += out[o];
+                im += out[o + d.outStride];
+            }
+            out[o] = re;
+            out[o + d.outStride] = im;
+        } else if (out_complex) {
+            double2* oc = reinterpret_cast<double2*>(out);
+            if (d.accumulate) {
+                double2 prev = oc[o];
+                re += prev.x;
+                im += prev.y;
+            }
+            double2 v;
+            v.x = re;
+            v.y = im;
+            oc[o] = v;
+        } else {
+            if (d.accumulate) re += out[o];
+            out[o] = re;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// dgbmv plan kernel (FP64): y = A_band x in place
+// PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel2/pfSolve_dgbmv.h:38-112,
+// PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_MatVecMul.h:383-446
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void jw_gbmv_kernel(jw_gbmv_desc d)
+{
+    extern __shared__ double xs[];
+    const double* A = reinterpret_cast<const double*>(d.mat);
+    double* rhs = reinterpret_cast<double*>(d.rhs);
+    for (int64_t sys = blockIdx.x; sys < d.nsys; sys += gridDim.x) {
+        const int64_t base = sys * d.stride + d.offSol;
+        __syncthreads();
+        for (int64_t r = threadIdx.x; r < d.N; r += blockDim.x)
+            xs[r] = (r >= d.in_lo && r < d.in_hi) ? rhs[base + r] : 0.0;
+        __syncthreads();
+        for (int64_t r = threadIdx.x; r < d.N; r += blockDim.x) {
+            double acc = 0.0;
+            for (int j = 0; j < d.LDA; j++) {
+                const int64_t col = r + d.KU - j;
+                if (col < 0 || col >= d.N) continue;
+                acc = __builtin_fma(xs[col], A[d.offM + (int64_t)j * d.N + col], acc);
+            }
+            if (r >= d.out_lo && r < d.out_hi) rhs[base + r] = acc;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// host-side launchers (extern "C")
+// ------------------------------------------------------------------------
+static thread_local char g_err[256];
+static void set_err(const char* what, hipError_t e)
+{
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+}
+
+typedef void (*main_fn)(jw_run_desc);
+
+#define JW_F64_CASE(LL, RR) \
+    if (L == LL && R == RR) return reinterpret_cast<const void*>(&jw_main_kernel<double, RR, LL>);
+#define JW_DD_CASE(LL, RR) \
+    if (L == LL && R == RR) return reinterpret_cast<const void*>(&jw_main_kernel<dd, RR, LL>);
+
+static const void* pick_main(int prec, int L, int R)
+{
+    if (prec == JW_PREC_F64) {
+        JW_F64_CASE(64, 1) JW_F64_CASE(64, 2) JW_F64_CASE(64, 3) JW_F64_CASE(64, 4)
+        JW_F64_CASE(64, 5) JW_F64_CASE(64, 6) JW_F64_CASE(64, 7) JW_F64_CASE(64, 8)
+        JW_F64_CASE(64, 9) JW_F64_CASE(64, 10) JW_F64_CASE(64, 11) JW_F64_CASE(64, 12)
+        JW_F64_CASE(64, 13) JW_F64_CASE(64, 14) JW_F64_CASE(64, 15) JW_F64_CASE(64, 16)
+        JW_F64_CASE(32, 1) JW_F64_CASE(32, 2) JW_F64_CASE(32, 4) JW_F64_CASE(32, 8)
+        JW_F64_CASE(16, 1) JW_F64_CASE(16, 2) JW_F64_CASE(16, 4)
+        JW_F64_CASE(8, 1) JW_F64_CASE(8, 2) JW_F64_CASE(8, 4) JW_F64_CASE(8, 8)
+    } else {
+        JW_DD_CASE(64, 1) JW_DD_CASE(64, 2) JW_DD_CASE(64, 3) JW_DD_CASE(64, 4)
+        JW_DD_CASE(64, 5) JW_DD_CASE(64, 6) JW_DD_CASE(64, 7) JW_DD_CASE(64, 8)
+        JW_DD_CASE(32, 1) JW_DD_CASE(32, 2) JW_DD_CASE(32, 4) JW_DD_CASE(32, 8)
+        JW_DD_CASE(16, 1) JW_DD_CASE(16, 2) JW_DD_CASE(16, 4)
+        JW_DD_CASE(8, 1) JW_DD_CASE(8, 2) JW_DD_CASE(8, 4) JW_DD_CASE(8, 8)
+    }
+    return nullptr;
+}
+
+extern "C" {
+
+const char* jwdev_last_error(void) { return g_err; }
+
+int jwdev_supported(int L, int R, int precision) { return pick_main(precision, L, R) != nullptr; }
+
+int64_t jwdev_table_elems(int L, int R)
+{
+    return (int64_t)JW_NCOEF * R * L + 4LL * ilog2_rt(L) * L;
+}
+
+int64_t jwdev_lds_bytes(int L, int R, int precision)
+{
+    return jwdev_table_elems(L, R) * (precision == JW_PREC_F64 ? 8 : 16);
+}
+
+int jwdev_device_info(int* cu_count, int* lds_per_block, int* arch_major, int* arch_minor)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { set_err("hipGetDevice", e); return -1; }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) { set_err("hipGetDeviceProperties", e); return -1; }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (lds_per_block) *lds_per_block = (int)prop.sharedMemPerBlock;
+    if (arch_major) *arch_major = prop.major;
+    if (arch_minor) *arch_minor = prop.minor;
+    return 0;
+}
+
+int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, int L, int R,
+                   int precision, void* stream)
+{
+    const int RL = R * L;
+    const size_t elt = precision == JW_PREC_F64 ? 8 : 16;
+    const size_t lds = 5 * (size_t)RL * elt;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const void* fn = precision == JW_PREC_F64 ? reinterpret_cast<const void*>(&jw_prologue_kernel<double>)
+                                              : reinterpret_cast<const void*>(&jw_prologue_kernel<dd>);
+    hipError_t ea = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (ea != hipSuccess) { set_err("hipFuncSetAttribute(prologue)", ea); return -2; }
+    if (precision == JW_PREC_F64) {
+        hipLaunchKernelGGL(jw_prologue_kernel<double>, dim3(1), dim3(64), lds, s, *d, matrices,
+                           reinterpret_cast<double*>(table), L, R);
+    } else {
+        hipLaunchKernelGGL(jw_prologue_kernel<dd>, dim3(1), dim3(64), lds, s, *d, matrices,
+                           reinterpret_cast<dd*>(table), L, R);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { set_err("prologue launch", e); return -1; }
+    return 0;
+}
+
+int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream)
+{
+    const void* f = pick_main(precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD, L, R);
+    if (!f) { snprintf(g_err, sizeof(g_err), "no kernel for L=%d R=%d prec=%d", L, R, precision); return -1; }
+    const size_t lds = (size_t)jwdev_lds_bytes(L, R, precision);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) { set_err("hipFuncSetAttribute", e); return -2; }
+    if (waves < 1 || waves > 8) waves = 4;
+    void* args[] = {const_cast<jw_run_desc*>(r)};
+    e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) { set_err("main launch", e); return -3; }
+    return 0;
+}
+
+int jwdev_main_occupancy(int L, int R, int precision, int waves)
+{
+    const void* f = pick_main(precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD, L, R);
+    if (!f) return 0;
+    const size_t lds = (size_t)jwdev_lds_bytes(L, R, precision);
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int nb = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * waves, lds);
+    if (e != hipSuccess) { set_err("occupancy", e); return 0; }
+    return nb;
+}
+
+int jwdev_block(const jw_block_desc* d, void* stream)
+{
+    const int64_t total = d->N * d->B;
+    int64_t grid = (total + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(jw_block_kernel, dim3((unsigned)grid), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), *d);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { set_err("block launch", e); return -1; }
+    return 0;
+}
+
+int jwdev_gbmv(const jw_gbmv_desc* d, void* stream)
+{
+    int64_t grid = d->nsys < 8192 ? d->nsys : 8192;
+    if (grid < 1) grid = 1;
+    const size_t lds = (size_t)d->N * 8;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&jw_gbmv_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) { set_err("hipFuncSetAttribute(gbmv)", e); return -2; }
+    hipLaunchKernelGGL(jw_gbmv_kernel, dim3((unsigned)grid), dim3(256), lds,
+                       reinterpret_cast<hipStream_t>(stream), *d);
+    e = hipGetLastError();
+    if (e != hipSuccess) { set_err("gbmv launch", e); return -1; }
+    return 0;
+}
+
+}  // extern "C"

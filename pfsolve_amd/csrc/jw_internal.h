@@ -1,0 +1,116 @@
+/*
+ * jw_internal.h -- descriptors shared by the C host code (jw_plan.c,
+ * pfsolve_api.c) and the gfx950 device code (jw_device.hip).  Plain C.
+ *
+ * Data model of the JW step on the device
+ * ---------------------------------------
+ * Every supported operator is y = B2^{-1} B1^{-1} F^{-1} (V x), row 0 scaled,
+ * where V is banded (|offset| <= 2) and F (unit lower bidiagonal), B1, B2
+ * (upper bidiagonal) are the factors of the solve matrices.  A prologue kernel
+ * turns the caller's matrices (in the reference storage, any mode) into one
+ * coefficient table per matrix set:
+ *   per row r:  vm2 vm1 v0 vp1 vp2 | l | g h | D E
+ *     u_r = sum_d v_d[r] x_{r+d}
+ *     w_r = u_r - l_r w_{r-1}                  (forward, F)
+ *     z_r = g_r w_r - h_r z_{r+1}              (backward, B1)
+ *     y_r = D_r z_r - E_r y_{r+1}              (backward, B2)
+ *   per lane t and scan level k: the products of the recurrence multipliers
+ *   over the lanes that level k combines.
+ * The main kernel applies the table to every right-hand side.
+ */
+#ifndef PFSOLVE_JW_INTERNAL_H
+#define PFSOLVE_JW_INTERNAL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { JW_BAND_NONE = 0, JW_BAND_DIAG = 1, JW_BAND_UPPER = 2, JW_BAND_LOWER = 3, JW_BAND_GB = 4 };
+enum { JW_SOLVE_NONE = 0, JW_SOLVE_LOWER = 1, JW_SOLVE_UPPER = 2, JW_SOLVE_TRI = 3 };
+enum { JW_PREC_F64 = 1, JW_PREC_DD = 2, JW_PREC_DD_F64MEM = 3 };
+
+#define JW_NCOEF 10           /* coefficients per row in the table */
+#define JW_NPAIR 5            /* ... stored as 5 pairs */
+#define JW_MAX_BAND 2         /* |offset| of V */
+
+/* Where the prologue finds the raw coefficients of one matrix set.
+ * Offsets are element offsets into the matrix buffer. */
+typedef struct jw_source_desc {
+    int64_t N;
+    int32_t band_kind;        /* JW_BAND_* */
+    int32_t solve_kind;       /* JW_SOLVE_* */
+    int64_t mdA, udA, ldA;    /* JW matvec diagonals (storage permuted when `permuted`) */
+    int64_t offV;             /* band-major V, natural order (JW_BAND_GB) */
+    int32_t KL, KU;
+    int64_t ldS, udS;         /* unit-diagonal solve off-diagonals */
+    int64_t mdS;              /* fused step: T pre-scale diagonal, -1 = none */
+    int32_t has_m;            /* fused step: upper bidiagonal M present */
+    int32_t permuted;         /* 1: diagonals stored in reorder_i0 order */
+    int64_t mdM, udM;
+    int32_t Msplit0;
+    int32_t mem_f64;          /* DD compute with FP64 storage */
+} jw_source_desc;
+
+/* One launch of the main kernel over a batch of right-hand sides. */
+typedef struct jw_run_desc {
+    void* rhs;                /* systems, solved in place */
+    const void* table;        /* coefficient table (prologue output) */
+    int64_t nsys;             /* real systems */
+    int64_t stride;           /* elements between systems */
+    int64_t offSol;           /* element offset of system 0 */
+    int64_t in_lo, in_hi;     /* rows read:    [in_lo, in_hi)  (already clipped to [0,N)) */
+    int64_t out_lo, out_hi;   /* rows written: [out_lo, out_hi) */
+    int32_t N;
+    int32_t band, kl, ku;     /* band present, lower / upper band width used */
+    int32_t fwd, b1, b2;      /* recurrences present */
+    int32_t apply_scale;
+    double scale_hi, scale_lo;
+    int32_t vec2;             /* FP64: 16-byte aligned pairs may use 128-bit access */
+    int32_t mem_f64;          /* DD compute with FP64 storage */
+} jw_run_desc;
+
+/* Uneven-size batch: one work item per matrix set. */
+typedef struct jw_batch_item {
+    void* rhs;
+    const void* table;
+    int64_t nsys, stride, offSol;
+    int32_t N, L, R;
+    int32_t table_bytes_lds;  /* bytes of table staged into LDS */
+} jw_batch_item;
+
+typedef struct jw_block_desc {
+    const void* in;
+    void* out;
+    int64_t N, B;
+    int64_t inStride, outStride, offIn, offOut;
+    int64_t in_lo, in_hi, out_lo, out_hi;
+    int32_t layout, accumulate, op, lshift, skip_read;
+    double scale;
+} jw_block_desc;
+
+typedef struct jw_gbmv_desc {
+    const void* mat;
+    void* rhs;
+    int64_t N, nsys, stride, offSol, offM;
+    int64_t in_lo, in_hi, out_lo, out_hi;
+    int32_t LDA, KU, KL;
+} jw_gbmv_desc;
+
+/* device entry points (jw_device.hip) */
+int jwdev_supported(int L, int R, int precision);
+int64_t jwdev_table_elems(int L, int R);
+int64_t jwdev_lds_bytes(int L, int R, int precision);
+int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, int L, int R,
+                   int precision, void* stream);
+int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream);
+int jwdev_block(const jw_block_desc* d, void* stream);
+int jwdev_gbmv(const jw_gbmv_desc* d, void* stream);
+int jwdev_device_info(int* cu_count, int* lds_per_block, int* arch_major, int* arch_minor);
+const char* jwdev_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,54 @@
+/*
+ * jw_plan.h -- plan state and the host-side resolution of reference modes.
+ */
+#ifndef PFSOLVE_JW_PLAN_H
+#define PFSOLVE_JW_PLAN_H
+
+#include "pfsolve_amd.h"
+#include "jw_internal.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PF_KIND_NONE = 0, PF_KIND_JW = 1, PF_KIND_BLOCK = 2, PF_KIND_GBMV = 3 };
+
+struct pfsolve_plan {
+    int kind;
+    int precision;            /* JW_PREC_* */
+    int64_t N;
+    int L, R, waves;
+    int cu_count, occupancy;
+    void* table;              /* device: coefficient table */
+    size_t table_bytes;
+    int64_t lds_bytes;
+    hipEvent_t ev_start, ev_stop;   /* optional timing events around the main kernel */
+};
+
+/* Applies the reference's configuration defaults (pfSolve_InitializeApp.h:981-1212)
+ * and validates the JW / block / dgbmv configuration. */
+PfSolveResult jw_configure(const PfSolveConfiguration* in, PfSolveConfiguration* out);
+/* kind of plan a configuration asks for */
+int jw_plan_kind(const PfSolveConfiguration* cfg);
+int jw_precision(const PfSolveConfiguration* cfg);
+/* geometry of the JW kernels for N rows */
+PfSolveResult jw_select_geometry(int64_t N, int precision, int* L, int* R);
+
+/* Per-launch resolution: which diagonals the prologue reads and what the main
+ * kernel applies (pfSolve_JonesWorland.h:117-348). */
+PfSolveResult jw_resolve_launch(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp,
+                                int64_t nsys, jw_source_desc* src, jw_run_desc* run);
+PfSolveResult jw_resolve_block(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp,
+                               int64_t nsys, jw_block_desc* d);
+PfSolveResult jw_resolve_gbmv(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp,
+                              int64_t nsys, jw_gbmv_desc* d);
+/* largest matrix / RHS element index a JW launch touches, and sanity of offsets */
+void jw_index_extent(const jw_source_desc* s, const jw_run_desc* r, int64_t* mat_max, int64_t* rhs_max);
+int jw_offsets_valid(const jw_source_desc* s, const jw_run_desc* r);
+/* keep range [lo, hi) of a zero-pad pair, clipped to [0, N) */
+void jw_keep_range(int64_t zp0, int64_t zp1, int runtime, int64_t N, int64_t* lo, int64_t* hi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,444 @@
+/*
+ * pfsolve_api.c -- the drop-in C entry points (pfSolve.h) and the extension
+ * introspection functions (pfsolve_amd.h).  Host code in C; device work is
+ * enqueued through the launchers of jw_device.hip.
+ *
+ *   initializePfSolve  PfSolve/pfSolve_AppManagement/pfSolve_InitializeApp.h:1480-1644
+ *   PfSolveAppend      PfSolve/pfSolve_AppManagement/pfSolve_RunApp.h:80-159
+ *   deletePfSolve      PfSolve/pfSolve_AppManagement/pfSolve_DeleteApp.h:28-312
+ */
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "jw_plan.h"
+
+int jwdev_main_occupancy(int L, int R, int precision, int waves);
+
+static hipStream_t plan_stream(const PfSolveApplication* app)
+{
+    return app->configuration.stream ? app->configuration.stream[0] : (hipStream_t)0;
+}
+
+static PfSolveResult fill_geometry(const PfSolveConfiguration* cfg, pfsolve_geometry* g)
+{
+    memset(g, 0, sizeof(*g));
+    g->precision = jw_precision(cfg);
+    g->waves = 4;
+    if (jw_plan_kind(cfg) != PF_KIND_JW) return PFSOLVE_SUCCESS;
+    int L = 0, R = 0;
+    PfSolveResult r = jw_select_geometry(cfg->M_size, g->precision, &L, &R);
+    if (r != PFSOLVE_SUCCESS) return r;
+    g->lanes = L;
+    g->rows_per_lane = R;
+    g->table_bytes = jwdev_table_elems(L, R) * (g->precision == JW_PREC_F64 ? 8 : 16);
+    g->lds_bytes = jwdev_lds_bytes(L, R, g->precision);
+    return PFSOLVE_SUCCESS;
+}
+
+PfSolveResult pfsolve_validate_configuration(const PfSolveConfiguration* cfg, pfsolve_geometry* g)
+{
+    if (!cfg) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    PfSolveConfiguration c;
+    PfSolveResult r = jw_configure(cfg, &c);
+    if (r != PFSOLVE_SUCCESS) return r;
+    pfsolve_geometry tmp;
+    r = fill_geometry(&c, g ? g : &tmp);
+    if (r != PFSOLVE_SUCCESS) return r;
+    if (jw_plan_kind(&c) == PF_KIND_JW) {
+        const pfsolve_geometry* gg = g ? g : &tmp;
+        if (!jwdev_supported(gg->lanes, gg->rows_per_lane, gg->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD))
+            return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+    }
+    return PFSOLVE_SUCCESS;
+}
+
+void deletePfSolve(PfSolveApplication* app)
+{
+    if (!app) return;
+    struct pfsolve_plan* p = app->localFFTPlan;
+    if (p) {
+        if (p->table) hipFree(p->table);
+        free(p);
+    }
+    app->localFFTPlan = NULL;
+}
+
+PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in)
+{
+    if (!app) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    app->localFFTPlan = NULL;
+    /* HIP branch of setConfigurationPfSolve: a device pointer is required (pfSolve_InitializeApp.h:656-659) */
+    if (in.device == NULL) return PFSOLVE_ERROR_INVALID_DEVICE;
+    PfSolveResult res = jw_configure(&in, &app->configuration);
+    if (res != PFSOLVE_SUCCESS) { deletePfSolve(app); return res; }
+    PfSolveConfiguration* cfg = &app->configuration;
+
+    struct pfsolve_plan* p = (struct pfsolve_plan*)calloc(1, sizeof(*p));
+    if (!p) return PFSOLVE_ERROR_MALLOC_FAILED;
+    app->localFFTPlan = p;
+    p->kind = jw_plan_kind(cfg);
+    p->precision = jw_precision(cfg);
+    p->N = cfg->M_size;
+
+    int cu = 0, lds_blk = 0, maj = 0, min = 0;
+    if (jwdev_device_info(&cu, &lds_blk, &maj, &min) != 0) { deletePfSolve(app); return PFSOLVE_ERROR_INVALID_DEVICE; }
+    p->cu_count = cu;
+    cfg->computeCapabilityMajor = (uint64_t)maj;
+    cfg->computeCapabilityMinor = (uint64_t)min;
+    cfg->sharedMemorySize = (uint64_t)lds_blk;
+    cfg->maxThreadsNum = 1024;
+
+    if (p->kind == PF_KIND_JW) {
+        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_jw_%llu_%llu_%d_%d_%lld_%llu_%d_%d",
+                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->upperBanded,
+                 cfg->jw_type, (long long)cfg->jw_control_bitmask, (unsigned long long)cfg->Msplit[0],
+                 cfg->numConsecutiveJWIterations, cfg->useMultipleInputBuffers);
+        pfsolve_geometry g;
+        res = fill_geometry(cfg, &g);
+        if (res != PFSOLVE_SUCCESS) { deletePfSolve(app); return res; }
+        p->L = g.lanes;
+        p->R = g.rows_per_lane;
+        p->waves = g.waves;
+        p->lds_bytes = g.lds_bytes;
+        p->table_bytes = (size_t)g.table_bytes;
+        const int kp = p->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD;
+        if (!jwdev_supported(p->L, p->R, kp)) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
+        if (hipMalloc(&p->table, p->table_bytes) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
+        p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
+        if (p->occupancy < 1) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY; }
+    } else if (p->kind == PF_KIND_BLOCK) {
+        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_block_%llu_%llu_%d_%d_%lld",
+                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->block, cfg->lshift,
+                 (long long)cfg->jw_control_bitmask);
+    } else {
+        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_JW_dgbmv_%llu_%llu_%d_%d_%d_%lld",
+                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->LDA, cfg->KU, cfg->KL,
+                 (long long)cfg->jw_control_bitmask);
+    }
+    return PFSOLVE_SUCCESS;
+}
+
+PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchParams* lp)
+{
+    (void)inverse;
+    if (!app || !app->localFFTPlan) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    struct pfsolve_plan* p = app->localFFTPlan;
+    PfSolveConfiguration* cfg = &app->configuration;
+    /* buffer swap and checks: PfSolve/pfSolve_PlanManagement/pfSolve_API_handles/pfSolve_UpdateBuffers.h:616-699 */
+    if (lp) {
+        if (lp->buffer) cfg->buffer = lp->buffer;
+        if (lp->outputBuffer) cfg->outputBuffer = lp->outputBuffer;
+        if (lp->inputBuffer) cfg->inputBuffer = lp->inputBuffer;
+        if (lp->kernel) cfg->kernel = lp->kernel;
+    }
+    if (cfg->inputBuffer == NULL) cfg->inputBuffer = cfg->buffer;
+    if (cfg->outputBuffer == NULL) cfg->outputBuffer = cfg->buffer;
+    if (cfg->buffer == NULL || cfg->buffer[0] == NULL) return PFSOLVE_ERROR_EMPTY_buffer;
+    if (cfg->outputBuffer == NULL || cfg->outputBuffer[0] == NULL) return PFSOLVE_ERROR_EMPTY_outputBuffer;
+    /* batchSize > 0 replaces size[1] (pfSolve_UpdateBuffers.h:1558) */
+    if (lp && lp->batchSize > 0) cfg->size[1] = lp->batchSize;
+    hipStream_t s = plan_stream(app);
+
+    if (p->kind == PF_KIND_JW) {
+        const int64_t nsys = (int64_t)(cfg->size[1] * cfg->size[2]);
+        jw_source_desc src;
+        jw_run_desc run;
+        PfSolveResult r = jw_resolve_launch(cfg, lp, nsys, &src, &run);
+        if (r != PFSOLVE_SUCCESS) return r;
+        /* index guard: every element the kernels touch must lie inside the
+         * buffers whose sizes the caller declared (bufferSize/outputBufferSize) */
+        if (!jw_offsets_valid(&src, &run)) return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+        {
+            int64_t mmax, rmax;
+            jw_index_extent(&src, &run, &mmax, &rmax);
+            const int64_t elt = p->precision == JW_PREC_DD ? 16 : 8;
+            if (cfg->bufferSize && mmax >= 0 && (uint64_t)((mmax + 1) * elt) > cfg->bufferSize[0])
+                return PFSOLVE_ERROR_EMPTY_bufferSize;
+            if (cfg->outputBufferSize && rmax >= 0 && (uint64_t)((rmax + 1) * elt) > cfg->outputBufferSize[0])
+                return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+        }
+        run.rhs = cfg->outputBuffer[0];
+        run.table = p->table;
+        const uintptr_t base = (uintptr_t)run.rhs;
+        run.vec2 = p->precision == JW_PREC_F64 && (base % 16) == 0 && (run.stride % 2) == 0 && (run.offSol % 2) == 0;
+        if (jwdev_prologue(&src, cfg->buffer[0], p->table, p->L, p->R, p->precision, s) != 0)
+            return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+        if (nsys <= 0) return PFSOLVE_SUCCESS;
+        const int G = 64 / p->L;
+        const int64_t npass = (nsys + 2 * G - 1) / (2 * G);
+        int64_t need = (npass + p->waves - 1) / p->waves;
+        int64_t grid = (int64_t)p->cu_count * p->occupancy;
+        if (grid > need) grid = need;
+        if (grid < 1) grid = 1;
+        if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+        if (jwdev_main(&run, p->L, p->R, p->precision, p->waves, (int)grid, s) != 0)
+            return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+        if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+        return PFSOLVE_SUCCESS;
+    }
+    if (p->kind == PF_KIND_BLOCK) {
+        jw_block_desc d;
+        PfSolveResult r = jw_resolve_block(cfg, lp, (int64_t)cfg->size[1], &d);
+        if (r != PFSOLVE_SUCCESS) return r;
+        d.in = cfg->buffer[0];
+        d.out = cfg->outputBuffer[0];
+        if (d.N * d.B <= 0) return PFSOLVE_SUCCESS;
+        if (d.inStride < 0 || d.outStride < 0 || d.offIn < 0 || d.offOut < 0) return PFSOLVE_ERROR_EMPTY_bufferSize;
+        {
+            /* doubles touched: complex buffers count 2 per element */
+            const int L = d.layout;
+            int64_t in_max = -1, out_max = -1;
+            if (d.in_hi > 0 && !d.skip_read) {
+                int64_t e = (d.B - 1) * d.inStride + d.offIn;
+                if (L == 2 || L == 3 || L == 4) in_max = 2 * (e + d.in_hi - 1) + 1;
+                else if (L == 7) in_max = 2 * e + d.in_hi - 1 + d.inStride;
+                else in_max = e + d.in_hi - 1;
+            }
+            if (d.out_hi > 0) {
+                int64_t e = (d.B - 1) * d.outStride + d.offOut;
+                if (L == 4) out_max = 2 * e + d.out_hi - 1 + d.outStride;
+                else if (L == 5 || L == 6 || L == 7) out_max = 2 * (e + d.out_hi - 1) + 1;
+                else out_max = e + d.out_hi - 1;
+            }
+            if (cfg->bufferSize && in_max >= 0 && (uint64_t)((in_max + 1) * 8) > cfg->bufferSize[0])
+                return PFSOLVE_ERROR_EMPTY_bufferSize;
+            if (cfg->outputBufferSize && out_max >= 0 && (uint64_t)((out_max + 1) * 8) > cfg->outputBufferSize[0])
+                return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+        }
+        return jwdev_block(&d, s) == 0 ? PFSOLVE_SUCCESS : PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    }
+    if (p->kind == PF_KIND_GBMV) {
+        jw_gbmv_desc d;
+        PfSolveResult r = jw_resolve_gbmv(cfg, lp, (int64_t)cfg->size[1], &d);
+        if (r != PFSOLVE_SUCCESS) return r;
+        d.mat = cfg->buffer[0];
+        d.rhs = cfg->outputBuffer[0];
+        if (d.nsys <= 0) return PFSOLVE_SUCCESS;
+        if (d.offM < 0 || d.offSol < 0 || d.stride < 0 || (d.nsys > 1 && d.stride < d.N))
+            return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+        if (cfg->bufferSize && (uint64_t)((d.offM + (int64_t)d.LDA * d.N) * 8) > cfg->bufferSize[0])
+            return PFSOLVE_ERROR_EMPTY_bufferSize;
+        if (cfg->outputBufferSize && (uint64_t)(((d.nsys - 1) * d.stride + d.offSol + d.N) * 8) > cfg->outputBufferSize[0])
+            return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+        return jwdev_gbmv(&d, s) == 0 ? PFSOLVE_SUCCESS : PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    }
+    return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+}
+
+/* With one stream per plan there is nothing to join (pfSolve_RunApp.h:28-57 waits on
+ * per-stream events only when num_streams > 1). */
+PfSolveResult PfSolveSync(PfSolveApplication* app)
+{
+    if (!app) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    return PFSOLVE_SUCCESS;
+}
+
+/* The reference returns the literal 00102, an octal constant (pfSolve.h:101-103). */
+int PfSolveGetVersion(void) { return 00102; }
+
+typedef struct { int code; const char* name; } err_name;
+static const err_name k_err_names[] = {
+    {0, "PFSOLVE_SUCCESS"},
+    {1, "PFSOLVE_ERROR_MALLOC_FAILED"},
+    {2, "PFSOLVE_ERROR_INSUFFICIENT_CODE_BUFFER"},
+    {3, "PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER"},
+    {4, "PFSOLVE_ERROR_PLAN_NOT_INITIALIZED"},
+    {5, "PFSOLVE_ERROR_NULL_TEMP_PASSED"},
+    {1001, "PFSOLVE_ERROR_INVALID_PHYSICAL_DEVICE"},
+    {1002, "PFSOLVE_ERROR_INVALID_DEVICE"},
+    {1003, "PFSOLVE_ERROR_INVALID_QUEUE"},
+    {1004, "PFSOLVE_ERROR_INVALID_COMMAND_POOL"},
+    {1005, "PFSOLVE_ERROR_INVALID_FENCE"},
+    {1006, "PFSOLVE_ERROR_ONLY_FORWARD_FFT_INITIALIZED"},
+    {1007, "PFSOLVE_ERROR_ONLY_INVERSE_FFT_INITIALIZED"},
+    {1008, "PFSOLVE_ERROR_INVALID_CONTEXT"},
+    {1009, "PFSOLVE_ERROR_INVALID_PLATFORM"},
+    {1010, "PFSOLVE_ERROR_ENABLED_saveApplicationToString"},
+    {1011, "PFSOLVE_ERROR_EMPTY_FILE"},
+    {2001, "PFSOLVE_ERROR_EMPTY_FFTdim"},
+    {2002, "PFSOLVE_ERROR_EMPTY_size"},
+    {2003, "PFSOLVE_ERROR_EMPTY_bufferSize"},
+    {2004, "PFSOLVE_ERROR_EMPTY_buffer"},
+    {2005, "PFSOLVE_ERROR_EMPTY_tempBufferSize"},
+    {2006, "PFSOLVE_ERROR_EMPTY_tempBuffer"},
+    {2007, "PFSOLVE_ERROR_EMPTY_inputBufferSize"},
+    {2008, "PFSOLVE_ERROR_EMPTY_inputBuffer"},
+    {2009, "PFSOLVE_ERROR_EMPTY_outputBufferSize"},
+    {2010, "PFSOLVE_ERROR_EMPTY_outputBuffer"},
+    {2011, "PFSOLVE_ERROR_EMPTY_kernelSize"},
+    {2012, "PFSOLVE_ERROR_EMPTY_kernel"},
+    {2013, "PFSOLVE_ERROR_EMPTY_applicationString"},
+    {2014, "PFSOLVE_ERROR_EMPTY_useCustomBluesteinPaddingPattern_arrays"},
+    {3001, "PFSOLVE_ERROR_UNSUPPORTED_RADIX"},
+    {3002, "PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH"},
+    {3003, "PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH_R2C"},
+    {3004, "PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH_DCT"},
+    {3005, "PFSOLVE_ERROR_UNSUPPORTED_FFT_OMIT"},
+    {4001, "PFSOLVE_ERROR_FAILED_TO_ALLOCATE"},
+    {4002, "PFSOLVE_ERROR_FAILED_TO_MAP_MEMORY"},
+    {4003, "PFSOLVE_ERROR_FAILED_TO_ALLOCATE_COMMAND_BUFFERS"},
+    {4004, "PFSOLVE_ERROR_FAILED_TO_BEGIN_COMMAND_BUFFER"},
+    {4005, "PFSOLVE_ERROR_FAILED_TO_END_COMMAND_BUFFER"},
+    {4006, "PFSOLVE_ERROR_FAILED_TO_SUBMIT_QUEUE"},
+    {4007, "PFSOLVE_ERROR_FAILED_TO_WAIT_FOR_FENCES"},
+    {4008, "PFSOLVE_ERROR_FAILED_TO_RESET_FENCES"},
+    {4009, "PFSOLVE_ERROR_FAILED_TO_CREATE_DESCRIPTOR_POOL"},
+    {4010, "PFSOLVE_ERROR_FAILED_TO_CREATE_DESCRIPTOR_SET_LAYOUT"},
+    {4011, "PFSOLVE_ERROR_FAILED_TO_ALLOCATE_DESCRIPTOR_SETS"},
+    {4012, "PFSOLVE_ERROR_FAILED_TO_CREATE_PIPELINE_LAYOUT"},
+    {4013, "PFSOLVE_ERROR_FAILED_SHADER_PREPROCESS"},
+    {4014, "PFSOLVE_ERROR_FAILED_SHADER_PARSE"},
+    {4015, "PFSOLVE_ERROR_FAILED_SHADER_LINK"},
+    {4016, "PFSOLVE_ERROR_FAILED_SPIRV_GENERATE"},
+    {4017, "PFSOLVE_ERROR_FAILED_TO_CREATE_SHADER_MODULE"},
+    {4018, "PFSOLVE_ERROR_FAILED_TO_CREATE_INSTANCE"},
+    {4019, "PFSOLVE_ERROR_FAILED_TO_SETUP_DEBUG_MESSENGER"},
+    {4020, "PFSOLVE_ERROR_FAILED_TO_FIND_PHYSICAL_DEVICE"},
+    {4021, "PFSOLVE_ERROR_FAILED_TO_CREATE_DEVICE"},
+    {4022, "PFSOLVE_ERROR_FAILED_TO_CREATE_FENCE"},
+    {4023, "PFSOLVE_ERROR_FAILED_TO_CREATE_COMMAND_POOL"},
+    {4024, "PFSOLVE_ERROR_FAILED_TO_CREATE_BUFFER"},
+    {4025, "PFSOLVE_ERROR_FAILED_TO_ALLOCATE_MEMORY"},
+    {4026, "PFSOLVE_ERROR_FAILED_TO_BIND_BUFFER_MEMORY"},
+    {4027, "PFSOLVE_ERROR_FAILED_TO_FIND_MEMORY"},
+    {4028, "PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE"},
+    {4029, "PFSOLVE_ERROR_FAILED_TO_COPY"},
+    {4030, "PFSOLVE_ERROR_FAILED_TO_CREATE_PROGRAM"},
+    {4031, "PFSOLVE_ERROR_FAILED_TO_COMPILE_PROGRAM"},
+    {4032, "PFSOLVE_ERROR_FAILED_TO_GET_CODE_SIZE"},
+    {4033, "PFSOLVE_ERROR_FAILED_TO_GET_CODE"},
+    {4034, "PFSOLVE_ERROR_FAILED_TO_DESTROY_PROGRAM"},
+    {4035, "PFSOLVE_ERROR_FAILED_TO_LOAD_MODULE"},
+    {4036, "PFSOLVE_ERROR_FAILED_TO_GET_FUNCTION"},
+    {4037, "PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY"},
+    {4038, "PFSOLVE_ERROR_FAILED_TO_MODULE_GET_GLOBAL"},
+    {4039, "PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL"},
+    {4040, "PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD"},
+    {4041, "PFSOLVE_ERROR_FAILED_TO_ADD_NAME_EXPRESSION"},
+    {4042, "PFSOLVE_ERROR_FAILED_TO_INITIALIZE"},
+    {4043, "PFSOLVE_ERROR_FAILED_TO_SET_DEVICE_ID"},
+    {4044, "PFSOLVE_ERROR_FAILED_TO_GET_DEVICE"},
+    {4045, "PFSOLVE_ERROR_FAILED_TO_CREATE_CONTEXT"},
+    {4046, "PFSOLVE_ERROR_FAILED_TO_CREATE_PIPELINE"},
+    {4047, "PFSOLVE_ERROR_FAILED_TO_SET_KERNEL_ARG"},
+    {4048, "PFSOLVE_ERROR_FAILED_TO_CREATE_COMMAND_QUEUE"},
+    {4049, "PFSOLVE_ERROR_FAILED_TO_RELEASE_COMMAND_QUEUE"},
+    {4050, "PFSOLVE_ERROR_FAILED_TO_ENUMERATE_DEVICES"},
+    {4051, "PFSOLVE_ERROR_FAILED_TO_GET_ATTRIBUTE"},
+    {4052, "PFSOLVE_ERROR_FAILED_TO_CREATE_EVENT"},
+    {4053, "PFSOLVE_ERROR_FAILED_TO_CREATE_COMMAND_LIST"},
+    {4054, "PFSOLVE_ERROR_FAILED_TO_DESTROY_COMMAND_LIST"},
+    {4055, "PFSOLVE_ERROR_FAILED_TO_SUBMIT_BARRIER"},
+};
+
+/* Same strings as the reference's switch (pfSolve_Structs.h:545-733), which has no
+ * case for PFSOLVE_ERROR_MATH_FAILED. */
+const char* getPfSolveErrorString(PfSolveResult result)
+{
+    for (size_t i = 0; i < sizeof(k_err_names) / sizeof(k_err_names[0]); i++)
+        if (k_err_names[i].code == (int)result) return k_err_names[i].name;
+    return "Unknown PfSolve error";
+}
+
+/* ---- extension introspection ---- */
+size_t pfsolve_sizeof_configuration(void) { return sizeof(PfSolveConfiguration); }
+size_t pfsolve_sizeof_launch_params(void) { return sizeof(PfSolveLaunchParams); }
+size_t pfsolve_sizeof_application(void) { return sizeof(PfSolveApplication); }
+
+#define OFF(S, f) {#f, offsetof(S, f)}
+typedef struct { const char* name; size_t off; } field_off;
+static const field_off k_cfg_fields[] = {
+    OFF(PfSolveConfiguration, FFTdim), OFF(PfSolveConfiguration, size), OFF(PfSolveConfiguration, device),
+    OFF(PfSolveConfiguration, stream), OFF(PfSolveConfiguration, num_streams),
+    OFF(PfSolveConfiguration, bufferSize), OFF(PfSolveConfiguration, outputBufferSize),
+    OFF(PfSolveConfiguration, kernelSize), OFF(PfSolveConfiguration, buffer),
+    OFF(PfSolveConfiguration, inputBuffer), OFF(PfSolveConfiguration, outputBuffer),
+    OFF(PfSolveConfiguration, kernel), OFF(PfSolveConfiguration, upperBanded),
+    OFF(PfSolveConfiguration, jw_type), OFF(PfSolveConfiguration, block), OFF(PfSolveConfiguration, lshift),
+    OFF(PfSolveConfiguration, LDA), OFF(PfSolveConfiguration, KU), OFF(PfSolveConfiguration, KL),
+    OFF(PfSolveConfiguration, ALT_dgbmv), OFF(PfSolveConfiguration, Msplit),
+    OFF(PfSolveConfiguration, jw_control_bitmask), OFF(PfSolveConfiguration, M_size),
+    OFF(PfSolveConfiguration, offsetM), OFF(PfSolveConfiguration, offsetV),
+    OFF(PfSolveConfiguration, offsetSolution), OFF(PfSolveConfiguration, inputZeropad),
+    OFF(PfSolveConfiguration, outputZeropad), OFF(PfSolveConfiguration, scaleC),
+    OFF(PfSolveConfiguration, doublePrecision), OFF(PfSolveConfiguration, quadDoubleDoublePrecision),
+    OFF(PfSolveConfiguration, quadDoubleDoublePrecisionDoubleMemory), OFF(PfSolveConfiguration, halfPrecision),
+    OFF(PfSolveConfiguration, isInputFormatted), OFF(PfSolveConfiguration, isOutputFormatted),
+    OFF(PfSolveConfiguration, inputBufferStride), OFF(PfSolveConfiguration, outputBufferStride),
+    OFF(PfSolveConfiguration, keepShaderCode), OFF(PfSolveConfiguration, disableCaching),
+    OFF(PfSolveConfiguration, numConsecutiveJWIterations), OFF(PfSolveConfiguration, useMultipleInputBuffers),
+    OFF(PfSolveConfiguration, offsetT), OFF(PfSolveConfiguration, autotune),
+};
+static const field_off k_lp_fields[] = {
+    OFF(PfSolveLaunchParams, buffer), OFF(PfSolveLaunchParams, outputBuffer), OFF(PfSolveLaunchParams, kernel),
+    OFF(PfSolveLaunchParams, M_size), OFF(PfSolveLaunchParams, offsetM), OFF(PfSolveLaunchParams, offsetV),
+    OFF(PfSolveLaunchParams, offsetSolution), OFF(PfSolveLaunchParams, inputZeropad),
+    OFF(PfSolveLaunchParams, outputZeropad), OFF(PfSolveLaunchParams, scaleC),
+    OFF(PfSolveLaunchParams, inputBufferStride), OFF(PfSolveLaunchParams, outputBufferStride),
+    OFF(PfSolveLaunchParams, batchSize), OFF(PfSolveLaunchParams, offsetT),
+};
+
+static int64_t find_off(const field_off* t, size_t n, const char* name)
+{
+    if (!name) return -1;
+    for (size_t i = 0; i < n; i++)
+        if (strcmp(t[i].name, name) == 0) return (int64_t)t[i].off;
+    return -1;
+}
+int64_t pfsolve_offsetof_configuration(const char* field)
+{
+    return find_off(k_cfg_fields, sizeof(k_cfg_fields) / sizeof(k_cfg_fields[0]), field);
+}
+int64_t pfsolve_offsetof_launch_params(const char* field)
+{
+    return find_off(k_lp_fields, sizeof(k_lp_fields) / sizeof(k_lp_fields[0]), field);
+}
+
+PfSolveResult pfsolve_plan_geometry(const PfSolveApplication* app, pfsolve_geometry* g)
+{
+    if (!app || !app->localFFTPlan || !g) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    const struct pfsolve_plan* p = app->localFFTPlan;
+    memset(g, 0, sizeof(*g));
+    g->lanes = p->L;
+    g->rows_per_lane = p->R;
+    g->waves = p->waves;
+    g->precision = p->precision;
+    g->table_bytes = (int64_t)p->table_bytes;
+    g->lds_bytes = p->lds_bytes;
+    return PFSOLVE_SUCCESS;
+}
+
+/* reorder_i0: PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:66-142 */
+int64_t pfsolve_reorder_i0(int64_t i, int64_t N, int64_t Nalign, int64_t Msplit0)
+{
+    if (Msplit0 == 0) Msplit0 = 704;
+    int64_t size0 = N, size1 = Nalign ? Nalign : N;
+    if (N > 3840) { size0 = Msplit0; size1 = Msplit0; }
+    int64_t ws = ((size1 + 1023) / 1024) * 64;
+    if (size1 < 512) ws = 64;
+    int64_t R = (size1 + ws - 1) / ws;
+    if (R % 2 == 0) R++;
+    int64_t ret_off = (i / size0) * size0, li = i % size0;
+    if (((i / size0) + 1) * size0 > N) size0 = N % size0;
+    int64_t ceilq = (size0 + R - 1) / R, ret = li / R;
+    if ((li % R) < (size0 % R)) ret += (li % R) * ceilq;
+    else ret += (size0 % R) * ceilq + ((li % R) - (size0 % R)) * (size0 / R);
+    return ret + ret_off;
+}
+
+int pfsolve_kernel_supported(int lanes, int rows_per_lane, int precision)
+{
+    return jwdev_supported(lanes, rows_per_lane, precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD);
+}
+
+const char* pfsolve_last_error(void) { return jwdev_last_error(); }
+
+PfSolveResult pfsolve_set_kernel_events(PfSolveApplication* app, hipEvent_t start, hipEvent_t stop)
+{
+    if (!app || !app->localFFTPlan) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    app->localFFTPlan->ev_start = start;
+    app->localFFTPlan->ev_stop = stop;
+    return PFSOLVE_SUCCESS;
+}

@@ -1,0 +1,157 @@
+"""CPU tests of the C-ABI library: it loads, exports every symbol the headers
+declare, its struct layout matches the ctypes mirror, and plan validation,
+geometry, layout maths and error strings behave like the reference."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pfsolve_amd as pf
+from pfsolve_amd import _abi
+
+import _oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in ("pfSolve.h", "pfsolve_amd.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text, re.M):
+            if m.group(1) not in ("defined",):
+                names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pf.lib()
+    names = declared_functions()
+    assert "initializePfSolve" in names and "PfSolveAppend" in names and "deletePfSolve" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", pf.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(l.split()[-1] for l in out.splitlines() if " T " in l)
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        getattr(lib, n)
+
+
+def _fields(struct):
+    out = []
+    for name, _t in struct._fields_:
+        out.append((name, getattr(struct, name).offset))
+    return out
+
+
+def test_struct_layout_matches_c_header():
+    lib = pf.lib()
+    assert lib.pfsolve_sizeof_configuration() == C.sizeof(_abi.PfSolveConfiguration)
+    assert lib.pfsolve_sizeof_launch_params() == C.sizeof(_abi.PfSolveLaunchParams)
+    assert lib.pfsolve_sizeof_application() == C.sizeof(_abi.PfSolveApplication)
+    checked = 0
+    for name, off in _fields(_abi.PfSolveConfiguration):
+        c_off = lib.pfsolve_offsetof_configuration(name.encode())
+        if c_off >= 0:
+            assert c_off == off, name
+            checked += 1
+    assert checked >= 40
+    for name, off in _fields(_abi.PfSolveLaunchParams):
+        c_off = lib.pfsolve_offsetof_launch_params(name.encode())
+        if c_off >= 0:
+            assert c_off == off, name
+
+
+def test_version_and_error_strings_match_reference():
+    assert pf.PfSolveGetVersion() == 0o102 == 66
+    if not os.path.exists(O.REF_SO):
+        pytest.skip("oracle/_ref not built")
+    ref = C.CDLL(O.REF_SO)
+    ref.ref_error_string.argtypes = [C.c_int]
+    ref.ref_error_string.restype = C.c_char_p
+    ref.ref_version.restype = C.c_int
+    assert ref.ref_version() == pf.PfSolveGetVersion()
+    for code in list(range(0, 12)) + list(range(1000, 1013)) + list(range(2000, 2016)) + \
+            list(range(3000, 3007)) + list(range(4000, 4057)) + [9999]:
+        assert pf.getPfSolveErrorString(code) == ref.ref_error_string(code).decode(), code
+
+
+def test_reorder_i0_library_matches_oracle():
+    lib = pf.lib()
+    for N in [1, 63, 64, 65, 192, 193, 512, 777, 1024, 3840, 4000]:
+        for i in range(0, N, max(1, N // 97)):
+            assert lib.pfsolve_reorder_i0(i, N, N, 0) == O.lib().pfo_reorder_i0(i, N, N, 0)
+
+
+def cfg(**kw):
+    c = _abi.PfSolveConfiguration()
+    c.size[0] = kw.pop("N", 64)
+    c.size[1] = kw.pop("B", 1)
+    c.doublePrecision = kw.pop("doublePrecision", 1)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def validate(c):
+    g = _abi.PfSolveGeometry()
+    rc = pf.lib().pfsolve_validate_configuration(C.byref(c), C.byref(g))
+    return rc, g
+
+
+def test_validation_error_codes():
+    assert validate(cfg(N=0, jw_type=13, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_EMPTY_size
+    assert validate(cfg(jw_type=0))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX          # no plan kind
+    assert validate(cfg(jw_type=13, block=104))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(jw_type=113, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX  # ALT
+    assert validate(cfg(jw_type=10, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(jw_type=13, upperBanded=3))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(jw_type=13, upperBanded=2, doublePrecision=0))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(N=5000, jw_type=13, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(jw_type=14, KL=3, KU=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(LDA=4, KL=2, KU=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(block=108))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    for jt, ub in [(10, 0), (10, 1), (11, 0), (12, 1), (13, 0), (13, 1), (13, 2), (14, 0), (1013, 2), (1014, 0)]:
+        assert validate(cfg(jw_type=jt, upperBanded=ub))[0] == 0, (jt, ub)
+    assert validate(cfg(block=104))[0] == 0
+    assert validate(cfg(LDA=5, KL=2, KU=2))[0] == 0
+
+
+@pytest.mark.parametrize("N", [1, 8, 64, 65, 128, 512, 513, 1000, 1024])
+def test_geometry_covers_every_row(N):
+    rc, g = validate(cfg(N=N, jw_type=14))
+    assert rc == 0
+    assert g.lanes * g.rows_per_lane >= N
+    assert g.lanes * (g.rows_per_lane - 1) < N or g.rows_per_lane == 1
+    assert pf.lib().pfsolve_kernel_supported(g.lanes, g.rows_per_lane, 1)
+    assert g.lds_bytes <= 160 * 1024
+
+
+def test_dd_geometry_and_limits():
+    rc, g = validate(cfg(N=256, jw_type=14, doublePrecision=0, quadDoubleDoublePrecision=1))
+    assert rc == 0 and g.precision == 2 and g.lanes * g.rows_per_lane >= 256
+    rc, _ = validate(cfg(N=1024, jw_type=14, doublePrecision=0, quadDoubleDoublePrecision=1))
+    assert rc == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+
+
+def test_initialize_requires_device_pointer_and_cleans_up():
+    app = _abi.PfSolveApplication()
+    c = cfg(jw_type=13, upperBanded=2)
+    assert pf.initializePfSolve(app, c) == _abi.PFSOLVE_ERROR_INVALID_DEVICE
+    assert app.localFFTPlan is None
+    # invalid configuration with a device pointer: validation fails before any device call
+    dev = C.c_int(0)
+    c = cfg(N=0, jw_type=13, upperBanded=2)
+    c.device = C.pointer(dev)
+    assert pf.initializePfSolve(app, c) == _abi.PFSOLVE_ERROR_EMPTY_size
+    assert app.localFFTPlan is None
+    pf.deletePfSolve(app)           # idempotent
+
+
+def test_append_without_plan():
+    app = _abi.PfSolveApplication()
+    assert pf.PfSolveAppend(app, 0, _abi.PfSolveLaunchParams()) == _abi.PFSOLVE_ERROR_PLAN_NOT_INITIALIZED
+    assert pf.PfSolveSync(app) == 0

@@ -1,0 +1,342 @@
+"""GPU parity of the HIP path (libpfsolve_gfx950.so through the C ABI) against
+the CPU oracle.
+
+Bar: FP64 max normwise relative error <= 1e-13 against the __float128 oracle
+(max_i |y_i - y*_i| / max_i |y*_i| per system); double-double <= 1e-30;
+rows outside the output keep-range and the gaps between systems are
+bit-exactly untouched; layout-only block plans are bit-exact.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pfsolve_amd as pf
+from pfsolve_amd import _abi
+
+import _oracle as O
+from jwgpu import run_plan
+
+pytestmark = pytest.mark.gpu
+
+RT_ALL = (_abi.RUNTIME_OFFSETM | _abi.RUNTIME_OFFSETV | _abi.RUNTIME_OFFSETSOLUTION | _abi.RUNTIME_INPUTZEROPAD
+          | _abi.RUNTIME_OUTPUTZEROPAD | _abi.RUNTIME_SCALEC | _abi.RUNTIME_OUTPUTBUFFERSTRIDE)
+TOL64 = 1e-13
+
+
+def md_base(jw, ub, N, rt, offM):
+    if ub == 2:
+        return N
+    if jw in (0, 1):
+        return (offM + N if rt else 0) if ub else (offM if rt else N)
+    if jw == 3:
+        return (offM if rt else N) if ub else (offM + N if rt else 0)
+    return None
+
+
+def mode_buffer(jw, ub, N, rt, offM, total, seed):
+    buf = O.uniform(seed, -0.3, 0.3, total)
+    mb = md_base(jw, ub, N, rt, offM)
+    if mb is not None:
+        buf[mb:mb + N] = O.uniform(seed + 1, 0.5, 1.5, N)
+    return buf
+
+
+def check_systems(got, want, x, nsys, stride, off, N, out_lo, out_hi, tol):
+    for b in range(nsys):
+        base = b * stride + off
+        g = got[base:base + N]
+        w = want[base:base + N]
+        if out_hi > out_lo:
+            err = O.relerr_norm(g[out_lo:out_hi], w[out_lo:out_hi])
+            assert err <= tol, (b, err)
+        np.testing.assert_array_equal(g[:out_lo], x[base:base + out_lo])
+        np.testing.assert_array_equal(g[out_hi:], x[base + out_hi:base + N])
+    # everything outside the systems is untouched
+    mask = np.ones(got.size, bool)
+    for b in range(nsys):
+        mask[b * stride + off:b * stride + off + N] = False
+    np.testing.assert_array_equal(got[mask], x[mask])
+
+
+MODES = [(10, 1), (10, 0), (11, 1), (11, 0), (12, 1), (12, 0), (13, 2), (13, 1), (13, 0)]
+SIZES = [1, 2, 3, 17, 64, 65, 127, 200, 512, 513, 1000, 1024]
+
+
+@pytest.mark.parametrize("jw_type,ub", MODES)
+@pytest.mark.parametrize("N", SIZES)
+@pytest.mark.parametrize("rt", [0, 1])
+def test_reference_modes(jw_type, ub, N, rt):
+    jw = jw_type % 10
+    B = 5
+    offM, offV = (13, 3 * N + 29) if rt else (0, 0)
+    total = max(3 * N, offV + N, offM + 2 * N) + 8
+    buf = mode_buffer(jw, ub, N, rt, offM, total, 1000 + N + jw)
+    S = N + 7 if rt else N
+    offS = 5 if rt else 0
+    izp = (N // 4, N) if rt else (0, 0)
+    ozp = (1, N - N // 8) if rt else (0, 0)
+    x = O.uniform(77 + N, -1, 1, B * S + offS + 9)
+    cfg = dict(size=[N, B, 1], jw_type=jw_type, upperBanded=ub, doublePrecision=1, scaleC=1.0,
+               outputBufferStride=[N, 0, 0], jw_control_bitmask=RT_ALL if rt else 0)
+    launch = dict(offsetM=offM, offsetV=offV, offsetSolution=offS, inputZeropad=izp, outputZeropad=ozp,
+                  scaleC=0.75, outputBufferStride=S) if rt else {}
+    got = run_plan(cfg, buf, x, launch)
+    p = O.problem(jw_type=jw_type, upperBanded=ub, N=N, B=B, mask=RT_ALL if rt else 0, cfg_outStride=N,
+                  cfg_scaleC=1.0, run_offsetM=offM, run_offsetV=offV, run_offsetSolution=offS,
+                  run_izp=list(izp), run_ozp=list(ozp), run_scaleC=0.75, run_outStride=S)
+    want = O.jw_f128(p, buf, x)
+    lo, hi = (ozp[0], min(ozp[1], N)) if rt else (0, N)
+    check_systems(got, want, x, B, S, offS, N, lo, max(lo, hi), TOL64)
+
+
+def fused_buffers(N, seed, permuted, pad=0):
+    offV, offT, offM = pad, pad + 5 * N + 3, pad + 8 * N + 11
+    raw = O.raw_matrices(seed, N)
+    buf = O.pack_fused(N, raw, permuted, offV, offT, offM, offM + 2 * N + 4)
+    return raw, buf, (offV, offT, offM)
+
+
+@pytest.mark.parametrize("N", [1, 2, 5, 16, 63, 64, 65, 100, 256, 511, 512, 777, 1024])
+@pytest.mark.parametrize("permuted", [1, 0])
+def test_fused_step(N, permuted):
+    B = 7
+    raw, buf, (offV, offT, offM) = fused_buffers(N, N, permuted)
+    x = O.uniform(5 + N, -1, 1, B * N)
+    jt = 14 if permuted else 1014
+    cfg = dict(size=[N, B, 1], jw_type=jt, doublePrecision=1, scaleC=1.0, KL=2, KU=2,
+               offsetV=offV, offsetT=offT, offsetM=offM)
+    got = run_plan(cfg, buf, x)
+    p = O.problem(jw_type=jt, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    want = O.jw_f128(p, buf, x)
+    check_systems(got, want, x, B, N, 0, N, 0, N, TOL64)
+    # and against the raw-matrix __float128 evaluation (independent of storage)
+    for b in (0, B - 1):
+        y = np.zeros(N)
+        O.lib().pfo_fused_raw_f128(N, 2, 2, *[O.P(a) for a in raw], O.P(np.ascontiguousarray(x[b * N:(b + 1) * N])),
+                                   O.P(y))
+        assert O.relerr_norm(got[b * N:(b + 1) * N], y) <= TOL64
+
+
+def test_fused_config0_exact_workload():
+    """N=64, 1024 real systems, seed 0, V/T/M/x from the stated distributions."""
+    N, B = 64, 1024
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 0, 1)
+    x = O.uniform(0, -1, 1, B * N)
+    cfg = dict(size=[N, B, 1], jw_type=14, doublePrecision=1, scaleC=1.0, offsetV=offV, offsetT=offT, offsetM=offM)
+    got = run_plan(cfg, buf, x)
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    want = O.jw_f128(p, buf, x)
+    want64 = O.jw_f64(p, buf, x)
+    check_systems(got, want, x, B, N, 0, N, 0, N, TOL64)
+    assert O.relerr_norm(got, want64) <= TOL64
+
+
+def cfg1_like(nsys_complex, N=512):
+    """complex XXII (2 real systems per complex one), runtime scaleC 0.75 and input keep-range [480,512)"""
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 1, 1)
+    B = 2 * nsys_complex
+    mask = _abi.RUNTIME_SCALEC | _abi.RUNTIME_INPUTZEROPAD
+    cfg = dict(size=[N, B, 1], jw_type=14, doublePrecision=1, scaleC=1.0, offsetV=offV, offsetT=offT,
+               offsetM=offM, jw_control_bitmask=mask)
+    launch = dict(scaleC=0.75, inputZeropad=[480, 512])
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, mask=mask, cfg_outStride=N, cfg_scaleC=1.0,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM, run_scaleC=0.75, run_izp=[480, 512])
+    return raw, buf, cfg, launch, p
+
+
+def test_fused_config1_reduced_batch():
+    raw, buf, cfg, launch, p = cfg1_like(256)
+    N, B = 512, 512
+    x = O.uniform(3, -1, 1, B * N)
+    got = run_plan(cfg, buf, x, launch)
+    want = O.jw_f128(p, buf, x)
+    check_systems(got, want, x, B, N, 0, N, 0, N, TOL64)
+
+
+def test_fused_config1_full_size_properties():
+    """65536 complex systems (131072 real) at N=512: linearity in x, re/im independence,
+    and a sample of systems against the oracle."""
+    import torch
+    raw, buf, cfg, launch, p = cfg1_like(65536)
+    N, B = 512, 131072
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x1 = torch.rand(B * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    x2 = torch.rand(B * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    dmat = torch.from_numpy(buf).cuda()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(x):
+        y = x.clone()
+        with pf.Plan(dmat.data_ptr(), y.data_ptr(), stream=stream, **cfg) as plan:
+            plan.append(**launch)
+        torch.cuda.synchronize()
+        return y
+
+    y1, y2 = step(x1), step(x2)
+    y12 = step(0.5 * x1 - 2.0 * x2)
+    lin = (y12 - (0.5 * y1 - 2.0 * y2)).abs().max().item() / y12.abs().max().item()
+    assert lin < 1e-13
+    xs = x1.cpu().numpy()
+    ys = y1.cpu().numpy()
+    rng = np.random.default_rng(0)
+    sample = np.unique(np.r_[0, 1, B - 2, B - 1, rng.integers(0, B, 60)])
+    for b in sample:
+        xb = xs[b * N:(b + 1) * N].copy()
+        pb = O.problem(**{f: getattr(p, f) for f, _ in O.JwProblem._fields_ if f not in ("B",)}, B=1)
+        want = O.jw_f128(pb, buf, xb)
+        assert O.relerr_norm(ys[b * N:(b + 1) * N], want) <= TOL64, b
+
+
+def test_fused_runtime_offsets_strides_and_buffers_swap():
+    N, B = 300, 9
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 4, 1, pad=17)
+    S, offS = N + 64, 12
+    x = O.uniform(9, -1, 1, B * S + offS)
+    mask = RT_ALL | _abi.RUNTIME_OFFSETT
+    cfg = dict(size=[N, 3, 1], jw_type=14, doublePrecision=1, jw_control_bitmask=mask)
+    launch = dict(offsetV=offV, offsetT=offT, offsetM=offM, offsetSolution=offS, outputBufferStride=S,
+                  inputZeropad=[0, N], outputZeropad=[2, N - 3], scaleC=-1.5, batchSize=B)
+    got = run_plan(cfg, buf, x, launch)
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, mask=mask, run_offsetV=offV, run_offsetT=offT,
+                  run_offsetM=offM, run_offsetSolution=offS, run_outStride=S, run_izp=[0, N],
+                  run_ozp=[2, N - 3], run_scaleC=-1.5)
+    want = O.jw_f128(p, buf, x)
+    check_systems(got, want, x, B, S, offS, N, 2, N - 3, TOL64)
+
+
+def test_odd_and_tiny_batches():
+    N = 70
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 2, 1)
+    for B in (1, 2, 3, 131):
+        x = O.uniform(B, -1, 1, B * N)
+        cfg = dict(size=[N, B, 1], jw_type=14, doublePrecision=1, scaleC=1.0, offsetV=offV, offsetT=offT,
+                   offsetM=offM)
+        got = run_plan(cfg, buf, x)
+        p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                      cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+        check_systems(got, O.jw_f128(p, buf, x), x, B, N, 0, N, 0, N, TOL64)
+
+
+def test_size_guard_turns_bad_offsets_into_errors():
+    N, B = 100, 4
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 2, 1)
+    x = O.uniform(1, -1, 1, B * N)
+    cfg = dict(size=[N, B, 1], jw_type=14, doublePrecision=1, offsetV=offV, offsetT=offT, offsetM=offM,
+               jw_control_bitmask=_abi.RUNTIME_OFFSETSOLUTION)
+    code = run_plan(cfg, buf, x, dict(offsetSolution=5), expect=1)
+    assert code == _abi.PFSOLVE_ERROR_EMPTY_outputBufferSize
+    cfg2 = dict(cfg, offsetM=buf.size)   # M beyond the matrix buffer
+    cfg2["jw_control_bitmask"] = 0
+    assert run_plan(cfg2, buf, x, {}, expect=1) == _abi.PFSOLVE_ERROR_EMPTY_bufferSize
+
+
+# ---------------------------------------------------------------- double-double
+def dd_inputs(N, B, seed):
+    raw = O.raw_matrices(seed, N)
+    rawdd = []
+    for k, a in enumerate(raw):
+        lo = O.uniform(seed * 7 + k, -1, 1, a.size) * 2.0 ** -60 * np.abs(a)
+        rawdd.append(np.ascontiguousarray(np.stack([a, lo], -1)))
+    offV, offT, offM = 0, 5 * N, 8 * N
+    buf = np.zeros((10 * N, 2))
+    O.lib().pfo_pack_fused_dd(N, 2, 2, *[O.PDD(a) for a in rawdd], 1, 0, offV, offT, offM, O.PDD(buf))
+    xh = O.uniform(seed + 1, -1, 1, B * N)
+    x = np.ascontiguousarray(np.stack([xh, O.uniform(seed + 2, -1, 1, B * N) * 2.0 ** -60 * np.abs(xh)], -1))
+    return buf, x, (offV, offT, offM)
+
+
+@pytest.mark.parametrize("N", [1, 7, 64, 100, 256, 512])
+def test_double_double_fused(N):
+    B = 64 if N == 256 else 6
+    buf, x, (offV, offT, offM) = dd_inputs(N, B, 31 + N)
+    cfg = dict(size=[N, B, 1], jw_type=14, quadDoubleDoublePrecision=1, scaleC=1.0,
+               offsetV=offV, offsetT=offT, offsetM=offM)
+    got = run_plan(cfg, buf.ravel(), x.ravel()).reshape(-1, 2)
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    want = O.jw_f128_dd(p, buf, x)
+    for b in range(B):
+        err = O.dd_relerr_norm(got[b * N:(b + 1) * N], want[b * N:(b + 1) * N])
+        assert err <= 1e-30, (b, err)
+
+
+def test_double_double_reference_mode_tridiagonal():
+    N, B = 300, 4
+    buf, x, _ = dd_inputs(N, B, 5)
+    # jw 13 / ub 2 reads [ld | md | ud] at [0, N, 2N): reuse the T block of the fused packing
+    tri = np.ascontiguousarray(buf[5 * N:8 * N])
+    cfg = dict(size=[N, B, 1], jw_type=13, upperBanded=2, quadDoubleDoublePrecision=1, scaleC=1.0)
+    got = run_plan(cfg, tri.ravel(), x.ravel()).reshape(-1, 2)
+    p = O.problem(jw_type=13, upperBanded=2, N=N, B=B, cfg_outStride=N, cfg_scaleC=1.0)
+    want = O.jw_f128_dd(p, tri, x)
+    assert O.dd_relerr_norm(got, want) <= 1e-30
+
+
+def test_double_double_compute_fp64_memory():
+    N, B = 200, 8
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 6, 1)
+    x = O.uniform(6, -1, 1, B * N)
+    cfg = dict(size=[N, B, 1], jw_type=14, quadDoubleDoublePrecisionDoubleMemory=1, scaleC=1.0,
+               offsetV=offV, offsetT=offT, offsetM=offM)
+    got = run_plan(cfg, buf, x)
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    want = O.jw_f128(p, buf, x)
+    # DD arithmetic, result rounded to FP64: within one rounding of the exact result
+    assert O.relerr_norm(got, want) <= 2.3e-16
+
+
+# ---------------------------------------------------------------- block plan
+@pytest.mark.parametrize("layout", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("acc", [0, 2])
+def test_block_plan(layout, op, acc):
+    N, B = 37, 6
+    block = op * 100 + acc * 10 + layout
+    in_c = layout in (2, 3, 4)
+    out_c = layout in (5, 6, 7)
+    inS = N + 3
+    outS = N + 5
+    offIn, offOut = 2, 4
+    n_in = (2 if (in_c or layout == 7) else 1) * (B * inS + offIn + 2 * inS)
+    n_out = (2 if (out_c or layout == 4) else 1) * (B * outS + offOut + 2 * outS)
+    inb = O.uniform(1, -1, 1, n_in)
+    outb = O.uniform(2, -1, 1, n_out)
+    mask = (_abi.RUNTIME_SCALEC | _abi.RUNTIME_INPUTBUFFERSTRIDE | _abi.RUNTIME_OUTPUTBUFFERSTRIDE
+            | _abi.RUNTIME_OFFSETM | _abi.RUNTIME_OFFSETSOLUTION | _abi.RUNTIME_INPUTZEROPAD | _abi.RUNTIME_OUTPUTZEROPAD)
+    cfg = dict(size=[N, B, 1], block=block, lshift=3, doublePrecision=1, jw_control_bitmask=mask)
+    launch = dict(scaleC=0.75, inputBufferStride=inS, outputBufferStride=outS, offsetM=offIn,
+                  offsetSolution=offOut, inputZeropad=[3, N - 2], outputZeropad=[1, N])
+    got = run_plan(cfg, inb, outb, launch)
+    bp = O.BlockProblem(block=block, lshift=3, N=N, B=B, mask=mask, offsetM=offIn, offsetSolution=offOut,
+                        izp_runtime=1, ozp_runtime=1, scaleC=0.75, scaleC_runtime=1, inStride=inS, outStride=outS)
+    bp.izp[0], bp.izp[1] = 3, N - 2
+    bp.ozp[0], bp.ozp[1] = 1, N
+    want = outb.copy()
+    assert O.lib().pfo_block_f64(C.byref(bp), O.P(inb), O.P(want)) == 0
+    if op == 0:
+        np.testing.assert_array_equal(got, want)
+    else:
+        np.testing.assert_allclose(got, want, rtol=4e-16, atol=0)
+
+
+# ---------------------------------------------------------------- dgbmv plan
+@pytest.mark.parametrize("KL,KU", [(2, 2), (1, 0), (0, 3), (3, 1), (0, 0)])
+@pytest.mark.parametrize("N", [1, 7, 100, 1000])
+def test_dgbmv_plan(KL, KU, N):
+    B = 5
+    LDA = KL + KU + 1
+    offM = 3
+    mat = O.uniform(N + KL, -1, 1, offM + LDA * N)
+    S = N + 2
+    x = O.uniform(4, -1, 1, B * S)
+    mask = _abi.RUNTIME_OUTPUTBUFFERSTRIDE
+    cfg = dict(size=[N, B, 1], LDA=LDA, KL=KL, KU=KU, doublePrecision=1, offsetM=offM, jw_control_bitmask=mask)
+    got = run_plan(cfg, mat, x, dict(outputBufferStride=S))
+    p = O.problem(N=N, B=B, mask=mask, cfg_offsetM=offM, run_outStride=S, cfg_outStride=N)
+    want = x.copy()
+    assert O.lib().pfo_dgbmv_f128(C.byref(p), LDA, KU, KL, O.P(mat), O.P(want)) == 0
+    check_systems(got, want, x, B, S, 0, N, 0, N, 1e-14)
