@@ -214,6 +214,7 @@ typedef struct {
     /* ---- extension fields (no reference counterpart), appended ---- */
     int64_t offsetT;             /* fused step (jw_type 14): tridiagonal block T */
     int64_t autotune;            /* 0: built-in table, 1: run the autotuner at plan time */
+    int64_t kernelVariant;       /* 0: automatic, 1: general kernel, 2: wave-per-system FP64 kernel */
 } PfSolveConfiguration;
 
 typedef struct {

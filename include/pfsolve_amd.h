@@ -41,6 +41,8 @@ typedef struct pfsolve_geometry {
     int32_t precision;        /* 1 FP64, 2 double-double, 3 double-double with FP64 memory */
     int64_t table_bytes;      /* coefficient table per matrix set */
     int64_t lds_bytes;        /* LDS per workgroup */
+    int32_t fast_rows_per_lane;   /* wave-per-system FP64 kernel: R (0 = not used) */
+    int32_t fast_waves;           /* ... and its waves per workgroup */
 } pfsolve_geometry;
 
 /* Validates a configuration as initializePfSolve does, without touching the

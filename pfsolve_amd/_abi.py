@@ -45,17 +45,16 @@ JWT_DIAGONAL_MATVECMUL = 3
 JWT_FUSED_BAND_TRI_BIDIAG = 4
 JWT_USE_PARALLEL_THOMAS = 10
 
-PFSOLVE_SUCCESS = 0
-PFSOLVE_ERROR_MALLOC_FAILED = 1
-PFSOLVE_ERROR_PLAN_NOT_INITIALIZED = 4
-PFSOLVE_ERROR_INVALID_DEVICE = 1002
-PFSOLVE_ERROR_EMPTY_size = 2002
-PFSOLVE_ERROR_EMPTY_buffer = 2004
-PFSOLVE_ERROR_EMPTY_outputBuffer = 2010
-PFSOLVE_ERROR_UNSUPPORTED_RADIX = 3001
-PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH = 3002
-PFSOLVE_ERROR_FAILED_TO_ALLOCATE = 4001
-PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL = 4039
+def _result_codes() -> dict:
+    """PfSolveResult enumerators, read from include/pfSolve.h."""
+    import os
+    import re
+    hdr = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "pfSolve.h")
+    text = open(hdr).read()
+    return {k: int(v) for k, v in re.findall(r"\b(PFSOLVE_(?:SUCCESS|ERROR_\w+))\s*=\s*(\d+)", text)}
+
+
+globals().update(_result_codes())
 
 
 class PfSolveConfiguration(C.Structure):
@@ -175,6 +174,7 @@ class PfSolveConfiguration(C.Structure):
         ("useStrict32BitAddress", i64),
         ("offsetT", i64),
         ("autotune", i64),
+        ("kernelVariant", i64),
     ]
 
 
@@ -222,6 +222,8 @@ class PfSolveGeometry(C.Structure):
         ("precision", C.c_int32),
         ("table_bytes", i64),
         ("lds_bytes", i64),
+        ("fast_rows_per_lane", C.c_int32),
+        ("fast_waves", C.c_int32),
     ]
 
 

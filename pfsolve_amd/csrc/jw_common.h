@@ -1,0 +1,211 @@
+// jw_common.h -- device helpers shared by the gfx950 JW kernels: double-double
+// arithmetic, arithmetic traits (one kernel source for FP64 and DD), cross-lane
+// moves (ds_bpermute, DPP row/wave shifts, readlane), caller-memory access and
+// the reference's storage permutation.
+#ifndef PFSOLVE_JW_COMMON_H
+#define PFSOLVE_JW_COMMON_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jw_internal.h"
+
+// ------------------------------------------------------------------------
+// double-double arithmetic (Dekker 1971; Knuth TwoSum; Joldes, Muller,
+// Popescu, ACM TOMS 44(2) 2017, Alg. 6 (add), 12 (mul), 17 (div)).
+// This file is compiled with -ffp-contract=off so that the error-free
+// transforms stay exact; FMAs are written explicitly.
+// ------------------------------------------------------------------------
+struct dd {
+    double hi, lo;
+};
+
+__device__ __forceinline__ dd dd_make(double h, double l) { dd r; r.hi = h; r.lo = l; return r; }
+
+__device__ __forceinline__ dd two_sum(double a, double b)
+{
+    double s = a + b;
+    double bb = s - a;
+    return dd_make(s, (a - (s - bb)) + (b - bb));
+}
+
+__device__ __forceinline__ dd fast_two_sum(double a, double b)
+{
+    double s = a + b;
+    return dd_make(s, b - (s - a));
+}
+
+__device__ __forceinline__ dd two_prod(double a, double b)
+{
+    double p = a * b;
+    return dd_make(p, __builtin_fma(a, b, -p));
+}
+
+__device__ __forceinline__ dd dd_add(dd x, dd y)
+{
+    dd s = two_sum(x.hi, y.hi);
+    dd t = two_sum(x.lo, y.lo);
+    double c = s.lo + t.hi;
+    dd v = fast_two_sum(s.hi, c);
+    double w = t.lo + v.lo;
+    return fast_two_sum(v.hi, w);
+}
+
+__device__ __forceinline__ dd dd_neg(dd x) { return dd_make(-x.hi, -x.lo); }
+__device__ __forceinline__ dd dd_sub(dd x, dd y) { return dd_add(x, dd_neg(y)); }
+
+__device__ __forceinline__ dd dd_mul(dd x, dd y)
+{
+    dd c = two_prod(x.hi, y.hi);
+    double tl0 = x.lo * y.lo;
+    double tl1 = __builtin_fma(x.hi, y.lo, tl0);
+    double cl2 = __builtin_fma(x.lo, y.hi, tl1);
+    double cl3 = c.lo + cl2;
+    return fast_two_sum(c.hi, cl3);
+}
+
+__device__ __forceinline__ dd dd_div(dd x, dd y)
+{
+    double th = x.hi / y.hi;
+    dd c = two_prod(y.hi, th);
+    double cl = __builtin_fma(y.lo, th, c.lo);
+    dd r = fast_two_sum(c.hi, cl);
+    double pih = x.hi - r.hi;
+    double dl = x.lo - r.lo;
+    double d = pih + dl;
+    double tl = d / y.hi;
+    return fast_two_sum(th, tl);
+}
+
+// ------------------------------------------------------------------------
+// cross-lane moves (ds_bpermute: any lane of the wavefront, 32 bits per op)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double lane_fetch(double v, int src_lane)
+{
+    int lo = __double2loint(v);
+    int hi = __double2hiint(v);
+    lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, lo);
+    hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, hi);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ dd lane_fetch(dd v, int src_lane)
+{
+    return dd_make(lane_fetch(v.hi, src_lane), lane_fetch(v.lo, src_lane));
+}
+
+// ------------------------------------------------------------------------
+// arithmetic traits: one kernel source for FP64 and double-double
+// ------------------------------------------------------------------------
+template <typename T> struct Ar;
+
+template <> struct Ar<double> {
+    __device__ static __forceinline__ double zero() { return 0.0; }
+    __device__ static __forceinline__ double one() { return 1.0; }
+    __device__ static __forceinline__ double from(double h, double) { return h; }
+    __device__ static __forceinline__ double add(double a, double b) { return a + b; }
+    __device__ static __forceinline__ double sub(double a, double b) { return a - b; }
+    __device__ static __forceinline__ double mul(double a, double b) { return a * b; }
+    __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+    __device__ static __forceinline__ double neg(double a) { return -a; }
+    // a + b*c and a - b*c as single roundings
+    __device__ static __forceinline__ double madd(double a, double b, double c) { return __builtin_fma(b, c, a); }
+    __device__ static __forceinline__ double msub(double a, double b, double c) { return __builtin_fma(-b, c, a); }
+    __device__ static __forceinline__ double hi(double a) { return a; }
+    __device__ static __forceinline__ double ldexp_(double a, int e) { return ldexp(a, e); }
+    __device__ static __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
+};
+
+template <> struct Ar<dd> {
+    __device__ static __forceinline__ dd zero() { return dd_make(0.0, 0.0); }
+    __device__ static __forceinline__ dd one() { return dd_make(1.0, 0.0); }
+    __device__ static __forceinline__ dd from(double h, double l) { return dd_make(h, l); }
+    __device__ static __forceinline__ dd add(dd a, dd b) { return dd_add(a, b); }
+    __device__ static __forceinline__ dd sub(dd a, dd b) { return dd_sub(a, b); }
+    __device__ static __forceinline__ dd mul(dd a, dd b) { return dd_mul(a, b); }
+    __device__ static __forceinline__ dd div(dd a, dd b) { return dd_div(a, b); }
+    __device__ static __forceinline__ dd neg(dd a) { return dd_neg(a); }
+    __device__ static __forceinline__ dd madd(dd a, dd b, dd c) { return dd_add(a, dd_mul(b, c)); }
+    __device__ static __forceinline__ dd msub(dd a, dd b, dd c) { return dd_sub(a, dd_mul(b, c)); }
+    __device__ static __forceinline__ double hi(dd a) { return a.hi; }
+    __device__ static __forceinline__ dd ldexp_(dd a, int e) { return dd_make(ldexp(a.hi, e), ldexp(a.lo, e)); }
+    __device__ static __forceinline__ dd sel(bool c, dd a, dd b) { return c ? a : b; }
+};
+
+// element access of caller memory: T storage, or FP64 storage promoted to T
+template <typename T> __device__ __forceinline__ T mem_load(const void* base, int64_t idx, int mem_f64)
+{
+    if (mem_f64) return Ar<T>::from(reinterpret_cast<const double*>(base)[idx], 0.0);
+    return reinterpret_cast<const T*>(base)[idx];
+}
+template <> __device__ __forceinline__ double mem_load<double>(const void* base, int64_t idx, int)
+{
+    return reinterpret_cast<const double*>(base)[idx];
+}
+template <typename T> __device__ __forceinline__ void mem_store(void* base, int64_t idx, T v, int mem_f64)
+{
+    if (mem_f64) {
+        // round DD to FP64 storage
+        reinterpret_cast<double*>(base)[idx] = Ar<T>::hi(Ar<T>::add(v, Ar<T>::zero()));
+        return;
+    }
+    reinterpret_cast<T*>(base)[idx] = v;
+}
+template <> __device__ __forceinline__ void mem_store<double>(void* base, int64_t idx, double v, int)
+{
+    reinterpret_cast<double*>(base)[idx] = v;
+}
+
+constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v / 2); }
+
+// reorder_i0 (storage position of logical row i), single-upload case N <= 3840:
+// PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:66-142
+__device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N)
+{
+    int64_t ws = ((N + 1023) / 1024) * 64;
+    if (N < 512) ws = 64;
+    int64_t R = (N + ws - 1) / ws;
+    if ((R & 1) == 0) R++;
+    int64_t ceilq = (N + R - 1) / R;
+    int64_t k = i % R;
+    int64_t rem = N % R;
+    int64_t ret = i / R;
+    if (k < rem)
+        ret += k * ceilq;
+    else
+        ret += rem * ceilq + (k - rem) * (N / R);
+    return ret;
+}
+
+
+// ------------------------------------------------------------------------
+// DPP moves of a double (two 32-bit v_mov_b32_dpp); lanes without a source
+// receive 0 (bound_ctrl).  CTRL: row_shl:n = 0x100+n, row_shr:n = 0x110+n,
+// wave_shl:1 = 0x130, wave_shr:1 = 0x138.
+// ------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, true);
+    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+constexpr int DPP_ROW_SHL(int n) { return 0x100 + n; }
+constexpr int DPP_ROW_SHR(int n) { return 0x110 + n; }
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+
+// coefficient table of format 2 (64 lanes): pair p of row-in-lane i, lane t
+__host__ __device__ __forceinline__ int tab_coef64(int p, int i, int t, int R)
+{
+    return ((p * R + i) * 64 + t) * 2;
+}
+
+#endif

@@ -30,174 +30,7 @@
 #include <stdint.h>
 #include <stdio.h>
 
-#include "jw_internal.h"
-
-// ------------------------------------------------------------------------
-// double-double arithmetic (Dekker 1971; Knuth TwoSum; Joldes, Muller,
-// Popescu, ACM TOMS 44(2) 2017, Alg. 6 (add), 12 (mul), 17 (div)).
-// This file is compiled with -ffp-contract=off so that the error-free
-// transforms stay exact; FMAs are written explicitly.
-// ------------------------------------------------------------------------
-struct dd {
-    double hi, lo;
-};
-
-__device__ __forceinline__ dd dd_make(double h, double l) { dd r; r.hi = h; r.lo = l; return r; }
-
-__device__ __forceinline__ dd two_sum(double a, double b)
-{
-    double s = a + b;
-    double bb = s - a;
-    return dd_make(s, (a - (s - bb)) + (b - bb));
-}
-
-__device__ __forceinline__ dd fast_two_sum(double a, double b)
-{
-    double s = a + b;
-    return dd_make(s, b - (s - a));
-}
-
-__device__ __forceinline__ dd two_prod(double a, double b)
-{
-    double p = a * b;
-    return dd_make(p, __builtin_fma(a, b, -p));
-}
-
-__device__ __forceinline__ dd dd_add(dd x, dd y)
-{
-    dd s = two_sum(x.hi, y.hi);
-    dd t = two_sum(x.lo, y.lo);
-    double c = s.lo + t.hi;
-    dd v = fast_two_sum(s.hi, c);
-    double w = t.lo + v.lo;
-    return fast_two_sum(v.hi, w);
-}
-
-__device__ __forceinline__ dd dd_neg(dd x) { return dd_make(-x.hi, -x.lo); }
-__device__ __forceinline__ dd dd_sub(dd x, dd y) { return dd_add(x, dd_neg(y)); }
-
-__device__ __forceinline__ dd dd_mul(dd x, dd y)
-{
-    dd c = two_prod(x.hi, y.hi);
-    double tl0 = x.lo * y.lo;
-    double tl1 = __builtin_fma(x.hi, y.lo, tl0);
-    double cl2 = __builtin_fma(x.lo, y.hi, tl1);
-    double cl3 = c.lo + cl2;
-    return fast_two_sum(c.hi, cl3);
-}
-
-__device__ __forceinline__ dd dd_div(dd x, dd y)
-{
-    double th = x.hi / y.hi;
-    dd c = two_prod(y.hi, th);
-    double cl = __builtin_fma(y.lo, th, c.lo);
-    dd r = fast_two_sum(c.hi, cl);
-    double pih = x.hi - r.hi;
-    double dl = x.lo - r.lo;
-    double d = pih + dl;
-    double tl = d / y.hi;
-    return fast_two_sum(th, tl);
-}
-
-// ------------------------------------------------------------------------
-// cross-lane moves (ds_bpermute: any lane of the wavefront, 32 bits per op)
-// ------------------------------------------------------------------------
-__device__ __forceinline__ double lane_fetch(double v, int src_lane)
-{
-    int lo = __double2loint(v);
-    int hi = __double2hiint(v);
-    lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, lo);
-    hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, hi);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ dd lane_fetch(dd v, int src_lane)
-{
-    return dd_make(lane_fetch(v.hi, src_lane), lane_fetch(v.lo, src_lane));
-}
-
-// ------------------------------------------------------------------------
-// arithmetic traits: one kernel source for FP64 and double-double
-// ------------------------------------------------------------------------
-template <typename T> struct Ar;
-
-template <> struct Ar<double> {
-    __device__ static __forceinline__ double zero() { return 0.0; }
-    __device__ static __forceinline__ double one() { return 1.0; }
-    __device__ static __forceinline__ double from(double h, double) { return h; }
-    __device__ static __forceinline__ double add(double a, double b) { return a + b; }
-    __device__ static __forceinline__ double sub(double a, double b) { return a - b; }
-    __device__ static __forceinline__ double mul(double a, double b) { return a * b; }
-    __device__ static __forceinline__ double div(double a, double b) { return a / b; }
-    __device__ static __forceinline__ double neg(double a) { return -a; }
-    // a + b*c and a - b*c as single roundings
-    __device__ static __forceinline__ double madd(double a, double b, double c) { return __builtin_fma(b, c, a); }
-    __device__ static __forceinline__ double msub(double a, double b, double c) { return __builtin_fma(-b, c, a); }
-    __device__ static __forceinline__ double hi(double a) { return a; }
-    __device__ static __forceinline__ double ldexp_(double a, int e) { return ldexp(a, e); }
-    __device__ static __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
-};
-
-template <> struct Ar<dd> {
-    __device__ static __forceinline__ dd zero() { return dd_make(0.0, 0.0); }
-    __device__ static __forceinline__ dd one() { return dd_make(1.0, 0.0); }
-    __device__ static __forceinline__ dd from(double h, double l) { return dd_make(h, l); }
-    __device__ static __forceinline__ dd add(dd a, dd b) { return dd_add(a, b); }
-    __device__ static __forceinline__ dd sub(dd a, dd b) { return dd_sub(a, b); }
-    __device__ static __forceinline__ dd mul(dd a, dd b) { return dd_mul(a, b); }
-    __device__ static __forceinline__ dd div(dd a, dd b) { return dd_div(a, b); }
-    __device__ static __forceinline__ dd neg(dd a) { return dd_neg(a); }
-    __device__ static __forceinline__ dd madd(dd a, dd b, dd c) { return dd_add(a, dd_mul(b, c)); }
-    __device__ static __forceinline__ dd msub(dd a, dd b, dd c) { return dd_sub(a, dd_mul(b, c)); }
-    __device__ static __forceinline__ double hi(dd a) { return a.hi; }
-    __device__ static __forceinline__ dd ldexp_(dd a, int e) { return dd_make(ldexp(a.hi, e), ldexp(a.lo, e)); }
-    __device__ static __forceinline__ dd sel(bool c, dd a, dd b) { return c ? a : b; }
-};
-
-// element access of caller memory: T storage, or FP64 storage promoted to T
-template <typename T> __device__ __forceinline__ T mem_load(const void* base, int64_t idx, int mem_f64)
-{
-    if (mem_f64) return Ar<T>::from(reinterpret_cast<const double*>(base)[idx], 0.0);
-    return reinterpret_cast<const T*>(base)[idx];
-}
-template <> __device__ __forceinline__ double mem_load<double>(const void* base, int64_t idx, int)
-{
-    return reinterpret_cast<const double*>(base)[idx];
-}
-template <typename T> __device__ __forceinline__ void mem_store(void* base, int64_t idx, T v, int mem_f64)
-{
-    if (mem_f64) {
-        // round DD to FP64 storage
-        reinterpret_cast<double*>(base)[idx] = Ar<T>::hi(Ar<T>::add(v, Ar<T>::zero()));
-        return;
-    }
-    reinterpret_cast<T*>(base)[idx] = v;
-}
-template <> __device__ __forceinline__ void mem_store<double>(void* base, int64_t idx, double v, int)
-{
-    reinterpret_cast<double*>(base)[idx] = v;
-}
-
-constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v / 2); }
-
-// reorder_i0 (storage position of logical row i), single-upload case N <= 3840:
-// PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:66-142
-__device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N)
-{
-    int64_t ws = ((N + 1023) / 1024) * 64;
-    if (N < 512) ws = 64;
-    int64_t R = (N + ws - 1) / ws;
-    if ((R & 1) == 0) R++;
-    int64_t ceilq = (N + R - 1) / R;
-    int64_t k = i % R;
-    int64_t rem = N % R;
-    int64_t ret = i / R;
-    if (k < rem)
-        ret += k * ceilq;
-    else
-        ret += rem * ceilq + (k - rem) * (N / R);
-    return ret;
-}
+#include "jw_common.h"
 
 // ------------------------------------------------------------------------
 // table layout (elements of T):
@@ -222,22 +55,41 @@ __host__ __device__ __forceinline__ int ilog2_rt(int v)
 
 // ------------------------------------------------------------------------
 // prologue: one wavefront (64 lanes) per matrix set
+//
+// Every per-row quantity is staged in LDS (10 arrays of R*L elements):
+//   v[-2..2], l, g, h, D, E   (rows >= N are zero)
+// then written in one of two table formats:
+//   format 1: the pairs above + Hillis-Steele multipliers over L lanes
+//             (jw_main_kernel);
+//   format 2 (L = 64): scalings folded in, a_r = D_r g_r,
+//       v'_d[r] = a_r v_d[r],  l'_r = a_r l_r / a_{r-1},  h'_r = D_r h_r / D_{r+1},
+//     so that the three recurrences are W_r = u'_r - l'_r W_{r-1},
+//     Z_r = W_r - h'_r Z_{r+1}, y_r = Z_r - E_r y_{r+1} (W = a w, Z = D z);
+//     8 coefficients per row, and per recurrence and lane 5 multipliers of the
+//     DPP scan: products over the 2^k lanes a row_shr/row_shl:2^k step
+//     combines (k = 0..3) and over the row prefix (forward) or suffix
+//     (backward) for the cross-row combine (jw_main64_kernel).
 // ------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const void* mat, T* table, int L, int R)
+__global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const void* mat, T* table, int L, int R,
+                                                         int format)
 {
     using A = Ar<T>;
     extern __shared__ __align__(16) unsigned char pro_smem[];
     const int RL = R * L;
-    T* sld = reinterpret_cast<T*>(pro_smem);
+    T* sv = reinterpret_cast<T*>(pro_smem);   // 5 arrays: v_{-2..2}
+    T* sld = sv + 5 * RL;
     T* sud = sld + RL;
-    T* sE = sud + RL;
+    T* sD = sud + RL;
+    T* sE = sD + RL;
     T* sl = sE + RL;
-    T* sh = sl + RL;
+    T* sg = sl + RL;
+    T* sh = sg + RL;
+    T* sB = sh + RL;                          // per-lane multipliers (3 x 64)
     const int lane = threadIdx.x;
     const int64_t N = d.N;
-    const int log2L = ilog2_rt(L);
     const int mf = d.mem_f64;
+    const bool has_b1 = d.solve_kind == JW_SOLVE_UPPER || d.solve_kind == JW_SOLVE_TRI;
 
     // phase 1: raw coefficients per row, zeros outside the matrix
     for (int r = lane; r < RL; r += 64) {
@@ -259,9 +111,10 @@ __global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const
                 if (r >= 1) v[1] = mem_load<T>(mat, d.ldA + P, mf);
                 break;
             case JW_BAND_GB:
-                for (int off = -d.KL; off <= d.KU; off++) {
-                    int64_t col = r + off;
-                    if (col < 0 || col >= N || off < -2 || off > 2) continue;
+#pragma unroll
+                for (int off = -2; off <= 2; off++) {
+                    const int64_t col = r + off;
+                    if (off < -d.KL || off > d.KU || col < 0 || col >= N) continue;
                     v[2 + off] = mem_load<T>(mat, d.offV + (int64_t)(d.KU - off) * N + col, mf);
                 }
                 break;
@@ -271,27 +124,24 @@ __global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const
             }
             if (d.mdS >= 0) {
                 T s = mem_load<T>(mat, d.mdS + P, mf);
+#pragma unroll
                 for (int k = 0; k < 5; k++) v[k] = A::mul(v[k], s);
             }
             if ((d.solve_kind == JW_SOLVE_LOWER || d.solve_kind == JW_SOLVE_TRI) && r >= 1)
                 ldv = mem_load<T>(mat, d.ldS + P, mf);
-            if ((d.solve_kind == JW_SOLVE_UPPER || d.solve_kind == JW_SOLVE_TRI) && r + 1 < N)
-                udv = mem_load<T>(mat, d.udS + P, mf);
+            if (has_b1 && r + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);
             if (d.has_m) {
                 Dv = mem_load<T>(mat, d.mdM + P, mf);
                 if (r + 1 < N) Ev = mem_load<T>(mat, d.udM + P, mf);
+            } else {
+                Dv = A::one();
             }
         }
-        const int t = r / R, i = r % R;
-        table[tab_coef(0, i, t, 0, R, L)] = v[0];
-        table[tab_coef(0, i, t, 1, R, L)] = v[1];
-        table[tab_coef(1, i, t, 0, R, L)] = v[2];
-        table[tab_coef(1, i, t, 1, R, L)] = v[3];
-        table[tab_coef(2, i, t, 0, R, L)] = v[4];
-        table[tab_coef(4, i, t, 0, R, L)] = Dv;
-        table[tab_coef(4, i, t, 1, R, L)] = Ev;
+#pragma unroll
+        for (int k = 0; k < 5; k++) sv[k * RL + r] = v[k];
         sld[r] = ldv;
         sud[r] = udv;
+        sD[r] = Dv;
         sE[r] = Ev;
     }
     __syncthreads();
@@ -299,6 +149,7 @@ __global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const
     // phase 2: factors of the unit-diagonal solve, lane t owns rows t*R .. t*R+R-1
     const int t = lane;
     const bool active = t < L;
+    const int log2L = ilog2_rt(L);
     if (d.solve_kind == JW_SOLVE_TRI) {
         // pivots p_r = 1 - m_r / p_{r-1}, m_r = ld_r ud_{r-1}: the Moebius map
         // p -> (p - m)/p is the 2x2 matrix [[1,-m],[1,0]] on (num, den).
@@ -340,52 +191,110 @@ __global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const
             }
         }
         // exclusive: composition of the maps of lanes 0..t-1 applied to (1,1)
-        {
-            const int src = (t >= 1) ? t - 1 : t;
-            T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
-            T pprev = A::one();
-            if (t >= 1) pprev = A::div(A::add(pa, pb), A::add(pc, pe));
-            if (active) {
-                for (int i = 0; i < R; i++) {
-                    const int r = t * R + i;
-                    T l = A::zero(), g = A::zero(), h = A::zero();
-                    if (r < N) {
-                        T m = (r >= 1) ? A::mul(sld[r], sud[r - 1]) : A::zero();
-                        l = (r >= 1) ? A::div(sld[r], pprev) : A::zero();
-                        T p = A::sub(A::one(), A::div(m, pprev));
-                        g = A::div(A::one(), p);
-                        h = A::mul(sud[r], g);
-                        pprev = p;
-                    }
-                    sl[r] = l;
-                    sh[r] = h;
-                    table[tab_coef(2, i, t, 1, R, L)] = l;
-                    table[tab_coef(3, i, t, 0, R, L)] = g;
-                    table[tab_coef(3, i, t, 1, R, L)] = h;
+        const int src = (t >= 1) ? t - 1 : t;
+        T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
+        T pprev = A::one();
+        if (t >= 1) pprev = A::div(A::add(pa, pb), A::add(pc, pe));
+        if (active) {
+            for (int i = 0; i < R; i++) {
+                const int r = t * R + i;
+                T l = A::zero(), g = A::zero(), h = A::zero();
+                if (r < N) {
+                    T m = (r >= 1) ? A::mul(sld[r], sud[r - 1]) : A::zero();
+                    l = (r >= 1) ? A::div(sld[r], pprev) : A::zero();
+                    T p = A::sub(A::one(), A::div(m, pprev));
+                    g = A::div(A::one(), p);
+                    h = A::mul(sud[r], g);
+                    pprev = p;
                 }
+                sl[r] = l;
+                sg[r] = g;
+                sh[r] = h;
             }
         }
     } else if (active) {
         for (int i = 0; i < R; i++) {
             const int r = t * R + i;
-            T l = A::zero(), g = A::zero(), h = A::zero();
-            if (r < N) {
-                if (d.solve_kind == JW_SOLVE_LOWER) l = sld[r];
-                if (d.solve_kind == JW_SOLVE_UPPER) { g = A::one(); h = sud[r]; }
-            }
-            sl[r] = l;
-            sh[r] = h;
-            table[tab_coef(2, i, t, 1, R, L)] = l;
-            table[tab_coef(3, i, t, 0, R, L)] = g;
-            table[tab_coef(3, i, t, 1, R, L)] = h;
+            const bool in = r < N;
+            sl[r] = (in && d.solve_kind == JW_SOLVE_LOWER) ? sld[r] : A::zero();
+            sg[r] = in ? A::one() : A::zero();
+            sh[r] = (in && d.solve_kind == JW_SOLVE_UPPER) ? sud[r] : A::zero();
         }
     }
+    __syncthreads();
 
-    // phase 3: per-lane products of the recurrence multipliers and the scan tables
+    if (format == 2) {
+        // fold the scalings: a_r = D_r g_r; each lane writes its own rows
+        if (active) {
+            T bf = A::one(), bb1 = A::one(), bb2 = A::one();
+            for (int i = 0; i < R; i++) {
+                const int r = t * R + i;
+                const T ar = A::mul(sD[r], sg[r]);
+                T cf[8];
+#pragma unroll
+                for (int k = 0; k < 5; k++) cf[k] = A::mul(ar, sv[k * RL + r]);
+                T lp = A::zero(), hp = A::zero();
+                if (r >= 1 && r < N) lp = A::div(A::mul(ar, sl[r]), A::mul(sD[r - 1], sg[r - 1]));
+                if (r + 1 < N) hp = A::div(A::mul(sD[r], sh[r]), sD[r + 1]);
+                cf[5] = lp;
+                cf[6] = hp;
+                cf[7] = sE[r];
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    table[tab_coef(p, i, t, 0, R, 64)] = cf[2 * p];
+                    table[tab_coef(p, i, t, 1, R, 64)] = cf[2 * p + 1];
+                }
+                bf = A::mul(bf, A::neg(lp));
+                bb1 = A::mul(bb1, A::neg(hp));
+                bb2 = A::mul(bb2, A::neg(sE[r]));
+            }
+            sB[t] = bf;
+            sB[64 + t] = bb1;
+            sB[128 + t] = bb2;
+        }
+        __syncthreads();
+        if (active) {
+            const int row0 = t & ~15, tr = t & 15;
+            const int64_t SC = (int64_t)8 * R * 64;
+            for (int j = 0; j < 3; j++) {
+                const T* Bj = sB + 64 * j;
+                for (int k = 0; k < 5; k++) {
+                    // k < 4: lanes combined by a shift of 2^k within the row; k = 4: row prefix / suffix
+                    T prod = A::one();
+                    int lo, hi;
+                    if (j == 0) {
+                        hi = t;
+                        lo = (k < 4) ? t - (1 << k) + 1 : row0;
+                        if (lo < row0) lo = row0;
+                    } else {
+                        lo = t;
+                        hi = (k < 4) ? t + (1 << k) - 1 : row0 + 15;
+                        if (hi > row0 + 15) hi = row0 + 15;
+                    }
+                    for (int u = lo; u <= hi; u++) prod = A::mul(prod, Bj[u]);
+                    (void)tr;
+                    table[SC + (int64_t)(j * 5 + k) * 64 + t] = prod;
+                }
+            }
+        }
+        return;
+    }
+
+    // format 1: pairs (vm2,vm1) (v0,vp1) (vp2,l) (g,h) (D,E) + Hillis-Steele multipliers
     T bf = A::one(), bb1 = A::one(), bb2 = A::one();
     if (active) {
         for (int i = 0; i < R; i++) {
             const int r = t * R + i;
+            table[tab_coef(0, i, t, 0, R, L)] = sv[0 * RL + r];
+            table[tab_coef(0, i, t, 1, R, L)] = sv[1 * RL + r];
+            table[tab_coef(1, i, t, 0, R, L)] = sv[2 * RL + r];
+            table[tab_coef(1, i, t, 1, R, L)] = sv[3 * RL + r];
+            table[tab_coef(2, i, t, 0, R, L)] = sv[4 * RL + r];
+            table[tab_coef(2, i, t, 1, R, L)] = sl[r];
+            table[tab_coef(3, i, t, 0, R, L)] = sg[r];
+            table[tab_coef(3, i, t, 1, R, L)] = sh[r];
+            table[tab_coef(4, i, t, 0, R, L)] = sD[r];
+            table[tab_coef(4, i, t, 1, R, L)] = sE[r];
             bf = A::mul(bf, A::neg(sl[r]));
             bb1 = A::mul(bb1, A::neg(sh[r]));
             bb2 = A::mul(bb2, A::neg(sE[r]));
@@ -852,11 +761,12 @@ int jwdev_device_info(int* cu_count, int* lds_per_block, int* arch_major, int* a
 }
 
 int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, int L, int R,
-                   int precision, void* stream)
+                   int precision, int format, void* stream)
 {
     const int RL = R * L;
     const size_t elt = precision == JW_PREC_F64 ? 8 : 16;
-    const size_t lds = 5 * (size_t)RL * elt;
+    const size_t lds = (10 * (size_t)RL + 192) * elt;
+    if (format == 2 && L != 64) { snprintf(g_err, sizeof(g_err), "format 2 needs 64 lanes"); return -1; }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const void* fn = precision == JW_PREC_F64 ? reinterpret_cast<const void*>(&jw_prologue_kernel<double>)
                                               : reinterpret_cast<const void*>(&jw_prologue_kernel<dd>);
@@ -864,10 +774,10 @@ int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, i
     if (ea != hipSuccess) { set_err("hipFuncSetAttribute(prologue)", ea); return -2; }
     if (precision == JW_PREC_F64) {
         hipLaunchKernelGGL(jw_prologue_kernel<double>, dim3(1), dim3(64), lds, s, *d, matrices,
-                           reinterpret_cast<double*>(table), L, R);
+                           reinterpret_cast<double*>(table), L, R, format);
     } else {
         hipLaunchKernelGGL(jw_prologue_kernel<dd>, dim3(1), dim3(64), lds, s, *d, matrices,
-                           reinterpret_cast<dd*>(table), L, R);
+                           reinterpret_cast<dd*>(table), L, R, format);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { set_err("prologue launch", e); return -1; }

@@ -103,7 +103,13 @@ int jwdev_supported(int L, int R, int precision);
 int64_t jwdev_table_elems(int L, int R);
 int64_t jwdev_lds_bytes(int L, int R, int precision);
 int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, int L, int R,
-                   int precision, void* stream);
+                   int precision, int format, void* stream);
+/* wave-per-system FP64 kernel (table format 2, 64 lanes, R in {2,4,8,16}) */
+int64_t jwdev_table64_elems(int R);
+int jwdev_main64_waves(int R);
+int64_t jwdev_main64_lds_bytes(int R, int waves);
+int jwdev_main64_occupancy(int R, int waves);
+int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream);
 int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream);
 int jwdev_block(const jw_block_desc* d, void* stream);
 int jwdev_gbmv(const jw_gbmv_desc* d, void* stream);

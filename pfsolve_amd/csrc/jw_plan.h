@@ -22,6 +22,9 @@ struct pfsolve_plan {
     void* table;              /* device: coefficient table */
     size_t table_bytes;
     int64_t lds_bytes;
+    /* wave-per-system FP64 path (table format 2): rows per lane (0 = not available),
+     * waves per workgroup and workgroups per CU */
+    int R64, waves64, occupancy64;
     hipEvent_t ev_start, ev_stop;   /* optional timing events around the main kernel */
 };
 

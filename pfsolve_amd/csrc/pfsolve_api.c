@@ -105,6 +105,22 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
         p->table_bytes = (size_t)g.table_bytes;
         const int kp = p->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD;
         if (!jwdev_supported(p->L, p->R, kp)) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
+        /* wave-per-system path: FP64, 64 lanes, R a power of two in [2, 16] */
+        if (p->precision == JW_PREC_F64 && cfg->kernelVariant != 1) {
+            int r64 = 2;
+            while (r64 * 64 < p->N) r64 *= 2;
+            if (r64 <= 16) {
+                p->R64 = r64;
+                p->waves64 = jwdev_main64_waves(r64);
+                p->occupancy64 = p->waves64 ? jwdev_main64_occupancy(r64, p->waves64) : 0;
+                if (p->occupancy64 < 1) p->R64 = 0;
+            }
+            if (p->R64) {
+                size_t b2 = (size_t)jwdev_table64_elems(p->R64) * 8;
+                if (b2 > p->table_bytes) p->table_bytes = b2;
+            }
+        }
+        if (cfg->kernelVariant == 2 && !p->R64) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
         if (hipMalloc(&p->table, p->table_bytes) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
         p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
         if (p->occupancy < 1) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY; }
@@ -163,7 +179,22 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
         run.table = p->table;
         const uintptr_t base = (uintptr_t)run.rhs;
         run.vec2 = p->precision == JW_PREC_F64 && (base % 16) == 0 && (run.stride % 2) == 0 && (run.offSol % 2) == 0;
-        if (jwdev_prologue(&src, cfg->buffer[0], p->table, p->L, p->R, p->precision, s) != 0)
+        if (p->R64 && run.vec2) {
+            /* wave-per-system kernel: table format 2 */
+            if (jwdev_prologue(&src, cfg->buffer[0], p->table, 64, p->R64, p->precision, 2, s) != 0)
+                return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            if (nsys <= 0) return PFSOLVE_SUCCESS;
+            const int64_t npass = (nsys + 1) / 2;
+            int64_t need = (npass + p->waves64 - 1) / p->waves64;
+            int64_t grid = (int64_t)p->cu_count * p->occupancy64;
+            if (grid > need) grid = need;
+            if (grid < 1) grid = 1;
+            if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+            if (jwdev_main64(&run, p->R64, p->waves64, (int)grid, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+            return PFSOLVE_SUCCESS;
+        }
+        if (jwdev_prologue(&src, cfg->buffer[0], p->table, p->L, p->R, p->precision, 1, s) != 0)
             return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
         if (nsys <= 0) return PFSOLVE_SUCCESS;
         const int G = 64 / p->L;
@@ -370,6 +401,7 @@ static const field_off k_cfg_fields[] = {
     OFF(PfSolveConfiguration, keepShaderCode), OFF(PfSolveConfiguration, disableCaching),
     OFF(PfSolveConfiguration, numConsecutiveJWIterations), OFF(PfSolveConfiguration, useMultipleInputBuffers),
     OFF(PfSolveConfiguration, offsetT), OFF(PfSolveConfiguration, autotune),
+    OFF(PfSolveConfiguration, kernelVariant),
 };
 static const field_off k_lp_fields[] = {
     OFF(PfSolveLaunchParams, buffer), OFF(PfSolveLaunchParams, outputBuffer), OFF(PfSolveLaunchParams, kernel),
@@ -407,6 +439,8 @@ PfSolveResult pfsolve_plan_geometry(const PfSolveApplication* app, pfsolve_geome
     g->precision = p->precision;
     g->table_bytes = (int64_t)p->table_bytes;
     g->lds_bytes = p->lds_bytes;
+    g->fast_rows_per_lane = p->R64;
+    g->fast_waves = p->waves64;
     return PFSOLVE_SUCCESS;
 }
 

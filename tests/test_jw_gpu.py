@@ -99,13 +99,16 @@ def fused_buffers(N, seed, permuted, pad=0):
 
 @pytest.mark.parametrize("N", [1, 2, 5, 16, 63, 64, 65, 100, 256, 511, 512, 777, 1024])
 @pytest.mark.parametrize("permuted", [1, 0])
-def test_fused_step(N, permuted):
-    B = 7
+@pytest.mark.parametrize("variant", [1, 2])
+def test_fused_step(N, permuted, variant):
+    """variant 1: general kernel (table format 1); variant 2: wave-per-system kernel
+    (format 2, used when the batch is 16-byte aligned: here B*N even needs N even)."""
+    B = 7 if variant == 1 else 8
     raw, buf, (offV, offT, offM) = fused_buffers(N, N, permuted)
     x = O.uniform(5 + N, -1, 1, B * N)
     jt = 14 if permuted else 1014
     cfg = dict(size=[N, B, 1], jw_type=jt, doublePrecision=1, scaleC=1.0, KL=2, KU=2,
-               offsetV=offV, offsetT=offT, offsetM=offM)
+               offsetV=offV, offsetT=offT, offsetM=offM, kernelVariant=variant)
     got = run_plan(cfg, buf, x)
     p = O.problem(jw_type=jt, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
                   cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
@@ -340,3 +343,38 @@ def test_dgbmv_plan(KL, KU, N):
     want = x.copy()
     assert O.lib().pfo_dgbmv_f128(C.byref(p), LDA, KU, KL, O.P(mat), O.P(want)) == 0
     check_systems(got, want, x, B, S, 0, N, 0, N, 1e-14)
+
+
+# ---------------------------------------------------------------- kernel selection
+def test_wave_per_system_kernel_is_selected_for_aligned_fp64():
+    import torch
+    N, B = 512, 64
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 3, 1)
+    dmat = torch.from_numpy(buf).cuda()
+    x = torch.zeros(B * N, dtype=torch.float64, device="cuda")
+    with pf.Plan(dmat.data_ptr(), x.data_ptr(), size=[N, B, 1], jw_type=14, doublePrecision=1,
+                 offsetV=offV, offsetT=offT, offsetM=offM) as plan:
+        g = plan.geometry()
+        assert g.fast_rows_per_lane == 8 and g.fast_waves >= 4
+    with pf.Plan(dmat.data_ptr(), x.data_ptr(), size=[N, B, 1], jw_type=14, doublePrecision=1,
+                 offsetV=offV, offsetT=offT, offsetM=offM, kernelVariant=1) as plan:
+        assert plan.geometry().fast_rows_per_lane == 0
+
+
+@pytest.mark.parametrize("jw_type,ub", MODES)
+@pytest.mark.parametrize("N", [2, 64, 130, 512, 1024])
+def test_reference_modes_wave_per_system(jw_type, ub, N):
+    """aligned batches of the reference modes run on the wave-per-system kernel"""
+    jw = jw_type % 10
+    B, S = 6, N + 64
+    buf = mode_buffer(jw, ub, N, 0, 0, 3 * N + 8, 2000 + N)
+    x = O.uniform(91 + N, -1, 1, B * S)
+    mask = _abi.RUNTIME_OUTPUTBUFFERSTRIDE | _abi.RUNTIME_SCALEC | _abi.RUNTIME_INPUTZEROPAD
+    cfg = dict(size=[N, B, 1], jw_type=jw_type, upperBanded=ub, doublePrecision=1, jw_control_bitmask=mask,
+               kernelVariant=2)
+    launch = dict(outputBufferStride=S, scaleC=1.25, inputZeropad=[1, N - 1])
+    got = run_plan(cfg, buf, x, launch)
+    p = O.problem(jw_type=jw_type, upperBanded=ub, N=N, B=B, mask=mask, cfg_outStride=N, run_outStride=S,
+                  run_scaleC=1.25, run_izp=[1, N - 1])
+    want = O.jw_f128(p, buf, x)
+    check_systems(got, want, x, B, S, 0, N, 0, N, TOL64)
