@@ -54,9 +54,9 @@ __host__ __device__ __forceinline__ int ilog2_rt(int v)
 }
 
 // ------------------------------------------------------------------------
-// prologue: one wavefront (64 lanes) per matrix set
+// prologue: one workgroup (256 threads) per matrix set
 //
-// Every per-row quantity is staged in LDS (10 arrays of R*L elements):
+// Every per-row quantity is staged in LDS (arrays of R*L elements):
 //   v[-2..2], l, g, h, D, E   (rows >= N are zero)
 // then written in one of two table formats:
 //   format 1: the pairs above + Hillis-Steele multipliers over L lanes
@@ -69,31 +69,40 @@ __host__ __device__ __forceinline__ int ilog2_rt(int v)
 //     DPP scan: products over the 2^k lanes a row_shr/row_shl:2^k step
 //     combines (k = 0..3) and over the row prefix (forward) or suffix
 //     (backward) for the cross-row combine (jw_main64_kernel).
+// Matrix loads are spread over all threads; the LU scan runs on wavefront 0.
 // ------------------------------------------------------------------------
+constexpr int kPrologueThreads = 256;
+
+// LDS of prologue_body: 12 arrays of R*L elements (v[-2..2], ld, ud, D, E, l, g, h)
+// followed by 3 x 64 per-lane multipliers
+static inline size_t prologue_lds_bytes(int RL, size_t elt) { return (12 * (size_t)RL + 192) * elt; }
+
 template <typename T>
-__global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const void* mat, T* table, int L, int R,
-                                                         int format)
+__device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table, int L, int R, int format,
+                              unsigned char* smem_raw)
 {
     using A = Ar<T>;
-    extern __shared__ __align__(16) unsigned char pro_smem[];
     const int RL = R * L;
-    T* sv = reinterpret_cast<T*>(pro_smem);   // 5 arrays: v_{-2..2}
-    T* sld = sv + 5 * RL;
-    T* sud = sld + RL;
+    T* sv = reinterpret_cast<T*>(smem_raw);   // 5 arrays: v_{-2..2}
+    T* sld = sv + 5 * RL;                      // raw ld, later l' (format 2)
+    T* sud = sld + RL;                         // raw ud, later h' (format 2)
     T* sD = sud + RL;
     T* sE = sD + RL;
     T* sl = sE + RL;
     T* sg = sl + RL;
     T* sh = sg + RL;
-    T* sB = sh + RL;                          // per-lane multipliers (3 x 64)
-    const int lane = threadIdx.x;
+    T* sB = sh + RL;                           // per-lane multipliers (3 x 64)
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
     const int64_t N = d.N;
     const int mf = d.mem_f64;
     const bool has_b1 = d.solve_kind == JW_SOLVE_UPPER || d.solve_kind == JW_SOLVE_TRI;
+    const bool has_f = d.solve_kind == JW_SOLVE_LOWER || d.solve_kind == JW_SOLVE_TRI;
 
     // phase 1: raw coefficients per row, zeros outside the matrix
-    for (int r = lane; r < RL; r += 64) {
+    for (int r = tid; r < RL; r += nthr) {
         T v[5];
+#pragma unroll
         for (int k = 0; k < 5; k++) v[k] = A::zero();
         T ldv = A::zero(), udv = A::zero(), Dv = A::zero(), Ev = A::zero();
         if (r < N) {
@@ -127,8 +136,7 @@ __global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const
 #pragma unroll
                 for (int k = 0; k < 5; k++) v[k] = A::mul(v[k], s);
             }
-            if ((d.solve_kind == JW_SOLVE_LOWER || d.solve_kind == JW_SOLVE_TRI) && r >= 1)
-                ldv = mem_load<T>(mat, d.ldS + P, mf);
+            if (has_f && r >= 1) ldv = mem_load<T>(mat, d.ldS + P, mf);
             if (has_b1 && r + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);
             if (d.has_m) {
                 Dv = mem_load<T>(mat, d.mdM + P, mf);
@@ -147,74 +155,75 @@ __global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const
     __syncthreads();
 
     // phase 2: factors of the unit-diagonal solve, lane t owns rows t*R .. t*R+R-1
-    const int t = lane;
-    const bool active = t < L;
-    const int log2L = ilog2_rt(L);
     if (d.solve_kind == JW_SOLVE_TRI) {
-        // pivots p_r = 1 - m_r / p_{r-1}, m_r = ld_r ud_{r-1}: the Moebius map
-        // p -> (p - m)/p is the 2x2 matrix [[1,-m],[1,0]] on (num, den).
-        T a = A::one(), b = A::zero(), c = A::zero(), e = A::one();
-        if (active) {
-            for (int i = 0; i < R; i++) {
-                const int r = t * R + i;
-                T m = (r >= 1 && r < N) ? A::mul(sld[r], sud[r - 1]) : A::zero();
-                T na = A::msub(a, m, c), nb = A::msub(b, m, e);
-                c = a;
-                e = b;
-                a = na;
-                b = nb;
-                double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
-                if (mx > 0.0) {
-                    int ex;
-                    frexp(mx, &ex);
-                    a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
+        if (tid < 64) {
+            const int t = tid;
+            const bool active = t < L;
+            const int log2L = ilog2_rt(L);
+            // pivots p_r = 1 - m_r / p_{r-1}, m_r = ld_r ud_{r-1}: the Moebius map
+            // p -> (p - m)/p is the 2x2 matrix [[1,-m],[1,0]] on (num, den).
+            T a = A::one(), b = A::zero(), c = A::zero(), e = A::one();
+            if (active) {
+                for (int i = 0; i < R; i++) {
+                    const int r = t * R + i;
+                    T m = (r >= 1 && r < N) ? A::mul(sld[r], sud[r - 1]) : A::zero();
+                    T na = A::msub(a, m, c), nb = A::msub(b, m, e);
+                    c = a;
+                    e = b;
+                    a = na;
+                    b = nb;
+                    double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
+                    if (mx > 0.0) {
+                        int ex;
+                        frexp(mx, &ex);
+                        a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
+                    }
                 }
             }
-        }
-        // inclusive scan over lanes: X_t <- X_t * X_{t-s}
-        for (int k = 0; k < log2L; k++) {
-            const int s = 1 << k;
-            const int src = (t >= s) ? t - s : t;
+            // inclusive scan over lanes: X_t <- X_t * X_{t-s}
+            for (int k = 0; k < log2L; k++) {
+                const int s = 1 << k;
+                const int src = (t >= s) ? t - s : t;
+                T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
+                if (active && t >= s) {
+                    T na = A::madd(A::mul(a, pa), b, pc);
+                    T nb = A::madd(A::mul(a, pb), b, pe);
+                    T nc = A::madd(A::mul(c, pa), e, pc);
+                    T ne = A::madd(A::mul(c, pb), e, pe);
+                    a = na; b = nb; c = nc; e = ne;
+                    double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
+                    if (mx > 0.0) {
+                        int ex;
+                        frexp(mx, &ex);
+                        a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
+                    }
+                }
+            }
+            // exclusive: composition of the maps of lanes 0..t-1 applied to (1,1)
+            const int src = (t >= 1) ? t - 1 : t;
             T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
-            if (active && t >= s) {
-                T na = A::madd(A::mul(a, pa), b, pc);
-                T nb = A::madd(A::mul(a, pb), b, pe);
-                T nc = A::madd(A::mul(c, pa), e, pc);
-                T ne = A::madd(A::mul(c, pb), e, pe);
-                a = na; b = nb; c = nc; e = ne;
-                double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
-                if (mx > 0.0) {
-                    int ex;
-                    frexp(mx, &ex);
-                    a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
+            T pprev = A::one();
+            if (t >= 1) pprev = A::div(A::add(pa, pb), A::add(pc, pe));
+            if (active) {
+                for (int i = 0; i < R; i++) {
+                    const int r = t * R + i;
+                    T l = A::zero(), g = A::zero(), h = A::zero();
+                    if (r < N) {
+                        T m = (r >= 1) ? A::mul(sld[r], sud[r - 1]) : A::zero();
+                        l = (r >= 1) ? A::div(sld[r], pprev) : A::zero();
+                        T p = A::sub(A::one(), A::div(m, pprev));
+                        g = A::div(A::one(), p);
+                        h = A::mul(sud[r], g);
+                        pprev = p;
+                    }
+                    sl[r] = l;
+                    sg[r] = g;
+                    sh[r] = h;
                 }
             }
         }
-        // exclusive: composition of the maps of lanes 0..t-1 applied to (1,1)
-        const int src = (t >= 1) ? t - 1 : t;
-        T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
-        T pprev = A::one();
-        if (t >= 1) pprev = A::div(A::add(pa, pb), A::add(pc, pe));
-        if (active) {
-            for (int i = 0; i < R; i++) {
-                const int r = t * R + i;
-                T l = A::zero(), g = A::zero(), h = A::zero();
-                if (r < N) {
-                    T m = (r >= 1) ? A::mul(sld[r], sud[r - 1]) : A::zero();
-                    l = (r >= 1) ? A::div(sld[r], pprev) : A::zero();
-                    T p = A::sub(A::one(), A::div(m, pprev));
-                    g = A::div(A::one(), p);
-                    h = A::mul(sud[r], g);
-                    pprev = p;
-                }
-                sl[r] = l;
-                sg[r] = g;
-                sh[r] = h;
-            }
-        }
-    } else if (active) {
-        for (int i = 0; i < R; i++) {
-            const int r = t * R + i;
+    } else {
+        for (int r = tid; r < RL; r += nthr) {
             const bool in = r < N;
             sl[r] = (in && d.solve_kind == JW_SOLVE_LOWER) ? sld[r] : A::zero();
             sg[r] = in ? A::one() : A::zero();
@@ -224,101 +233,131 @@ __global__ __launch_bounds__(64) void jw_prologue_kernel(jw_source_desc d, const
     __syncthreads();
 
     if (format == 2) {
-        // fold the scalings: a_r = D_r g_r; each lane writes its own rows
-        if (active) {
+        // fold the scalings (row-parallel); l' and h' go to the raw ld/ud arrays
+        for (int r = tid; r < RL; r += nthr) {
+            const int t = r / R, i = r % R;
+            const T ar = A::mul(sD[r], sg[r]);
+            T cf[8];
+#pragma unroll
+            for (int k = 0; k < 5; k++) cf[k] = A::mul(ar, sv[k * RL + r]);
+            T lp = A::zero(), hp = A::zero();
+            if (r >= 1 && r < N) lp = A::div(A::mul(ar, sl[r]), A::mul(sD[r - 1], sg[r - 1]));
+            if (r + 1 < N) hp = A::div(A::mul(sD[r], sh[r]), sD[r + 1]);
+            cf[5] = lp;
+            cf[6] = hp;
+            cf[7] = sE[r];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                table[tab_coef64(p, i, t, R)] = cf[2 * p];
+                table[tab_coef64(p, i, t, R) + 1] = cf[2 * p + 1];
+            }
+        }
+        __syncthreads();
+        // l', h' recomputed into LDS by the owning lanes for the per-lane products
+        if (tid < 64) {
+            const int t = tid;
             T bf = A::one(), bb1 = A::one(), bb2 = A::one();
             for (int i = 0; i < R; i++) {
                 const int r = t * R + i;
                 const T ar = A::mul(sD[r], sg[r]);
-                T cf[8];
-#pragma unroll
-                for (int k = 0; k < 5; k++) cf[k] = A::mul(ar, sv[k * RL + r]);
                 T lp = A::zero(), hp = A::zero();
                 if (r >= 1 && r < N) lp = A::div(A::mul(ar, sl[r]), A::mul(sD[r - 1], sg[r - 1]));
                 if (r + 1 < N) hp = A::div(A::mul(sD[r], sh[r]), sD[r + 1]);
-                cf[5] = lp;
-                cf[6] = hp;
-                cf[7] = sE[r];
-#pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    table[tab_coef(p, i, t, 0, R, 64)] = cf[2 * p];
-                    table[tab_coef(p, i, t, 1, R, 64)] = cf[2 * p + 1];
-                }
                 bf = A::mul(bf, A::neg(lp));
                 bb1 = A::mul(bb1, A::neg(hp));
                 bb2 = A::mul(bb2, A::neg(sE[r]));
             }
-            sB[t] = bf;
-            sB[64 + t] = bb1;
-            sB[128 + t] = bb2;
-        }
-        __syncthreads();
-        if (active) {
-            const int row0 = t & ~15, tr = t & 15;
+            // in-row Hillis-Steele: before step k the value covers the 2^k lanes a
+            // shift of 2^k combines; after 4 steps, the row prefix / suffix
+            const int tr = t & 15;
             const int64_t SC = (int64_t)8 * R * 64;
-            for (int j = 0; j < 3; j++) {
-                const T* Bj = sB + 64 * j;
-                for (int k = 0; k < 5; k++) {
-                    // k < 4: lanes combined by a shift of 2^k within the row; k = 4: row prefix / suffix
-                    T prod = A::one();
-                    int lo, hi;
-                    if (j == 0) {
-                        hi = t;
-                        lo = (k < 4) ? t - (1 << k) + 1 : row0;
-                        if (lo < row0) lo = row0;
-                    } else {
-                        lo = t;
-                        hi = (k < 4) ? t + (1 << k) - 1 : row0 + 15;
-                        if (hi > row0 + 15) hi = row0 + 15;
-                    }
-                    for (int u = lo; u <= hi; u++) prod = A::mul(prod, Bj[u]);
-                    (void)tr;
-                    table[SC + (int64_t)(j * 5 + k) * 64 + t] = prod;
+            T cur[3] = {bf, bb1, bb2};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int s = 1 << k;
+#pragma unroll
+                for (int j = 0; j < 3; j++) table[SC + (int64_t)(j * 5 + k) * 64 + t] = cur[j];
+                T up = lane_fetch(cur[0], tr >= s ? t - s : t);
+                T dn1 = lane_fetch(cur[1], tr + s <= 15 ? t + s : t);
+                T dn2 = lane_fetch(cur[2], tr + s <= 15 ? t + s : t);
+                if (tr >= s) cur[0] = A::mul(cur[0], up);
+                if (tr + s <= 15) {
+                    cur[1] = A::mul(cur[1], dn1);
+                    cur[2] = A::mul(cur[2], dn2);
                 }
             }
+#pragma unroll
+            for (int j = 0; j < 3; j++) table[SC + (int64_t)(j * 5 + 4) * 64 + t] = cur[j];
         }
         return;
     }
 
     // format 1: pairs (vm2,vm1) (v0,vp1) (vp2,l) (g,h) (D,E) + Hillis-Steele multipliers
-    T bf = A::one(), bb1 = A::one(), bb2 = A::one();
-    if (active) {
-        for (int i = 0; i < R; i++) {
-            const int r = t * R + i;
-            table[tab_coef(0, i, t, 0, R, L)] = sv[0 * RL + r];
-            table[tab_coef(0, i, t, 1, R, L)] = sv[1 * RL + r];
-            table[tab_coef(1, i, t, 0, R, L)] = sv[2 * RL + r];
-            table[tab_coef(1, i, t, 1, R, L)] = sv[3 * RL + r];
-            table[tab_coef(2, i, t, 0, R, L)] = sv[4 * RL + r];
-            table[tab_coef(2, i, t, 1, R, L)] = sl[r];
-            table[tab_coef(3, i, t, 0, R, L)] = sg[r];
-            table[tab_coef(3, i, t, 1, R, L)] = sh[r];
-            table[tab_coef(4, i, t, 0, R, L)] = sD[r];
-            table[tab_coef(4, i, t, 1, R, L)] = sE[r];
-            bf = A::mul(bf, A::neg(sl[r]));
-            bb1 = A::mul(bb1, A::neg(sh[r]));
-            bb2 = A::mul(bb2, A::neg(sE[r]));
-        }
+    for (int r = tid; r < RL; r += nthr) {
+        const int t = r / R, i = r % R;
+        table[tab_coef(0, i, t, 0, R, L)] = sv[0 * RL + r];
+        table[tab_coef(0, i, t, 1, R, L)] = sv[1 * RL + r];
+        table[tab_coef(1, i, t, 0, R, L)] = sv[2 * RL + r];
+        table[tab_coef(1, i, t, 1, R, L)] = sv[3 * RL + r];
+        table[tab_coef(2, i, t, 0, R, L)] = sv[4 * RL + r];
+        table[tab_coef(2, i, t, 1, R, L)] = sl[r];
+        table[tab_coef(3, i, t, 0, R, L)] = sg[r];
+        table[tab_coef(3, i, t, 1, R, L)] = sh[r];
+        table[tab_coef(4, i, t, 0, R, L)] = sD[r];
+        table[tab_coef(4, i, t, 1, R, L)] = sE[r];
     }
-    for (int k = 0; k < log2L; k++) {
-        const int s = 1 << k;
+    if (tid < 64) {
+        const int t = tid;
+        const bool active = t < L;
+        const int log2L = ilog2_rt(L);
+        T bf = A::one(), bb1 = A::one(), bb2 = A::one();
         if (active) {
-            table[tab_scan(k, t, 0, R, L)] = bf;
-            table[tab_scan(k, t, 1, R, L)] = bb1;
-            table[tab_scan(k, t, 2, R, L)] = bb2;
-            table[tab_scan(k, t, 3, R, L)] = A::zero();
+            for (int i = 0; i < R; i++) {
+                const int r = t * R + i;
+                bf = A::mul(bf, A::neg(sl[r]));
+                bb1 = A::mul(bb1, A::neg(sh[r]));
+                bb2 = A::mul(bb2, A::neg(sE[r]));
+            }
         }
-        const int su = (t >= s) ? t - s : t;
-        const int sdn = (t + s < L) ? t + s : t;
-        T ubf = lane_fetch(bf, su);
-        T ub1 = lane_fetch(bb1, sdn);
-        T ub2 = lane_fetch(bb2, sdn);
-        if (active && t >= s) bf = A::mul(bf, ubf);
-        if (active && t + s < L) {
-            bb1 = A::mul(bb1, ub1);
-            bb2 = A::mul(bb2, ub2);
+        for (int k = 0; k < log2L; k++) {
+            const int s = 1 << k;
+            if (active) {
+                table[tab_scan(k, t, 0, R, L)] = bf;
+                table[tab_scan(k, t, 1, R, L)] = bb1;
+                table[tab_scan(k, t, 2, R, L)] = bb2;
+                table[tab_scan(k, t, 3, R, L)] = A::zero();
+            }
+            const int su = (t >= s) ? t - s : t;
+            const int sdn = (t + s < L) ? t + s : t;
+            T ubf = lane_fetch(bf, su);
+            T ub1 = lane_fetch(bb1, sdn);
+            T ub2 = lane_fetch(bb2, sdn);
+            if (active && t >= s) bf = A::mul(bf, ubf);
+            if (active && t + s < L) {
+                bb1 = A::mul(bb1, ub1);
+                bb2 = A::mul(bb2, ub2);
+            }
         }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPrologueThreads) void jw_prologue_kernel(jw_source_desc d, const void* mat, T* table,
+                                                                       int L, int R, int format)
+{
+    extern __shared__ __align__(16) unsigned char pro_smem[];
+    prologue_body<T>(d, mat, table, L, R, format, pro_smem);
+}
+
+// batched: one workgroup per matrix set; workgroup 0 also clears the work-queue counters
+template <typename T>
+__global__ __launch_bounds__(kPrologueThreads) void jw_prologue_batch_kernel(const jw_prologue_job* jobs, int L_unused,
+                                                                             int* counters, int ncounters)
+{
+    extern __shared__ __align__(16) unsigned char pro_smem[];
+    if (blockIdx.x == 0 && counters && (int)threadIdx.x < ncounters) counters[threadIdx.x] = 0;
+    const jw_prologue_job& j = jobs[blockIdx.x];
+    prologue_body<T>(j.src, j.matrices, reinterpret_cast<T*>(j.table), j.L, j.R, j.format, pro_smem);
 }
 
 // ------------------------------------------------------------------------
@@ -765,7 +804,8 @@ int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, i
 {
     const int RL = R * L;
     const size_t elt = precision == JW_PREC_F64 ? 8 : 16;
-    const size_t lds = (10 * (size_t)RL + 192) * elt;
+    const size_t lds = prologue_lds_bytes(RL, elt);
+    if (lds > 160 * 1024) { snprintf(g_err, sizeof(g_err), "prologue LDS too large"); return -1; }
     if (format == 2 && L != 64) { snprintf(g_err, sizeof(g_err), "format 2 needs 64 lanes"); return -1; }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const void* fn = precision == JW_PREC_F64 ? reinterpret_cast<const void*>(&jw_prologue_kernel<double>)
@@ -773,14 +813,38 @@ int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, i
     hipError_t ea = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (ea != hipSuccess) { set_err("hipFuncSetAttribute(prologue)", ea); return -2; }
     if (precision == JW_PREC_F64) {
-        hipLaunchKernelGGL(jw_prologue_kernel<double>, dim3(1), dim3(64), lds, s, *d, matrices,
+        hipLaunchKernelGGL(jw_prologue_kernel<double>, dim3(1), dim3(kPrologueThreads), lds, s, *d, matrices,
                            reinterpret_cast<double*>(table), L, R, format);
     } else {
-        hipLaunchKernelGGL(jw_prologue_kernel<dd>, dim3(1), dim3(64), lds, s, *d, matrices,
+        hipLaunchKernelGGL(jw_prologue_kernel<dd>, dim3(1), dim3(kPrologueThreads), lds, s, *d, matrices,
                            reinterpret_cast<dd*>(table), L, R, format);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { set_err("prologue launch", e); return -1; }
+    return 0;
+}
+
+int jwdev_prologue_batch(const jw_prologue_job* d_jobs, int njobs, int max_RL, int precision, int* d_counters,
+                         int ncounters, void* stream)
+{
+    const size_t elt = precision == JW_PREC_F64 ? 8 : 16;
+    const size_t lds = prologue_lds_bytes(max_RL, elt);
+    if (lds > 160 * 1024) { snprintf(g_err, sizeof(g_err), "prologue LDS too large"); return -1; }
+    if (njobs < 1) return 0;
+    if (ncounters > kPrologueThreads) { snprintf(g_err, sizeof(g_err), "too many counters"); return -1; }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const void* fn = precision == JW_PREC_F64 ? reinterpret_cast<const void*>(&jw_prologue_batch_kernel<double>)
+                                              : reinterpret_cast<const void*>(&jw_prologue_batch_kernel<dd>);
+    hipError_t ea = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (ea != hipSuccess) { set_err("hipFuncSetAttribute(prologue batch)", ea); return -2; }
+    if (precision == JW_PREC_F64)
+        hipLaunchKernelGGL(jw_prologue_batch_kernel<double>, dim3(njobs), dim3(kPrologueThreads), lds, s, d_jobs, 0,
+                           d_counters, ncounters);
+    else
+        hipLaunchKernelGGL(jw_prologue_batch_kernel<dd>, dim3(njobs), dim3(kPrologueThreads), lds, s, d_jobs, 0,
+                           d_counters, ncounters);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { set_err("prologue batch launch", e); return -1; }
     return 0;
 }
 

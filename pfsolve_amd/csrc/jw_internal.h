@@ -71,6 +71,14 @@ typedef struct jw_run_desc {
     int32_t mem_f64;          /* DD compute with FP64 storage */
 } jw_run_desc;
 
+/* One matrix set for the batched prologue (one workgroup each). */
+typedef struct jw_prologue_job {
+    jw_source_desc src;
+    const void* matrices;
+    void* table;
+    int32_t L, R, format, pad;
+} jw_prologue_job;
+
 /* Uneven-size batch: one work item per matrix set. */
 typedef struct jw_batch_item {
     void* rhs;
@@ -104,6 +112,8 @@ int64_t jwdev_table_elems(int L, int R);
 int64_t jwdev_lds_bytes(int L, int R, int precision);
 int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, int L, int R,
                    int precision, int format, void* stream);
+int jwdev_prologue_batch(const jw_prologue_job* d_jobs, int njobs, int max_RL, int precision, int* d_counters,
+                         int ncounters, void* stream);
 /* wave-per-system FP64 kernel (table format 2, 64 lanes, R in {2,4,8,16}) */
 int64_t jwdev_table64_elems(int R);
 int jwdev_main64_waves(int R);
