@@ -65,10 +65,11 @@ __host__ __device__ __forceinline__ int ilog2_rt(int v)
 //       v'_d[r] = a_r v_d[r],  l'_r = a_r l_r / a_{r-1},  h'_r = D_r h_r / D_{r+1},
 //     so that the three recurrences are W_r = u'_r - l'_r W_{r-1},
 //     Z_r = W_r - h'_r Z_{r+1}, y_r = Z_r - E_r y_{r+1} (W = a w, Z = D z);
-//     8 coefficients per row, and per recurrence and lane 5 multipliers of the
+//     8 coefficients per row, and per recurrence and lane 8 multipliers of the
 //     DPP scan: products over the 2^k lanes a row_shr/row_shl:2^k step
-//     combines (k = 0..3) and over the row prefix (forward) or suffix
-//     (backward) for the cross-row combine (jw_main64_kernel).
+//     combines (k = 0..3), then the cross-row multipliers (forward: row
+//     prefix and the row_bcast31 step; backward: one per later row start)
+//     (jw_main64_kernel).  Table size JW_TAB64_ELEMS(R).
 // Matrix loads are spread over all threads; the LU scan runs on wavefront 0.
 // ------------------------------------------------------------------------
 constexpr int kPrologueThreads = 256;
@@ -276,7 +277,7 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
             for (int k = 0; k < 4; k++) {
                 const int s = 1 << k;
 #pragma unroll
-                for (int j = 0; j < 3; j++) table[SC + (int64_t)(j * 5 + k) * 64 + t] = cur[j];
+                for (int j = 0; j < 3; j++) table[SC + (int64_t)(j * 8 + k) * 64 + t] = cur[j];
                 T up = lane_fetch(cur[0], tr >= s ? t - s : t);
                 T dn1 = lane_fetch(cur[1], tr + s <= 15 ? t + s : t);
                 T dn2 = lane_fetch(cur[2], tr + s <= 15 ? t + s : t);
@@ -286,8 +287,35 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
                     cur[2] = A::mul(cur[2], dn2);
                 }
             }
-#pragma unroll
-            for (int j = 0; j < 3; j++) table[SC + (int64_t)(j * 5 + 4) * 64 + t] = cur[j];
+            const int rw = t >> 4;
+            // forward cross-row combine in two broadcasts: rows 1,3 take the end of
+            // rows 0,2 (multiplier: row prefix), then rows 2,3 take lane 31, which
+            // row 3 reaches through all of row 2 (prefix of lane 47 times its own)
+            {
+                const T pre47 = lane_fetch(cur[0], 47);
+                T m2 = A::zero();
+                if (rw == 2) m2 = cur[0];
+                if (rw == 3) m2 = A::mul(pre47, cur[0]);
+                table[SC + (int64_t)4 * 64 + t] = cur[0];
+                table[SC + (int64_t)5 * 64 + t] = m2;
+                table[SC + (int64_t)6 * 64 + t] = A::zero();
+                table[SC + (int64_t)7 * 64 + t] = A::zero();
+            }
+            // backward cross-row combine from the in-row results S_1..S_3 at the
+            // row starts (lanes 16, 32, 48): a lane of row r adds
+            //   suf_t * F_{r+1},  F_q = S_q + P_q F_{q+1},  P_q = suffix product of lane 16q,
+            // i.e. c_k,t * S_k with c_k,t = suf_t * prod_{r<q<k} P_q for k > r, else 0
+            for (int j = 1; j < 3; j++) {
+                const T P1 = lane_fetch(cur[j], 16), P2 = lane_fetch(cur[j], 32);
+                T c1 = A::zero(), c2 = A::zero(), c3 = A::zero();
+                if (rw == 0) { c1 = cur[j]; c2 = A::mul(cur[j], P1); c3 = A::mul(c2, P2); }
+                if (rw == 1) { c2 = cur[j]; c3 = A::mul(cur[j], P2); }
+                if (rw == 2) c3 = cur[j];
+                table[SC + (int64_t)(j * 8 + 4) * 64 + t] = c1;
+                table[SC + (int64_t)(j * 8 + 5) * 64 + t] = c2;
+                table[SC + (int64_t)(j * 8 + 6) * 64 + t] = c3;
+                table[SC + (int64_t)(j * 8 + 7) * 64 + t] = A::zero();
+            }
         }
         return;
     }

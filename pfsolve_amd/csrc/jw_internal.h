@@ -34,6 +34,9 @@ enum { JW_PREC_F64 = 1, JW_PREC_DD = 2, JW_PREC_DD_F64MEM = 3 };
 #define JW_NCOEF 10           /* coefficients per row in the table */
 #define JW_NPAIR 5            /* ... stored as 5 pairs */
 #define JW_MAX_BAND 2         /* |offset| of V */
+/* table format 2 (64 lanes, R rows per lane): 8 coefficients per row, then
+ * 3 recurrences x 8 scan multipliers per lane */
+#define JW_TAB64_ELEMS(R) (8 * (R) * 64 + 24 * 64)
 
 /* Where the prologue finds the raw coefficients of one matrix set.
  * Offsets are element offsets into the matrix buffer. */

@@ -3,18 +3,21 @@
 // One wavefront owns a pair of systems (the re/im halves of a complex system
 // in XXII layout, or two consecutive real systems); lane t owns rows
 // t*R .. t*R+R-1 of both.  Per pair:
-//   1. input: 16-byte chunks DMA'd global -> LDS (global_load_lds_dwordx4),
-//      64 lanes per instruction on one contiguous KiB, landing in an
-//      XOR-swizzled order so that the blocked read-back (lane t reads its R
-//      rows) is bank-conflict free; chunks outside the input keep-range are
-//      not fetched;
+//   1. input: the 16-byte chunks (row pairs) inside the input keep-range are
+//      DMA'd global -> LDS (global_load_lds_dwordx4), 64 lanes per
+//      instruction on one contiguous KiB, each lane fetching the chunk whose
+//      XOR-swizzled slot it lands in, so the blocked read-back (lane t reads its
+//      R rows) is bank-conflict free; an odd range end is read by its owning
+//      lane alone;
 //   2. banded matvec u' = a.(V x) with neighbour rows by DPP wave shifts;
 //   3. three first-order recurrences (forward W, backward Z, backward y),
-//      each: lane-local sweep, affine scan across the wavefront (DPP row
-//      shifts 1,2,4,8 inside 16-lane rows, then readlane of the row ends for
-//      the cross-row combine), lane-local redo;
+//      each: lane-local sweep, affine scan across the wavefront, lane-local
+//      redo.  Scan: DPP row_shr/row_shl 1,2,4,8 inside 16-lane rows; across
+//      rows DPP row_bcast15/row_bcast31 (forward) or readlane of the row
+//      starts (backward);
 //   4. row-0 scaleC; output written blocked to LDS, read back contiguous and
-//      stored with 16-byte stores (rows outside the output keep-range untouched).
+//      stored with 16-byte stores (rows outside the output keep-range untouched;
+//      an odd range end is written by its owning lane alone).
 // Coefficients (8 per row, scalings folded) and scan multipliers come from the
 // per-matrix-set table written by jw_prologue_kernel (format 2), staged once
 // per workgroup in LDS.
@@ -36,17 +39,39 @@ __device__ __forceinline__ int swz(int c)
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
 
+// DPP move of a double with an explicit row mask; rows outside the mask and
+// lanes without a source receive 0
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_rm(double v)
+{
+    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, false);
+    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
 template <int R>
 constexpr int max_threads()
 {
     return R <= 4 ? 1024 : (R == 8 ? 768 : 320);
 }
 
+// value of row r (which lane `lane` owns when r / R == lane) from the lane's registers
+template <int R>
+__device__ __forceinline__ double pick_row(const double (&x)[R], int i)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) v = (k == i) ? x[k] : v;
+    return v;
+}
+
 template <int R>
 __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc d)
 {
     constexpr int H = R / 2;                  // 16-byte chunks per lane per system
-    constexpr int NTAB = 8 * R * 64 + 15 * 64;
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
     constexpr int SYSC = 64 * H;              // chunks per system region
     constexpr int SC = 8 * R * 64;            // start of the scan multipliers
     extern __shared__ __align__(16) unsigned char smem[];
@@ -58,87 +83,115 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc
         constexpr int n16 = NTAB * 8 / 16;
         for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
     }
-    __syncthreads();
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = blockDim.x >> 6;
-    const int row = lane >> 4;
     double2* stg = reinterpret_cast<double2*>(smem + NTAB * 8) + (size_t)wave * 2 * SYSC;
     double* rhs = reinterpret_cast<double*>(d.rhs);
     const int64_t npass = (d.nsys + 1) >> 1;
 
+    // full chunks of the keep-ranges and their odd ends (32-bit row arithmetic)
+    const int in_lo = (int)d.in_lo, in_hi = (int)d.in_hi;
+    const int out_lo = (int)d.out_lo, out_hi = (int)d.out_hi;
+    const int icl = (in_lo + 1) >> 1, ich = in_hi >> 1;
+    const int ocl = (out_lo + 1) >> 1, och = out_hi >> 1;
+    const bool in_edge_lo = (in_lo & 1) && in_lo < in_hi;
+    const bool in_edge_hi = (in_hi & 1) && in_hi - 1 >= in_lo;   // never the row of in_edge_lo: lo odd => lo+1 even
+    const bool out_edge_lo = (out_lo & 1) && out_lo < out_hi;
+    const bool out_edge_hi = (out_hi & 1) && out_hi - 1 >= out_lo;
+
+    const double row0_scale = lane == 0 ? d.scale_hi : 1.0;
+
+    // Staging slots that the input DMA never fills (chunks outside the input
+    // keep-range) hold zeros, so the blocked read-back needs no masking.
+    // The output phase overwrites every slot and re-zeroes those afterwards.
+#pragma unroll
+    for (int k = 0; k < 2 * H; k++) stg[64 * k + lane] = make_double2(0.0, 0.0);
+    __syncthreads();
+
     for (int64_t q = (int64_t)blockIdx.x * nwaves + wave; q < npass; q += (int64_t)gridDim.x * nwaves) {
         const int64_t sys0 = 2 * q;
+        const bool ok1 = sys0 + 1 < d.nsys;
+        const double* src0 = rhs + sys0 * d.stride + d.offSol;
 
-        // ---- 1. input: DMA of the chunks inside the keep-range ----
+        // ---- 1. input ----
 #pragma unroll
         for (int s = 0; s < 2; s++) {
-            const int64_t sys = sys0 + s;
-            if (sys < d.nsys) {
-                const double* src = rhs + sys * d.stride + d.offSol;
+            if (s == 1 && !ok1) continue;
+            const double* src = src0 + s * d.stride;
 #pragma unroll
-                for (int k = 0; k < H; k++) {
-                    const int c = swz<H>(64 * k + lane);
-                    const int r = 2 * c;
-                    if (r >= d.in_lo && r + 2 <= d.in_hi) {
-                        __builtin_amdgcn_global_load_lds(
-                            src + r, (__attribute__((address_space(3))) void*)(stg + s * SYSC + 64 * k), 16, 0, 0);
-                    }
-                }
+            for (int k = 0; k < H; k++) {
+                const int c = swz<H>(64 * k + lane);
+                if (c >= icl && c < ich)
+                    __builtin_amdgcn_global_load_lds(
+                        src + 2 * c, (__attribute__((address_space(3))) void*)(stg + s * SYSC + 64 * k), 16, 0, 0);
             }
         }
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the DMA writes have landed in LDS
         compiler_fence();
 
-        double x[2][R];
+        double x0[R], x1[R];
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const int64_t sys = sys0 + s;
-            const bool ok = sys < d.nsys;
-            const double* src = rhs + sys * d.stride + d.offSol;
+        for (int m = 0; m < H; m++) {
+            const double2 v0 = stg[swz<H>(lane * H + m)];
+            const double2 v1 = stg[SYSC + swz<H>(lane * H + m)];
+            x0[2 * m] = v0.x;
+            x0[2 * m + 1] = v0.y;
+            x1[2 * m] = v1.x;
+            x1[2 * m + 1] = v1.y;
+        }
+        if (in_edge_lo || in_edge_hi) {
 #pragma unroll
-            for (int m = 0; m < H; m++) {
-                const int c = lane * H + m;
-                const int r = 2 * c;
-                const double2 v = stg[s * SYSC + swz<H>(c)];
-                const bool k0 = ok && r >= d.in_lo && r < d.in_hi;
-                const bool k1 = ok && r + 1 >= d.in_lo && r + 1 < d.in_hi;
-                const bool full = k0 && k1;
-                x[s][2 * m] = full ? v.x : 0.0;
-                x[s][2 * m + 1] = full ? v.y : 0.0;
-                if (!full && (k0 || k1)) {   // chunk straddling a keep-range boundary
-                    if (k0) x[s][2 * m] = src[r];
-                    if (k1) x[s][2 * m + 1] = src[r + 1];
+            for (int e = 0; e < 2; e++) {
+                const bool on = e == 0 ? in_edge_lo : in_edge_hi;
+                const int r = e == 0 ? in_lo : in_hi - 1;
+                if (on && lane == r / R) {
+                    const double v0 = src0[r];
+                    const double v1 = ok1 ? src0[d.stride + r] : 0.0;
+#pragma unroll
+                    for (int k = 0; k < R; k++)
+                        if (k == r % R) {
+                            x0[k] = v0;
+                            x1[k] = v1;
+                        }
                 }
             }
         }
 
         // ---- 2. banded matvec (coefficients carry the folded row scaling) ----
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            double xm1 = 0.0, xm2 = 0.0, xp1 = 0.0, xp2 = 0.0;
-            if (d.kl >= 1) xm1 = dpp_d<DPP_WAVE_SHR1>(x[s][R - 1]);
-            if (d.kl >= 2) xm2 = dpp_d<DPP_WAVE_SHR1>(x[s][R - 2]);
-            if (d.ku >= 1) xp1 = dpp_d<DPP_WAVE_SHL1>(x[s][0]);
-            if (d.ku >= 2) xp2 = dpp_d<DPP_WAVE_SHL1>(x[s][1]);
-            double pm2 = xm2, pm1 = xm1;
+        {
+            double m1a = 0.0, m2a = 0.0, p1a = 0.0, p2a = 0.0;
+            double m1b = 0.0, m2b = 0.0, p1b = 0.0, p2b = 0.0;
+            if (d.kl >= 1) { m1a = dpp_d<DPP_WAVE_SHR1>(x0[R - 1]); m1b = dpp_d<DPP_WAVE_SHR1>(x1[R - 1]); }
+            if (d.kl >= 2) { m2a = dpp_d<DPP_WAVE_SHR1>(x0[R - 2]); m2b = dpp_d<DPP_WAVE_SHR1>(x1[R - 2]); }
+            if (d.ku >= 1) { p1a = dpp_d<DPP_WAVE_SHL1>(x0[0]); p1b = dpp_d<DPP_WAVE_SHL1>(x1[0]); }
+            if (d.ku >= 2) { p2a = dpp_d<DPP_WAVE_SHL1>(x0[1]); p2b = dpp_d<DPP_WAVE_SHL1>(x1[1]); }
 #pragma unroll
             for (int i = 0; i < R; i++) {
                 const double2 c0 = *reinterpret_cast<const double2*>(tab + tab_coef64(0, i, lane, R));
                 const double2 c1 = *reinterpret_cast<const double2*>(tab + tab_coef64(1, i, lane, R));
                 const double vp2 = tab[tab_coef64(2, i, lane, R)];
-                const double xi = x[s][i];
-                const double n1 = (i + 1 < R) ? x[s][(i + 1) % R] : xp1;
-                const double n2 = (i + 2 < R) ? x[s][(i + 2) % R] : ((i + 2 == R) ? xp1 : xp2);
-                double u = c1.x * xi;
-                u = __builtin_fma(c0.y, pm1, u);
-                u = __builtin_fma(c0.x, pm2, u);
-                u = __builtin_fma(c1.y, n1, u);
-                u = __builtin_fma(vp2, n2, u);
-                pm2 = pm1;
-                pm1 = xi;
-                x[s][i] = u;
+                const double ia = x0[i], ib = x1[i];
+                const double n1a = (i + 1 < R) ? x0[(i + 1) % R] : p1a;
+                const double n1b = (i + 1 < R) ? x1[(i + 1) % R] : p1b;
+                const double n2a = (i + 2 < R) ? x0[(i + 2) % R] : ((i + 2 == R) ? p1a : p2a);
+                const double n2b = (i + 2 < R) ? x1[(i + 2) % R] : ((i + 2 == R) ? p1b : p2b);
+                double ua = c1.x * ia, ub = c1.x * ib;
+                ua = __builtin_fma(c0.y, m1a, ua);
+                ub = __builtin_fma(c0.y, m1b, ub);
+                ua = __builtin_fma(c0.x, m2a, ua);
+                ub = __builtin_fma(c0.x, m2b, ub);
+                ua = __builtin_fma(c1.y, n1a, ua);
+                ub = __builtin_fma(c1.y, n1b, ub);
+                ua = __builtin_fma(vp2, n2a, ua);
+                ub = __builtin_fma(vp2, n2b, ub);
+                m2a = m1a;
+                m2b = m1b;
+                m1a = ia;
+                m1b = ib;
+                x0[i] = ua;
+                x1[i] = ub;
             }
         }
 
@@ -147,35 +200,42 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc
             double lr[R];
 #pragma unroll
             for (int i = 0; i < R; i++) lr[i] = tab[tab_coef64(2, i, lane, R) + 1];
-            double acc[2];
+            double a = x0[0], b = x1[0];
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
-                double a = 0.0;
-#pragma unroll
-                for (int i = 0; i < R; i++) a = __builtin_fma(-lr[i], a, x[s][i]);
-                acc[s] = a;
+            for (int i = 1; i < R; i++) {
+                a = __builtin_fma(-lr[i], a, x0[i]);
+                b = __builtin_fma(-lr[i], b, x1[i]);
             }
-            const double b0 = tab[SC + 0 * 64 + lane], b1 = tab[SC + 1 * 64 + lane];
-            const double b2 = tab[SC + 2 * 64 + lane], b3 = tab[SC + 3 * 64 + lane];
-            const double bp = tab[SC + 4 * 64 + lane];
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
-                double a = acc[s];
-                a = __builtin_fma(b0, dpp_d<DPP_ROW_SHR(1)>(a), a);
-                a = __builtin_fma(b1, dpp_d<DPP_ROW_SHR(2)>(a), a);
-                a = __builtin_fma(b2, dpp_d<DPP_ROW_SHR(4)>(a), a);
-                a = __builtin_fma(b3, dpp_d<DPP_ROW_SHR(8)>(a), a);
-                const double e0 = readlane_d(a, 15);
-                const double e1 = readlane_d(__builtin_fma(bp, e0, a), 31);
-                const double e2 = readlane_d(__builtin_fma(bp, e1, a), 47);
-                const double ep = row == 0 ? 0.0 : (row == 1 ? e0 : (row == 2 ? e1 : e2));
-                a = __builtin_fma(bp, ep, a);
-                double w = dpp_d<DPP_WAVE_SHR1>(a);
-#pragma unroll
-                for (int i = 0; i < R; i++) {
-                    w = __builtin_fma(-lr[i], w, x[s][i]);
-                    x[s][i] = w;
+            for (int k = 0; k < 4; k++) {
+                const double f = tab[SC + k * 64 + lane];
+                double sa, sb;
+                switch (k) {
+                case 0: sa = dpp_d<DPP_ROW_SHR(1)>(a); sb = dpp_d<DPP_ROW_SHR(1)>(b); break;
+                case 1: sa = dpp_d<DPP_ROW_SHR(2)>(a); sb = dpp_d<DPP_ROW_SHR(2)>(b); break;
+                case 2: sa = dpp_d<DPP_ROW_SHR(4)>(a); sb = dpp_d<DPP_ROW_SHR(4)>(b); break;
+                default: sa = dpp_d<DPP_ROW_SHR(8)>(a); sb = dpp_d<DPP_ROW_SHR(8)>(b); break;
                 }
+                a = __builtin_fma(f, sa, a);
+                b = __builtin_fma(f, sb, b);
+            }
+            {
+                // rows 1 and 3 take the ends of rows 0 and 2; rows 2 and 3 then take lane 31
+                const double fp = tab[SC + 4 * 64 + lane], fp2 = tab[SC + 5 * 64 + lane];
+                const double sa = dpp_rm<DPP_ROW_BCAST15, 0xa>(a), sb = dpp_rm<DPP_ROW_BCAST15, 0xa>(b);
+                a = __builtin_fma(fp, sa, a);
+                b = __builtin_fma(fp, sb, b);
+                const double ta = dpp_rm<DPP_ROW_BCAST31, 0xc>(a), tb = dpp_rm<DPP_ROW_BCAST31, 0xc>(b);
+                a = __builtin_fma(fp2, ta, a);
+                b = __builtin_fma(fp2, tb, b);
+            }
+            double wa = dpp_d<DPP_WAVE_SHR1>(a), wb = dpp_d<DPP_WAVE_SHR1>(b);
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                wa = __builtin_fma(-lr[i], wa, x0[i]);
+                wb = __builtin_fma(-lr[i], wb, x1[i]);
+                x0[i] = wa;
+                x1[i] = wb;
             }
         }
 
@@ -188,79 +248,97 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc
 #pragma unroll
             for (int i = 0; i < R; i++) hh[i] = tab[tab_coef64(3, i, lane, R) + stage];
             const int j = 1 + stage;
-            double acc[2];
+            double a = x0[R - 1], b = x1[R - 1];
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
-                double a = 0.0;
-#pragma unroll
-                for (int i = R - 1; i >= 0; i--) a = __builtin_fma(-hh[i], a, x[s][i]);
-                acc[s] = a;
+            for (int i = R - 2; i >= 0; i--) {
+                a = __builtin_fma(-hh[i], a, x0[i]);
+                b = __builtin_fma(-hh[i], b, x1[i]);
             }
-            const double b0 = tab[SC + (j * 5 + 0) * 64 + lane], b1 = tab[SC + (j * 5 + 1) * 64 + lane];
-            const double b2 = tab[SC + (j * 5 + 2) * 64 + lane], b3 = tab[SC + (j * 5 + 3) * 64 + lane];
-            const double bs = tab[SC + (j * 5 + 4) * 64 + lane];
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
-                double a = acc[s];
-                a = __builtin_fma(b0, dpp_d<DPP_ROW_SHL(1)>(a), a);
-                a = __builtin_fma(b1, dpp_d<DPP_ROW_SHL(2)>(a), a);
-                a = __builtin_fma(b2, dpp_d<DPP_ROW_SHL(4)>(a), a);
-                a = __builtin_fma(b3, dpp_d<DPP_ROW_SHL(8)>(a), a);
-                const double e3 = readlane_d(a, 48);
-                const double e2 = readlane_d(__builtin_fma(bs, e3, a), 32);
-                const double e1 = readlane_d(__builtin_fma(bs, e2, a), 16);
-                const double en = row == 3 ? 0.0 : (row == 2 ? e3 : (row == 1 ? e2 : e1));
-                a = __builtin_fma(bs, en, a);
-                double z = dpp_d<DPP_WAVE_SHL1>(a);
-#pragma unroll
-                for (int i = R - 1; i >= 0; i--) {
-                    z = __builtin_fma(-hh[i], z, x[s][i]);
-                    x[s][i] = z;
+            for (int k = 0; k < 4; k++) {
+                const double f = tab[SC + (j * 8 + k) * 64 + lane];
+                double sa, sb;
+                switch (k) {
+                case 0: sa = dpp_d<DPP_ROW_SHL(1)>(a); sb = dpp_d<DPP_ROW_SHL(1)>(b); break;
+                case 1: sa = dpp_d<DPP_ROW_SHL(2)>(a); sb = dpp_d<DPP_ROW_SHL(2)>(b); break;
+                case 2: sa = dpp_d<DPP_ROW_SHL(4)>(a); sb = dpp_d<DPP_ROW_SHL(4)>(b); break;
+                default: sa = dpp_d<DPP_ROW_SHL(8)>(a); sb = dpp_d<DPP_ROW_SHL(8)>(b); break;
                 }
+                a = __builtin_fma(f, sa, a);
+                b = __builtin_fma(f, sb, b);
+            }
+            {
+                // every row adds the in-row results of the later row starts (lanes 16, 32, 48),
+                // weighted by matrix-only multipliers (zero for rows at or after that start)
+                const double c1 = tab[SC + (j * 8 + 4) * 64 + lane];
+                const double c2 = tab[SC + (j * 8 + 5) * 64 + lane];
+                const double c3 = tab[SC + (j * 8 + 6) * 64 + lane];
+                const double a1 = readlane_d(a, 16), a2 = readlane_d(a, 32), a3 = readlane_d(a, 48);
+                const double b1 = readlane_d(b, 16), b2 = readlane_d(b, 32), b3 = readlane_d(b, 48);
+                a = __builtin_fma(c1, a1, a);
+                b = __builtin_fma(c1, b1, b);
+                a = __builtin_fma(c2, a2, a);
+                b = __builtin_fma(c2, b2, b);
+                a = __builtin_fma(c3, a3, a);
+                b = __builtin_fma(c3, b3, b);
+            }
+            double za = dpp_d<DPP_WAVE_SHL1>(a), zb = dpp_d<DPP_WAVE_SHL1>(b);
+#pragma unroll
+            for (int i = R - 1; i >= 0; i--) {
+                za = __builtin_fma(-hh[i], za, x0[i]);
+                zb = __builtin_fma(-hh[i], zb, x1[i]);
+                x0[i] = za;
+                x1[i] = zb;
             }
         }
 
         // ---- row-0 scaleC (pfSolve_TridiagonalSolve.h:1918-1927) ----
-        if (d.apply_scale && lane == 0) {
-            x[0][0] *= d.scale_hi;
-            x[1][0] *= d.scale_hi;
+        if (d.apply_scale) {
+            x0[0] *= row0_scale;
+            x1[0] *= row0_scale;
         }
 
         // ---- 4. output: blocked -> LDS -> contiguous 16-byte stores ----
         compiler_fence();
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-#pragma unroll
-            for (int m = 0; m < H; m++) {
-                double2 v;
-                v.x = x[s][2 * m];
-                v.y = x[s][2 * m + 1];
-                stg[s * SYSC + swz<H>(lane * H + m)] = v;
-            }
+        for (int m = 0; m < H; m++) {
+            stg[swz<H>(lane * H + m)] = make_double2(x0[2 * m], x0[2 * m + 1]);
+            stg[SYSC + swz<H>(lane * H + m)] = make_double2(x1[2 * m], x1[2 * m + 1]);
         }
         compiler_fence();
 #pragma unroll
         for (int s = 0; s < 2; s++) {
-            const int64_t sys = sys0 + s;
-            if (sys >= d.nsys) continue;
-            double* dst = rhs + sys * d.stride + d.offSol;
-            // the host only selects this kernel when sys*stride + offSol is even and rhs 16-byte aligned
-            double2* dst2 = reinterpret_cast<double2*>(rhs) + ((sys * d.stride + d.offSol) >> 1);
+            if (s == 1 && !ok1) continue;
+            // the host only selects this kernel when sys*stride + offSol is even and rhs is 16-byte aligned
+            double2* dst2 = reinterpret_cast<double2*>(rhs) + (((sys0 + s) * d.stride + d.offSol) >> 1);
 #pragma unroll
             for (int k = 0; k < H; k++) {
                 const int c = 64 * k + lane;
-                const int r = 2 * c;
-                const double2 v = stg[s * SYSC + swz<H>(c)];
-                const bool k0 = r >= d.out_lo && r < d.out_hi;
-                const bool k1 = r + 1 >= d.out_lo && r + 1 < d.out_hi;
-                if (k0 && k1) {
-                    dst2[c] = v;
-                } else {
-                    if (k0) dst[r] = v.x;
-                    if (k1) dst[r + 1] = v.y;
+                if (c >= ocl && c < och) dst2[c] = stg[s * SYSC + swz<H>(c)];
+            }
+        }
+        if (out_edge_lo || out_edge_hi) {
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const bool on = e == 0 ? out_edge_lo : out_edge_hi;
+                const int r = e == 0 ? out_lo : out_hi - 1;
+                if (on && lane == r / R) {
+                    double* dst = rhs + sys0 * d.stride + d.offSol + r;
+                    dst[0] = pick_row<R>(x0, r % R);
+                    if (ok1) dst[d.stride] = pick_row<R>(x1, r % R);
                 }
             }
         }
+        // restore the zeros of the slots the input DMA skips
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+                const int c = swz<H>(64 * k + lane);
+                if (!(c >= icl && c < ich)) stg[s * SYSC + 64 * k + lane] = make_double2(0.0, 0.0);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): LDS traffic done before the next DMA lands
         compiler_fence();
     }
 }
@@ -290,7 +368,7 @@ thread_local char g_err64[256];
 
 extern "C" {
 
-int64_t jwdev_table64_elems(int R) { return 8LL * R * 64 + 15LL * 64; }
+int64_t jwdev_table64_elems(int R) { return JW_TAB64_ELEMS(R); }
 
 /* waves per workgroup: as many as LDS (table + 2 systems of staging per wave)
  * and the kernel's launch bound allow */
