@@ -105,12 +105,117 @@ def cpu_baseline(buf, offs, N, budget_s=10.0):
     }
 
 
+def harmonic_batch_sets(lib, dmat_ptr: int, drhs_ptr: int, L: int = 1024, rhs_complex: int = 32):
+    """Uneven-size workload: degrees l = 0..L-1, N_l = 1024 - floor(l/2), `rhs_complex`
+    complex (2x real, XXII) systems per degree, one V/T/M set per degree drawn from the
+    same distributions as the throughput workload (seed l)."""
+    sets, mats = [], []
+    moff = roff = 0
+    for l in range(L):
+        N = 1024 - l // 2
+        buf, (offV, offT, offM) = gen_fused_matrices(N, 1000 + l, lib.pfsolve_reorder_i0)
+        buf = np.concatenate([buf, np.zeros((-buf.size) % 2)])
+        stride = N + (N % 2)
+        nsys = 2 * rhs_complex
+        sets.append(dict(N=N, nsys=nsys, matrices=dmat_ptr, offsetV=moff + offV, offsetT=moff + offT,
+                         offsetM=moff + offM, rhs=drhs_ptr + 8 * roff, stride=stride, offsetSolution=0,
+                         naturalOrder=0, KL=2, KU=2, scaleC=1.0))
+        mats.append(buf)
+        moff += buf.size
+        roff += nsys * stride
+    return sets, np.concatenate(mats), roff
+
+
+def bench_batch(torch, pf, steps: int, warmup: int) -> dict:
+    """Uneven-size batch: one tuned, graph-captured step (batched prologue + one
+    persistent launch) against one plain plan launch pair per degree."""
+    lib = pf.lib()
+    sets, mats, nrhs = harmonic_batch_sets(lib, 0, 0)
+    dmat = torch.from_numpy(mats).cuda()
+    g = torch.Generator(device="cuda").manual_seed(1)
+    x = torch.rand(nrhs, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    for st in sets:   # offsets -> device pointers
+        st["matrices"] = dmat.data_ptr()
+        st["rhs"] += x.data_ptr()
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    out = {}
+    with pf.Batch(sets) as b:
+        tuned = b.tune(sp, repeats=2)
+        info = b.info()
+        b.capture_graph(sp)
+
+        def timed(fn, k):
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(k + 1)]
+            with torch.cuda.stream(stream):
+                for i in range(k):
+                    evs[i].record(stream)
+                    fn()
+                evs[k].record(stream)
+            stream.synchronize()
+            per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(k)])
+            return float(evs[0].elapsed_time(evs[k]) / k), per
+
+        for _ in range(warmup):
+            b.launch_graph(sp)
+        ms_b, per_b = timed(lambda: b.launch_graph(sp), steps)
+        ms_e, _ = timed(lambda: b.run(sp), max(3, steps // 4))
+        for _ in range(2):
+            b.run_plain(sp)
+        ms_p, per_p = timed(lambda: b.run_plain(sp), max(3, steps // 10))
+        assert torch.isfinite(x).all().item(), "non-finite batch output"
+        systems = info.systems // 2   # complex systems
+        out = {
+            "workload": "1024 degrees, N_l = 1024 - floor(l/2), 32 complex systems (XXII) each, FP64, "
+                        "fused step, one V/T/M set per degree",
+            "complex_systems": int(systems),
+            "batched_graph": {"value": systems / (ms_b * 1e-3), "unit": "systems/s", "ms_per_step": ms_b,
+                              "p50_ms": float(np.percentile(per_b, 50)), "p99_ms": float(np.percentile(per_b, 99)),
+                              "launches": int(info.launches)},
+            "batched_eager_ms": ms_e,
+            "plain_per_size": {"value": systems / (ms_p * 1e-3), "unit": "systems/s", "ms_per_step": ms_p,
+                               "launches": 2 * int(info.sets)},
+            "batched_vs_plain": ms_p / ms_b,
+            "algorithmic_bytes": int(info.algorithmic_bytes),
+            "hbm_frac_of_step": info.algorithmic_bytes / (ms_b * 1e-3) / HBM_PEAK,
+            "tuning": {"waves": list(tuned.waves), "unit_pairs": list(tuned.unit_pairs),
+                       "grid_per_cu": list(tuned.grid_per_cu), "rows_per_lane_shift": list(tuned.rows_per_lane_shift),
+                       "order": "largest N first" if tuned.order == 0 else "as given"},
+        }
+    return out
+
+
+def stream_reference(torch, nbytes: int, reps: int = 20) -> dict:
+    """What plain streaming kernels reach on this GPU for the same byte count: copy_
+    (half read, half written) and fill_ (written only)."""
+    n = nbytes // 8
+    a = torch.empty(n // 2, dtype=torch.float64, device="cuda").uniform_()
+    b = torch.empty(n // 2, dtype=torch.float64, device="cuda")
+    f = torch.empty(n, dtype=torch.float64, device="cuda")
+    out = {}
+    for name, fn in (("copy", lambda: b.copy_(a)), ("fill", lambda: f.fill_(1.0))):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_GBps"] = nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    out["bytes"] = nbytes
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-size batch measurement")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also time torch copy_ / fill_ over the same byte count (streaming reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the JW batch shards across GPUs with no exchange, "
@@ -141,6 +246,9 @@ def main() -> int:
     plan.app.configuration.bufferSize = C.cast(bsize, pf._abi.u64p)
     plan.app.configuration.outputBufferSize = C.cast(osize, pf._abi.u64p)
     geo = plan.geometry()
+    fast = geo.fast_rows_per_lane > 0
+    kname = (f"jw_main64_kernel<{geo.fast_rows_per_lane}>" if fast
+             else f"jw_main_kernel<double,{geo.rows_per_lane},{geo.lanes}>")
     launch = dict(scaleC=SCALE_C, inputZeropad=list(ZP))
 
     for _ in range(args.warmup):
@@ -205,14 +313,16 @@ def main() -> int:
             "workload": "fused JW step y=M^-1 T^-1 (V x), N=512, 65536 complex systems (131072 real, XXII), "
                         "KL=KU=2, scaleC=0.75 (runtime, row 0), inputZeropad=[480,512) (runtime keep-range), FP64",
             "N": N, "complex_systems": N_COMPLEX, "real_systems": nreal, "jw_type": 14,
-            "lanes": geo.lanes, "rows_per_lane": geo.rows_per_lane, "waves_per_workgroup": geo.waves,
+            "lanes": 64 if fast else geo.lanes,
+            "rows_per_lane": geo.fast_rows_per_lane if fast else geo.rows_per_lane,
+            "waves_per_workgroup": geo.fast_waves if fast else geo.waves,
         },
         "p50_ms": float(np.percentile(steps_ms, 50)),
         "p99_ms": float(np.percentile(steps_ms, 99)),
         "wall_s": wall,
         "roofline": {
             "bound": "hbm",
-            "kernel": "jw_main_kernel<double,8,64>",
+            "kernel": kname,
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -222,6 +332,10 @@ def main() -> int:
             "kernel_ms": kavg,
         },
     }
+    if not args.no_batch:
+        result["batch"] = bench_batch(torch, pf, max(10, K), max(3, args.warmup))
+    if args.calibrate:
+        result["stream_reference"] = stream_reference(torch, alg)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(buf, (offV, offT, offM), N)
     print(json.dumps(result))

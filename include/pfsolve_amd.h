@@ -60,6 +60,86 @@ const char* pfsolve_last_error(void);
  * `stop` right after the main JW kernel on the plan's stream (NULL clears). */
 PfSolveResult pfsolve_set_kernel_events(PfSolveApplication* app, hipEvent_t start, hipEvent_t stop);
 
+/* ---- uneven-size batch: many matrix sets (one per harmonic degree), one step ----
+ *
+ * Each set holds the fused-step matrices of one size N (the jw_type 14 layout:
+ * V band-major at offsetV, T [a/b | 1/b | c/b] at offsetT, M [1/d | e/d] at
+ * offsetM, T and M in reorder_i0 order unless naturalOrder) and `nsys` real
+ * systems solved in place.  pfsolve_batch_run turns every set's matrices into
+ * its coefficient table in one launch, then solves all systems in one
+ * persistent launch per row class (N <= 128, 256, 512, 1024): workgroups take
+ * work units (a set and a range of its system pairs) from a queue ordered
+ * largest N first.  pfsolve_batch_run_plain does the same work with one
+ * plain plan launch pair per set, for comparison.  No counterpart in the
+ * reference, which launches one plan per size. */
+typedef struct pfsolve_batch_set {
+    int64_t N;                /* rows, 2 <= N <= 1024 */
+    int64_t nsys;             /* real systems sharing the set's matrices (>= 0) */
+    const void* matrices;     /* device pointer, FP64 */
+    int64_t offsetV, offsetT, offsetM;
+    void* rhs;                /* device pointer, FP64, 16-byte aligned */
+    int64_t stride;           /* elements between systems: >= N and even */
+    int64_t offsetSolution;   /* element offset of system 0: even */
+    int32_t naturalOrder;     /* 1: T and M diagonals in natural order (as jw_type 1014) */
+    int32_t KL, KU;           /* band of V, 0..2 each */
+    int32_t pad;
+    double scaleC;            /* multiplies solution row 0 (1.0: none) */
+} pfsolve_batch_set;
+
+#define PFSOLVE_BATCH_CLASSES 4   /* row classes R = 2, 4, 8, 16 rows per lane */
+
+/* What the autotuner chooses, per row class c (R = 2 << c). 0 = default. */
+typedef struct pfsolve_batch_tuning {
+    int32_t waves[PFSOLVE_BATCH_CLASSES];       /* wavefronts per workgroup (LDS tiling) */
+    int32_t unit_pairs[PFSOLVE_BATCH_CLASSES];  /* system pairs per queue unit */
+    int32_t grid_per_cu[PFSOLVE_BATCH_CLASSES]; /* persistent workgroups per CU */
+    int32_t rows_per_lane_shift[PFSOLVE_BATCH_CLASSES]; /* 1: sets of this class run with 2R rows
+                                                           per lane (half the lanes, one scan level
+                                                           fewer), when 2R <= 16 */
+    int32_t order;            /* 0: largest N first, 1: sets as given */
+    int32_t pad;
+} pfsolve_batch_tuning;
+
+typedef struct pfsolve_batch_info {
+    int64_t sets, systems, units;
+    int64_t table_bytes;      /* device tables of all sets */
+    int64_t algorithmic_bytes;    /* per step: 8 * (N read + N written) per system + 80 N per set */
+    int32_t classes_used;     /* bit c set: row class c has work */
+    int32_t launches;         /* kernel launches per pfsolve_batch_run */
+    pfsolve_batch_tuning tuning;  /* in effect */
+} pfsolve_batch_info;
+
+typedef struct pfsolve_batch pfsolve_batch;
+
+/* Validates the sets, allocates the tables and the work queue (device memory
+ * owned by the batch) and uploads the queue; synchronous.  tuning: NULL uses
+ * the defaults, or the entry of a table loaded with pfsolve_tuning_load. */
+PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set* sets, int64_t nsets,
+                                   const pfsolve_batch_tuning* tuning);
+PfSolveResult pfsolve_batch_run(pfsolve_batch* b, hipStream_t stream);
+PfSolveResult pfsolve_batch_run_plain(pfsolve_batch* b, hipStream_t stream);
+/* Records pfsolve_batch_run on `stream` (not the null stream) into a hipGraph,
+ * then replays it with pfsolve_batch_launch_graph.  The queue counters are
+ * reset by a kernel inside the graph, so every replay is a full step. */
+PfSolveResult pfsolve_batch_capture_graph(pfsolve_batch* b, hipStream_t stream);
+PfSolveResult pfsolve_batch_launch_graph(pfsolve_batch* b, hipStream_t stream);
+/* Autotuner: times every candidate (waves, unit size, grid, rows per lane,
+ * queue order) on scratch right-hand sides of the batch's shapes (the
+ * caller's systems are not touched), keeps the fastest per class, applies it
+ * to the batch and returns it in *best (may be NULL).  `repeats` timed runs
+ * per candidate (>= 1).  Synchronises the stream. */
+PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repeats, pfsolve_batch_tuning* best);
+PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tuning* t);
+PfSolveResult pfsolve_batch_get_info(const pfsolve_batch* b, pfsolve_batch_info* info);
+void pfsolve_batch_destroy(pfsolve_batch* b);
+
+/* Tuning table on disk (text, one line per row class and batch size bucket):
+ * pfsolve_batch_tune results are added to the process-wide table, which
+ * pfsolve_batch_create consults when given no tuning. */
+PfSolveResult pfsolve_tuning_save(const char* path);
+PfSolveResult pfsolve_tuning_load(const char* path);
+void pfsolve_tuning_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

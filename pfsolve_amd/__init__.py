@@ -152,3 +152,67 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """An uneven-size batch of fused JW steps (pfsolve_batch_* in include/pfsolve_amd.h).
+
+    ``sets`` is a list of dicts with the pfsolve_batch_set fields (device
+    pointers as ints): N, nsys, matrices, offsetV, offsetT, offsetM, rhs,
+    stride, offsetSolution, naturalOrder, KL, KU, scaleC.
+    """
+
+    def __init__(self, sets, tuning: _abi.PfSolveBatchTuning | None = None):
+        arr = (_abi.PfSolveBatchSet * len(sets))()
+        for i, st in enumerate(sets):
+            d = {"KL": 2, "KU": 2, "scaleC": 1.0}
+            d.update(st)
+            for k, v in d.items():
+                setattr(arr[i], k, v)
+        self._sets = arr
+        self.handle = C.c_void_p()
+        check(lib().pfsolve_batch_create(C.byref(self.handle), arr, len(sets),
+                                         C.byref(tuning) if tuning is not None else None),
+              "pfsolve_batch_create")
+
+    def run(self, stream: int = 0) -> None:
+        check(lib().pfsolve_batch_run(self.handle, stream), "pfsolve_batch_run")
+
+    def run_plain(self, stream: int = 0) -> None:
+        check(lib().pfsolve_batch_run_plain(self.handle, stream), "pfsolve_batch_run_plain")
+
+    def capture_graph(self, stream: int) -> None:
+        check(lib().pfsolve_batch_capture_graph(self.handle, stream), "pfsolve_batch_capture_graph")
+
+    def launch_graph(self, stream: int) -> None:
+        check(lib().pfsolve_batch_launch_graph(self.handle, stream), "pfsolve_batch_launch_graph")
+
+    def tune(self, stream: int, repeats: int = 3) -> _abi.PfSolveBatchTuning:
+        t = _abi.PfSolveBatchTuning()
+        check(lib().pfsolve_batch_tune(self.handle, stream, repeats, C.byref(t)), "pfsolve_batch_tune")
+        return t
+
+    def set_tuning(self, t: _abi.PfSolveBatchTuning) -> None:
+        check(lib().pfsolve_batch_set_tuning(self.handle, C.byref(t)), "pfsolve_batch_set_tuning")
+
+    def info(self) -> _abi.PfSolveBatchInfo:
+        i = _abi.PfSolveBatchInfo()
+        check(lib().pfsolve_batch_get_info(self.handle, C.byref(i)), "pfsolve_batch_get_info")
+        return i
+
+    def close(self) -> None:
+        if self.handle:
+            lib().pfsolve_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

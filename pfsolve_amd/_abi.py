@@ -227,6 +227,54 @@ class PfSolveGeometry(C.Structure):
     ]
 
 
+BATCH_CLASSES = 4
+
+
+class PfSolveBatchSet(C.Structure):
+    """pfsolve_batch_set (include/pfsolve_amd.h)."""
+    _fields_ = [
+        ("N", i64),
+        ("nsys", i64),
+        ("matrices", vp),
+        ("offsetV", i64),
+        ("offsetT", i64),
+        ("offsetM", i64),
+        ("rhs", vp),
+        ("stride", i64),
+        ("offsetSolution", i64),
+        ("naturalOrder", C.c_int32),
+        ("KL", C.c_int32),
+        ("KU", C.c_int32),
+        ("pad", C.c_int32),
+        ("scaleC", C.c_double),
+    ]
+
+
+class PfSolveBatchTuning(C.Structure):
+    """pfsolve_batch_tuning: per row class c (R = 2 << c)."""
+    _fields_ = [
+        ("waves", C.c_int32 * BATCH_CLASSES),
+        ("unit_pairs", C.c_int32 * BATCH_CLASSES),
+        ("grid_per_cu", C.c_int32 * BATCH_CLASSES),
+        ("rows_per_lane_shift", C.c_int32 * BATCH_CLASSES),
+        ("order", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+class PfSolveBatchInfo(C.Structure):
+    _fields_ = [
+        ("sets", i64),
+        ("systems", i64),
+        ("units", i64),
+        ("table_bytes", i64),
+        ("algorithmic_bytes", i64),
+        ("classes_used", C.c_int32),
+        ("launches", C.c_int32),
+        ("tuning", PfSolveBatchTuning),
+    ]
+
+
 def declare(lib: C.CDLL) -> None:
     """Argument and result types of every exported entry point."""
     P = C.POINTER
@@ -263,3 +311,25 @@ def declare(lib: C.CDLL) -> None:
 def declare_ext(lib: C.CDLL) -> None:
     lib.pfsolve_set_kernel_events.argtypes = [C.POINTER(PfSolveApplication), C.c_void_p, C.c_void_p]
     lib.pfsolve_set_kernel_events.restype = C.c_int
+    P = C.POINTER
+    lib.pfsolve_batch_create.argtypes = [P(vp), P(PfSolveBatchSet), i64, P(PfSolveBatchTuning)]
+    lib.pfsolve_batch_create.restype = C.c_int
+    for name in ("pfsolve_batch_run", "pfsolve_batch_run_plain", "pfsolve_batch_capture_graph",
+                 "pfsolve_batch_launch_graph"):
+        f = getattr(lib, name)
+        f.argtypes = [vp, vp]
+        f.restype = C.c_int
+    lib.pfsolve_batch_tune.argtypes = [vp, vp, C.c_int, P(PfSolveBatchTuning)]
+    lib.pfsolve_batch_tune.restype = C.c_int
+    lib.pfsolve_batch_set_tuning.argtypes = [vp, P(PfSolveBatchTuning)]
+    lib.pfsolve_batch_set_tuning.restype = C.c_int
+    lib.pfsolve_batch_get_info.argtypes = [vp, P(PfSolveBatchInfo)]
+    lib.pfsolve_batch_get_info.restype = C.c_int
+    lib.pfsolve_batch_destroy.argtypes = [vp]
+    lib.pfsolve_batch_destroy.restype = None
+    lib.pfsolve_tuning_save.argtypes = [C.c_char_p]
+    lib.pfsolve_tuning_save.restype = C.c_int
+    lib.pfsolve_tuning_load.argtypes = [C.c_char_p]
+    lib.pfsolve_tuning_load.restype = C.c_int
+    lib.pfsolve_tuning_clear.argtypes = []
+    lib.pfsolve_tuning_clear.restype = None

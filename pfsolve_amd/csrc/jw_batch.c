@@ -1,0 +1,683 @@
+/*
+ * jw_batch.c -- uneven-size batches of fused JW steps (pfsolve_amd.h).
+ *
+ * A batch is a list of matrix sets of different sizes N (one per harmonic
+ * degree in a spectral code) with their right-hand sides.  The reference
+ * solves such a batch with one plan per size (one JIT-compiled kernel launch
+ * per N, PfSolve/pfSolve_AppManagement/pfSolve_RunApp.h:80-159 per plan).
+ * Here one step is:
+ *   1. jw_prologue_batch_kernel: one workgroup per set writes the set's
+ *      coefficient table (format 2) and zeroes the queue counters;
+ *   2. per row class (R rows per lane, N <= 64 R): one persistent launch of
+ *      jw_batch64_kernel over that class's work units.
+ * Work units are (set, range of system pairs), listed largest N first, so
+ * the long units start early and the short ones fill the tail.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "jw_plan.h"
+
+#define NCLS PFSOLVE_BATCH_CLASSES
+
+struct pfsolve_batch {
+    int64_t nsets;
+    pfsolve_batch_set* sets;          /* host copy */
+    int* cls;                         /* row class of each set (R = 2 << cls) */
+    int cu_count;
+    /* device memory */
+    void* tables;                     /* one table slot per set, sized for 2R rows per lane */
+    int64_t* table_off;               /* host: byte offset of each set's slot */
+    int64_t table_bytes;
+    jw_prologue_job* d_jobs;
+    jw_batch_item* d_items;
+    jw_batch_unit* d_units;           /* units of all launch groups, back to back */
+    int* d_counters;                  /* one queue counter per launch group */
+    /* launch groups: sets run with R_eff = 2 << g rows per lane */
+    int64_t unit_first[NCLS], unit_count[NCLS];
+    int group_waves[NCLS], group_grid[NCLS];
+    int64_t total_units, total_systems;
+    pfsolve_batch_tuning tuning;
+    int max_RL;
+    /* captured step */
+    hipGraph_t graph;
+    hipGraphExec_t graph_exec;
+};
+
+/* ---- tuning table (process-wide) ---- */
+typedef struct tune_entry {
+    int cls, bucket;
+    int waves, unit_pairs, grid_per_cu, shift, order;
+} tune_entry;
+static tune_entry* g_tune;
+static int g_ntune;
+
+static int pairs_bucket(int64_t pairs)
+{
+    int b = 0;
+    while (b < 40 && ((int64_t)1 << (b + 1)) <= pairs) b++;
+    return b;
+}
+
+static void tune_put(int cls, int bucket, const pfsolve_batch_tuning* t)
+{
+    for (int i = 0; i < g_ntune; i++) {
+        if (g_tune[i].cls == cls && g_tune[i].bucket == bucket) {
+            g_tune[i].waves = t->waves[cls];
+            g_tune[i].unit_pairs = t->unit_pairs[cls];
+            g_tune[i].grid_per_cu = t->grid_per_cu[cls];
+            g_tune[i].shift = t->rows_per_lane_shift[cls];
+            g_tune[i].order = t->order;
+            return;
+        }
+    }
+    tune_entry* n = (tune_entry*)realloc(g_tune, sizeof(tune_entry) * (size_t)(g_ntune + 1));
+    if (!n) return;
+    g_tune = n;
+    tune_entry e = {cls, bucket, t->waves[cls], t->unit_pairs[cls], t->grid_per_cu[cls], t->rows_per_lane_shift[cls],
+                    t->order};
+    g_tune[g_ntune++] = e;
+}
+
+static const tune_entry* tune_get(int cls, int bucket)
+{
+    const tune_entry* best = NULL;
+    int bd = 1 << 30;
+    for (int i = 0; i < g_ntune; i++) {
+        if (g_tune[i].cls != cls) continue;
+        const int dd = abs(g_tune[i].bucket - bucket);
+        if (dd < bd) { bd = dd; best = &g_tune[i]; }
+    }
+    return best;
+}
+
+void pfsolve_tuning_clear(void)
+{
+    free(g_tune);
+    g_tune = NULL;
+    g_ntune = 0;
+}
+
+PfSolveResult pfsolve_tuning_save(const char* path)
+{
+    if (!path) return PFSOLVE_ERROR_EMPTY_FILE;
+    FILE* f = fopen(path, "w");
+    if (!f) return PFSOLVE_ERROR_EMPTY_FILE;
+    fprintf(f, "# pfsolve_amd batch tuning: class rows_per_lane pairs_log2 waves unit_pairs grid_per_cu shift order\n");
+    for (int i = 0; i < g_ntune; i++)
+        fprintf(f, "%d %d %d %d %d %d %d %d\n", g_tune[i].cls, 2 << g_tune[i].cls, g_tune[i].bucket, g_tune[i].waves,
+                g_tune[i].unit_pairs, g_tune[i].grid_per_cu, g_tune[i].shift, g_tune[i].order);
+    fclose(f);
+    return PFSOLVE_SUCCESS;
+}
+
+PfSolveResult pfsolve_tuning_load(const char* path)
+{
+    if (!path) return PFSOLVE_ERROR_EMPTY_FILE;
+    FILE* f = fopen(path, "r");
+    if (!f) return PFSOLVE_ERROR_EMPTY_FILE;
+    char line[256];
+    int bad = 0;
+    while (fgets(line, sizeof(line), f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int c, r, b, w, u, g, s, o;
+        if (sscanf(line, "%d %d %d %d %d %d %d %d", &c, &r, &b, &w, &u, &g, &s, &o) != 8 || c < 0 || c >= NCLS ||
+            r != (2 << c) || w < 0 || u < 0 || g < 0 || s < 0 || s > 1 || o < 0 || o > 1) {
+            bad = 1;
+            continue;
+        }
+        pfsolve_batch_tuning t;
+        memset(&t, 0, sizeof(t));
+        t.waves[c] = w;
+        t.unit_pairs[c] = u;
+        t.grid_per_cu[c] = g;
+        t.rows_per_lane_shift[c] = s;
+        t.order = o;
+        tune_put(c, b, &t);
+    }
+    fclose(f);
+    return bad ? PFSOLVE_ERROR_EMPTY_FILE : PFSOLVE_SUCCESS;
+}
+
+/* ---- helpers ---- */
+static int set_class(int64_t N)
+{
+    int c = 0;
+    while (c < NCLS - 1 && (int64_t)(128 << c) < N) c++;
+    return c;   /* N <= 64 * (2 << c) */
+}
+
+static int eff_group(const pfsolve_batch* b, int64_t s)
+{
+    int g = b->cls[s] + (b->tuning.rows_per_lane_shift[b->cls[s]] ? 1 : 0);
+    return g < NCLS ? g : NCLS - 1;
+}
+
+static void source_of(const pfsolve_batch_set* st, jw_source_desc* src)
+{
+    memset(src, 0, sizeof(*src));
+    src->N = st->N;
+    src->band_kind = JW_BAND_GB;
+    src->offV = st->offsetV;
+    src->KL = st->KL;
+    src->KU = st->KU;
+    src->solve_kind = JW_SOLVE_TRI;
+    src->ldS = st->offsetT;
+    src->mdS = st->offsetT + st->N;
+    src->udS = st->offsetT + 2 * st->N;
+    src->has_m = 1;
+    src->mdM = st->offsetM;
+    src->udM = st->offsetM + st->N;
+    src->Msplit0 = 704;
+    /* the reference's JW plans read permuted storage except on their N <= 64 path */
+    src->permuted = !st->naturalOrder && st->N > 64;
+}
+
+static void item_of(const pfsolve_batch_set* st, const void* table, void* rhs, jw_batch_item* it)
+{
+    memset(it, 0, sizeof(*it));
+    it->rhs = rhs;
+    it->table = table;
+    it->nsys = st->nsys;
+    it->stride = st->stride;
+    it->offSol = st->offsetSolution;
+    it->N = (int32_t)st->N;
+    it->kl = st->KL;
+    it->ku = st->KU;
+    it->apply_scale = st->scaleC != 1.0;
+    it->scale = st->scaleC;
+}
+
+static int group_waves_default(int g)
+{
+    return jwdev_main64_waves(2 << g);
+}
+
+typedef struct n_index {
+    int64_t N, idx;
+} n_index;
+
+/* largest N first; equal N keep their given order */
+static int cmp_n_desc(const void* a, const void* c)
+{
+    const n_index* x = (const n_index*)a;
+    const n_index* y = (const n_index*)c;
+    if (x->N != y->N) return x->N > y->N ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* fills the units (host, grouped by effective row class, queue order inside a
+ * group) and the launch shapes from b->tuning */
+static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
+{
+    const int64_t S = b->nsets;
+    int64_t* order = (int64_t*)malloc(sizeof(int64_t) * (size_t)(S > 0 ? S : 1));
+    n_index* keys = (n_index*)malloc(sizeof(n_index) * (size_t)(S > 0 ? S : 1));
+    if (!order || !keys) { free(order); free(keys); return PFSOLVE_ERROR_MALLOC_FAILED; }
+    for (int64_t i = 0; i < S; i++) { keys[i].N = b->sets[i].N; keys[i].idx = i; }
+    if (b->tuning.order == 0 && S > 1) qsort(keys, (size_t)S, sizeof(n_index), cmp_n_desc);
+    for (int64_t i = 0; i < S; i++) order[i] = keys[i].idx;
+    free(keys);
+    /* count units per group */
+    int64_t cnt[NCLS] = {0};
+    for (int64_t k = 0; k < S; k++) {
+        const int64_t s = order[k];
+        const int g = eff_group(b, s);
+        const int64_t pairs = (b->sets[s].nsys + 1) / 2;
+        int64_t up = b->tuning.unit_pairs[b->cls[s]];
+        if (up <= 0) up = pairs > 0 ? pairs : 1;
+        cnt[g] += (pairs + up - 1) / up;
+    }
+    int64_t total = 0;
+    for (int g = 0; g < NCLS; g++) {
+        b->unit_first[g] = total;
+        b->unit_count[g] = cnt[g];
+        total += cnt[g];
+    }
+    if (total > 0x7fffffff) { free(order); return PFSOLVE_ERROR_EMPTY_size; }
+    jw_batch_unit* u = (jw_batch_unit*)calloc((size_t)(total > 0 ? total : 1), sizeof(jw_batch_unit));
+    if (!u) { free(order); return PFSOLVE_ERROR_MALLOC_FAILED; }
+    int64_t fill[NCLS];
+    for (int g = 0; g < NCLS; g++) fill[g] = b->unit_first[g];
+    for (int64_t k = 0; k < S; k++) {
+        const int64_t s = order[k];
+        const int g = eff_group(b, s);
+        const int64_t pairs = (b->sets[s].nsys + 1) / 2;
+        int64_t up = b->tuning.unit_pairs[b->cls[s]];
+        if (up <= 0) up = pairs > 0 ? pairs : 1;
+        for (int64_t p = 0; p < pairs; p += up) {
+            jw_batch_unit* x = &u[fill[g]++];
+            x->item = (int32_t)s;
+            x->pair0 = (int32_t)p;
+            x->pair1 = (int32_t)(p + up < pairs ? p + up : pairs);
+        }
+    }
+    free(order);
+    b->total_units = total;
+    *out_units = u;
+    /* launch shapes */
+    for (int g = 0; g < NCLS; g++) {
+        b->group_waves[g] = 0;
+        b->group_grid[g] = 0;
+        if (!b->unit_count[g]) continue;
+        /* the tuning of the native class that feeds this group wins; default otherwise */
+        int w = 0, gpc = 0;
+        for (int c = 0; c < NCLS; c++) {
+            const int ge = c + (b->tuning.rows_per_lane_shift[c] ? 1 : 0);
+            if ((ge < NCLS ? ge : NCLS - 1) != g) continue;
+            if (b->tuning.waves[c] > w) w = b->tuning.waves[c];
+            if (b->tuning.grid_per_cu[c] > gpc) gpc = b->tuning.grid_per_cu[c];
+        }
+        const int wmax = group_waves_default(g);
+        if (w <= 0 || w > wmax) w = wmax;
+        if (w < 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        const int occ = jwdev_batch64_occupancy(2 << g, w);
+        if (occ < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
+        if (gpc <= 0 || gpc > occ) gpc = occ;
+        int64_t grid = (int64_t)b->cu_count * gpc;
+        if (grid > b->unit_count[g]) grid = b->unit_count[g];
+        b->group_waves[g] = w;
+        b->group_grid[g] = (int)(grid > 0 ? grid : 1);
+    }
+    return PFSOLVE_SUCCESS;
+}
+
+static PfSolveResult upload_queue(pfsolve_batch* b)
+{
+    jw_batch_unit* u = NULL;
+    PfSolveResult r = build_queue(b, &u);
+    if (r != PFSOLVE_SUCCESS) return r;
+    if (b->d_units) { hipFree(b->d_units); b->d_units = NULL; }
+    const size_t bytes = sizeof(jw_batch_unit) * (size_t)(b->total_units > 0 ? b->total_units : 1);
+    if (hipMalloc((void**)&b->d_units, bytes) != hipSuccess) { free(u); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
+    hipError_t e = hipMemcpy(b->d_units, u, bytes, hipMemcpyHostToDevice);
+    free(u);
+    if (e != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_COPY;
+    /* the prologue jobs carry the effective rows per lane of each set */
+    jw_prologue_job* jobs = (jw_prologue_job*)calloc((size_t)(b->nsets > 0 ? b->nsets : 1), sizeof(jw_prologue_job));
+    if (!jobs) return PFSOLVE_ERROR_MALLOC_FAILED;
+    b->max_RL = 0;
+    for (int64_t s = 0; s < b->nsets; s++) {
+        source_of(&b->sets[s], &jobs[s].src);
+        jobs[s].matrices = b->sets[s].matrices;
+        jobs[s].table = (char*)b->tables + b->table_off[s];
+        jobs[s].L = 64;
+        jobs[s].R = 2 << eff_group(b, s);
+        jobs[s].format = 2;
+        if (64 * jobs[s].R > b->max_RL) b->max_RL = 64 * jobs[s].R;
+    }
+    e = hipMemcpy(b->d_jobs, jobs, sizeof(jw_prologue_job) * (size_t)b->nsets, hipMemcpyHostToDevice);
+    free(jobs);
+    if (e != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_COPY;
+    if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = NULL; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = NULL; }
+    return PFSOLVE_SUCCESS;
+}
+
+static PfSolveResult upload_items(pfsolve_batch* b, jw_batch_item* d_items, void* const* rhs_override)
+{
+    jw_batch_item* it = (jw_batch_item*)calloc((size_t)(b->nsets > 0 ? b->nsets : 1), sizeof(jw_batch_item));
+    if (!it) return PFSOLVE_ERROR_MALLOC_FAILED;
+    for (int64_t s = 0; s < b->nsets; s++)
+        item_of(&b->sets[s], (char*)b->tables + b->table_off[s], rhs_override ? rhs_override[s] : b->sets[s].rhs,
+                &it[s]);
+    hipError_t e = hipMemcpy(d_items, it, sizeof(jw_batch_item) * (size_t)b->nsets, hipMemcpyHostToDevice);
+    free(it);
+    return e == hipSuccess ? PFSOLVE_SUCCESS : PFSOLVE_ERROR_FAILED_TO_COPY;
+}
+
+void pfsolve_batch_destroy(pfsolve_batch* b)
+{
+    if (!b) return;
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->tables) hipFree(b->tables);
+    if (b->d_jobs) hipFree(b->d_jobs);
+    if (b->d_items) hipFree(b->d_items);
+    if (b->d_units) hipFree(b->d_units);
+    if (b->d_counters) hipFree(b->d_counters);
+    free(b->sets);
+    free(b->cls);
+    free(b->table_off);
+    free(b);
+}
+
+static PfSolveResult check_set(const pfsolve_batch_set* s)
+{
+    if (s->N < 2 || s->N > 1024) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+    if (s->nsys < 0) return PFSOLVE_ERROR_EMPTY_size;
+    if (s->KL < 0 || s->KL > 2 || s->KU < 0 || s->KU > 2) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+    if (!s->matrices) return PFSOLVE_ERROR_EMPTY_buffer;
+    if (s->offsetV < 0 || s->offsetT < 0 || s->offsetM < 0) return PFSOLVE_ERROR_EMPTY_bufferSize;
+    if (s->nsys == 0) return PFSOLVE_SUCCESS;
+    if (!s->rhs) return PFSOLVE_ERROR_EMPTY_outputBuffer;
+    if (((uintptr_t)s->rhs % 16) != 0 || s->stride < s->N || (s->stride % 2) != 0 || s->offsetSolution < 0 ||
+        (s->offsetSolution % 2) != 0)
+        return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+    return PFSOLVE_SUCCESS;
+}
+
+PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set* sets, int64_t nsets,
+                                   const pfsolve_batch_tuning* tuning)
+{
+    if (!out) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    *out = NULL;
+    if (nsets < 1 || !sets) return PFSOLVE_ERROR_EMPTY_size;
+    if (nsets > 0x7fffffff) return PFSOLVE_ERROR_EMPTY_size;
+    for (int64_t s = 0; s < nsets; s++) {
+        PfSolveResult r = check_set(&sets[s]);
+        if (r != PFSOLVE_SUCCESS) return r;
+    }
+    pfsolve_batch* b = (pfsolve_batch*)calloc(1, sizeof(*b));
+    if (!b) return PFSOLVE_ERROR_MALLOC_FAILED;
+    b->nsets = nsets;
+    b->sets = (pfsolve_batch_set*)malloc(sizeof(pfsolve_batch_set) * (size_t)nsets);
+    b->cls = (int*)malloc(sizeof(int) * (size_t)nsets);
+    b->table_off = (int64_t*)malloc(sizeof(int64_t) * (size_t)nsets);
+    if (!b->sets || !b->cls || !b->table_off) { pfsolve_batch_destroy(b); return PFSOLVE_ERROR_MALLOC_FAILED; }
+    memcpy(b->sets, sets, sizeof(pfsolve_batch_set) * (size_t)nsets);
+    int lds = 0, maj = 0, min = 0;
+    if (jwdev_device_info(&b->cu_count, &lds, &maj, &min) != 0) { pfsolve_batch_destroy(b); return PFSOLVE_ERROR_INVALID_DEVICE; }
+    int64_t off = 0, pairs_cls[NCLS] = {0};
+    for (int64_t s = 0; s < nsets; s++) {
+        b->cls[s] = set_class(sets[s].N);
+        b->table_off[s] = off;
+        int rmax = 2 << b->cls[s];
+        if (rmax < 16) rmax *= 2;   /* room for the doubled rows-per-lane variant */
+        off += ((int64_t)jwdev_table64_elems(rmax) * 8 + 255) / 256 * 256;
+        b->total_systems += sets[s].nsys;
+        pairs_cls[b->cls[s]] += (sets[s].nsys + 1) / 2;
+    }
+    b->table_bytes = off;
+    if (hipMalloc(&b->tables, (size_t)off) != hipSuccess ||
+        hipMalloc((void**)&b->d_jobs, sizeof(jw_prologue_job) * (size_t)nsets) != hipSuccess ||
+        hipMalloc((void**)&b->d_items, sizeof(jw_batch_item) * (size_t)nsets) != hipSuccess ||
+        hipMalloc((void**)&b->d_counters, sizeof(int) * NCLS) != hipSuccess) {
+        pfsolve_batch_destroy(b);
+        return PFSOLVE_ERROR_FAILED_TO_ALLOCATE;
+    }
+    if (tuning) {
+        b->tuning = *tuning;
+    } else {
+        memset(&b->tuning, 0, sizeof(b->tuning));
+        for (int c = 0; c < NCLS; c++) {
+            if (!pairs_cls[c]) continue;
+            const tune_entry* e = tune_get(c, pairs_bucket(pairs_cls[c]));
+            if (!e) continue;
+            b->tuning.waves[c] = e->waves;
+            b->tuning.unit_pairs[c] = e->unit_pairs;
+            b->tuning.grid_per_cu[c] = e->grid_per_cu;
+            b->tuning.rows_per_lane_shift[c] = e->shift;
+            b->tuning.order = e->order;
+        }
+    }
+    for (int c = 0; c < NCLS; c++)
+        if (b->tuning.rows_per_lane_shift[c] && c == NCLS - 1) b->tuning.rows_per_lane_shift[c] = 0;
+    PfSolveResult r = upload_queue(b);
+    if (r == PFSOLVE_SUCCESS) r = upload_items(b, b->d_items, NULL);
+    if (r != PFSOLVE_SUCCESS) { pfsolve_batch_destroy(b); return r; }
+    *out = b;
+    return PFSOLVE_SUCCESS;
+}
+
+PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tuning* t)
+{
+    if (!b || !t) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    for (int c = 0; c < NCLS; c++)
+        if (t->waves[c] < 0 || t->unit_pairs[c] < 0 || t->grid_per_cu[c] < 0 || t->rows_per_lane_shift[c] < 0 ||
+            t->rows_per_lane_shift[c] > 1)
+            return PFSOLVE_ERROR_EMPTY_size;
+    if (t->order < 0 || t->order > 1) return PFSOLVE_ERROR_EMPTY_size;
+    b->tuning = *t;
+    b->tuning.rows_per_lane_shift[NCLS - 1] = 0;
+    return upload_queue(b);
+}
+
+static PfSolveResult launch_step(pfsolve_batch* b, const jw_batch_item* d_items, hipStream_t stream, int only_group,
+                                 int with_prologue)
+{
+    if (with_prologue &&
+        jwdev_prologue_batch(b->d_jobs, (int)b->nsets, b->max_RL, JW_PREC_F64, b->d_counters, NCLS, stream) != 0)
+        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    for (int g = 0; g < NCLS; g++) {
+        if (!b->unit_count[g] || (only_group >= 0 && g != only_group)) continue;
+        if (jwdev_batch64(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g], b->d_counters + g, 2 << g,
+                          b->group_waves[g], b->group_grid[g], stream) != 0)
+            return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    }
+    return PFSOLVE_SUCCESS;
+}
+
+PfSolveResult pfsolve_batch_run(pfsolve_batch* b, hipStream_t stream)
+{
+    if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    return launch_step(b, b->d_items, stream, -1, 1);
+}
+
+PfSolveResult pfsolve_batch_run_plain(pfsolve_batch* b, hipStream_t stream)
+{
+    if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    for (int64_t s = 0; s < b->nsets; s++) {
+        const pfsolve_batch_set* st = &b->sets[s];
+        const int R = 2 << b->cls[s];
+        void* table = (char*)b->tables + b->table_off[s];
+        jw_source_desc src;
+        source_of(st, &src);
+        if (jwdev_prologue(&src, st->matrices, table, 64, R, JW_PREC_F64, 2, stream) != 0)
+            return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+        if (st->nsys <= 0) continue;
+        jw_run_desc run;
+        memset(&run, 0, sizeof(run));
+        run.rhs = st->rhs;
+        run.table = table;
+        run.nsys = st->nsys;
+        run.stride = st->stride;
+        run.offSol = st->offsetSolution;
+        run.in_lo = 0;
+        run.in_hi = st->N;
+        run.out_lo = 0;
+        run.out_hi = st->N;
+        run.N = (int32_t)st->N;
+        run.band = 1;
+        run.kl = st->KL;
+        run.ku = st->KU;
+        run.fwd = run.b1 = run.b2 = 1;
+        run.apply_scale = st->scaleC != 1.0;
+        run.scale_hi = st->scaleC;
+        run.vec2 = 1;
+        const int waves = jwdev_main64_waves(R);
+        const int occ = jwdev_main64_occupancy(R, waves);
+        if (waves < 1 || occ < 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        const int64_t npass = (st->nsys + 1) / 2;
+        int64_t grid = (npass + waves - 1) / waves;
+        if (grid > (int64_t)b->cu_count * occ) grid = (int64_t)b->cu_count * occ;
+        if (jwdev_main64(&run, R, waves, (int)grid, stream) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    }
+    return PFSOLVE_SUCCESS;
+}
+
+PfSolveResult pfsolve_batch_capture_graph(pfsolve_batch* b, hipStream_t stream)
+{
+    if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    if (!stream) return PFSOLVE_ERROR_INVALID_QUEUE;
+    if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = NULL; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = NULL; }
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    PfSolveResult r = launch_step(b, b->d_items, stream, -1, 1);
+    hipGraph_t g = NULL;
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (r != PFSOLVE_SUCCESS) { if (g) hipGraphDestroy(g); return r; }
+    if (e != hipSuccess || !g) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    b->graph = g;
+    if (hipGraphInstantiate(&b->graph_exec, g, NULL, NULL, 0) != hipSuccess) {
+        hipGraphDestroy(g);
+        b->graph = NULL;
+        b->graph_exec = NULL;
+        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    }
+    return PFSOLVE_SUCCESS;
+}
+
+PfSolveResult pfsolve_batch_launch_graph(pfsolve_batch* b, hipStream_t stream)
+{
+    if (!b || !b->graph_exec) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    return hipGraphLaunch(b->graph_exec, stream) == hipSuccess ? PFSOLVE_SUCCESS : PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+}
+
+PfSolveResult pfsolve_batch_get_info(const pfsolve_batch* b, pfsolve_batch_info* info)
+{
+    if (!b || !info) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    memset(info, 0, sizeof(*info));
+    info->sets = b->nsets;
+    info->systems = b->total_systems;
+    info->units = b->total_units;
+    info->table_bytes = b->table_bytes;
+    int64_t alg = 0;
+    for (int64_t s = 0; s < b->nsets; s++) alg += 16 * b->sets[s].N * b->sets[s].nsys + 80 * b->sets[s].N;
+    info->algorithmic_bytes = alg;
+    int launches = 1;
+    for (int g = 0; g < NCLS; g++)
+        if (b->unit_count[g]) { info->classes_used |= 1 << g; launches++; }
+    info->launches = launches;
+    info->tuning = b->tuning;
+    return PFSOLVE_SUCCESS;
+}
+
+/* ---- autotuner ---- */
+static int time_group(pfsolve_batch* b, const jw_batch_item* d_items, hipStream_t stream, int g, int repeats,
+                      float* ms_out)
+{
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -1;
+    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return -1; }
+    float best = 1e30f;
+    int rc = 0;
+    /* warm-up (also writes the tables) */
+    if (launch_step(b, d_items, stream, g, 1) != PFSOLVE_SUCCESS) rc = -1;
+    for (int k = 0; rc == 0 && k < repeats; k++) {
+        /* the prologue resets the queue counters; only the solve is timed */
+        if (jwdev_prologue_batch(b->d_jobs, (int)b->nsets, b->max_RL, JW_PREC_F64, b->d_counters, NCLS, stream) != 0) {
+            rc = -1;
+            break;
+        }
+        hipEventRecord(e0, stream);
+        if (launch_step(b, d_items, stream, g, 0) != PFSOLVE_SUCCESS) { rc = -1; break; }
+        hipEventRecord(e1, stream);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) best = ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms_out = best;
+    return rc;
+}
+
+PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repeats, pfsolve_batch_tuning* best_out)
+{
+    if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    if (repeats < 1) repeats = 1;
+    /* scratch right-hand sides with the batch's shapes */
+    int64_t total = 0;
+    int64_t* roff = (int64_t*)malloc(sizeof(int64_t) * (size_t)b->nsets);
+    void** rptr = (void**)malloc(sizeof(void*) * (size_t)b->nsets);
+    if (!roff || !rptr) { free(roff); free(rptr); return PFSOLVE_ERROR_MALLOC_FAILED; }
+    for (int64_t s = 0; s < b->nsets; s++) {
+        const pfsolve_batch_set* st = &b->sets[s];
+        roff[s] = total;
+        const int64_t n = st->nsys > 0 ? (st->nsys - 1) * st->stride + st->offsetSolution + st->N : 0;
+        total += (n + 1) / 2 * 2;
+    }
+    double* scratch = NULL;
+    jw_batch_item* d_items = NULL;
+    PfSolveResult res = PFSOLVE_SUCCESS;
+    if (hipMalloc((void**)&scratch, (size_t)(total > 0 ? total : 2) * 8) != hipSuccess ||
+        hipMalloc((void**)&d_items, sizeof(jw_batch_item) * (size_t)b->nsets) != hipSuccess) {
+        res = PFSOLVE_ERROR_FAILED_TO_ALLOCATE;
+        goto done;
+    }
+    if (hipMemsetAsync(scratch, 0, (size_t)(total > 0 ? total : 2) * 8, stream) != hipSuccess) {
+        res = PFSOLVE_ERROR_FAILED_TO_COPY;
+        goto done;
+    }
+    for (int64_t s = 0; s < b->nsets; s++) rptr[s] = scratch + roff[s];
+    res = upload_items(b, d_items, rptr);
+    if (res != PFSOLVE_SUCCESS) goto done;
+    {
+        const pfsolve_batch_tuning start = b->tuning;
+        pfsolve_batch_tuning cur = start;
+        int64_t pairs_cls[NCLS] = {0};
+        for (int64_t s = 0; s < b->nsets; s++) pairs_cls[b->cls[s]] += (b->sets[s].nsys + 1) / 2;
+        static const int units_try[] = {0, 1, 2, 4, 8, 16, 32, 64};
+        for (int c = 0; c < NCLS && res == PFSOLVE_SUCCESS; c++) {
+            if (!pairs_cls[c]) continue;
+            float best_ms = 1e30f;
+            pfsolve_batch_tuning best = cur;
+            const int shifts = c < NCLS - 1 ? 2 : 1;
+            for (int sh = 0; sh < shifts && res == PFSOLVE_SUCCESS; sh++) {
+                const int g = c + sh;
+                const int wmax = group_waves_default(g);
+                for (int w = 1; w <= wmax && res == PFSOLVE_SUCCESS; w++) {
+                    /* a few wave counts: the largest, its half and quarter */
+                    if (w != wmax && w != (wmax + 1) / 2 && w != (wmax + 3) / 4) continue;
+                    const int occ = jwdev_batch64_occupancy(2 << g, w);
+                    for (int gpc = 1; gpc <= occ && gpc <= 2; gpc++) {
+                        for (size_t ui = 0; ui < sizeof(units_try) / sizeof(units_try[0]); ui++) {
+                            pfsolve_batch_tuning t = cur;
+                            t.rows_per_lane_shift[c] = sh;
+                            t.waves[c] = w;
+                            t.grid_per_cu[c] = gpc;
+                            t.unit_pairs[c] = units_try[ui];
+                            /* the whole solve is timed: a class moved to the next group shares
+                             * that launch with the next class's sets */
+                            b->tuning = t;
+                            res = upload_queue(b);
+                            if (res != PFSOLVE_SUCCESS) break;
+                            float ms = 0.f;
+                            if (time_group(b, d_items, stream, -1, repeats, &ms) != 0) {
+                                res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+                                break;
+                            }
+                            if (ms < best_ms) { best_ms = ms; best = t; }
+                        }
+                        if (res != PFSOLVE_SUCCESS) break;
+                    }
+                }
+            }
+            cur = best;
+        }
+        if (res == PFSOLVE_SUCCESS) {
+            /* queue order: largest N first vs as given, on the whole step */
+            float ms_order[2] = {1e30f, 1e30f};
+            for (int o = 0; o < 2 && res == PFSOLVE_SUCCESS; o++) {
+                pfsolve_batch_tuning t = cur;
+                t.order = o;
+                b->tuning = t;
+                res = upload_queue(b);
+                if (res != PFSOLVE_SUCCESS) break;
+                if (time_group(b, d_items, stream, -1, repeats, &ms_order[o]) != 0) res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            }
+            cur.order = ms_order[1] < ms_order[0] ? 1 : 0;
+        }
+        b->tuning = res == PFSOLVE_SUCCESS ? cur : start;
+        PfSolveResult r2 = upload_queue(b);
+        if (res == PFSOLVE_SUCCESS) res = r2;
+        if (res == PFSOLVE_SUCCESS) {
+            for (int c = 0; c < NCLS; c++)
+                if (pairs_cls[c]) tune_put(c, pairs_bucket(pairs_cls[c]), &cur);
+            if (best_out) *best_out = cur;
+        }
+    }
+done:
+    if (stream) hipStreamSynchronize(stream);
+    else hipDeviceSynchronize();
+    if (d_items) hipFree(d_items);
+    if (scratch) hipFree(scratch);
+    free(roff);
+    free(rptr);
+    return res;
+}

@@ -92,7 +92,6 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
     T* sl = sE + RL;
     T* sg = sl + RL;
     T* sh = sg + RL;
-    T* sB = sh + RL;                           // per-lane multipliers (3 x 64)
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int64_t N = d.N;

@@ -82,14 +82,21 @@ typedef struct jw_prologue_job {
     int32_t L, R, format, pad;
 } jw_prologue_job;
 
-/* Uneven-size batch: one work item per matrix set. */
+/* Uneven-size batch: one item per matrix set, and work units (a range of
+ * the set's system pairs) in queue order. */
 typedef struct jw_batch_item {
-    void* rhs;
-    const void* table;
+    void* rhs;                /* systems of this set, solved in place */
+    const void* table;        /* the set's format-2 table */
     int64_t nsys, stride, offSol;
-    int32_t N, L, R;
-    int32_t table_bytes_lds;  /* bytes of table staged into LDS */
+    int32_t N, kl, ku, apply_scale;
+    double scale;             /* row-0 scaleC */
 } jw_batch_item;
+
+typedef struct jw_batch_unit {
+    int32_t item;             /* index into the item array */
+    int32_t pair0, pair1;     /* system pairs [pair0, pair1) of the item */
+    int32_t pad;
+} jw_batch_unit;
 
 typedef struct jw_block_desc {
     const void* in;
@@ -123,6 +130,10 @@ int jwdev_main64_waves(int R);
 int64_t jwdev_main64_lds_bytes(int R, int waves);
 int jwdev_main64_occupancy(int R, int waves);
 int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream);
+/* persistent uneven-batch kernel over a unit queue; *d_counter must be 0 at launch */
+int jwdev_batch64_occupancy(int R, int waves);
+int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
+                  int waves, int grid, void* stream);
 int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream);
 int jwdev_block(const jw_block_desc* d, void* stream);
 int jwdev_gbmv(const jw_gbmv_desc* d, void* stream);

@@ -39,6 +39,10 @@ __device__ __forceinline__ int swz(int c)
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
 
+// keeps the scheduler from hoisting one phase's table loads into the previous
+// phase, which would raise register pressure past the launch bound
+__device__ __forceinline__ void phase_barrier() { __builtin_amdgcn_sched_barrier(0); }
+
 // DPP move of a double with an explicit row mask; rows outside the mask and
 // lanes without a source receive 0
 template <int CTRL, int ROWMASK>
@@ -50,6 +54,13 @@ __device__ __forceinline__ double dpp_rm(double v)
 }
 constexpr int DPP_ROW_BCAST15 = 0x142;
 constexpr int DPP_ROW_BCAST31 = 0x143;
+
+// register budget: 512 / waves_per_eu VGPRs per lane (128 for R <= 8, 256 for R = 16)
+template <int R>
+constexpr int waves_per_eu()
+{
+    return R <= 8 ? 4 : 2;
+}
 
 template <int R>
 constexpr int max_threads()
@@ -67,295 +78,404 @@ __device__ __forceinline__ double pick_row(const double (&x)[R], int i)
     return v;
 }
 
+// Everything one pass over a system pair needs besides the table.
+struct jw_pass64 {
+    double* rhs;
+    int64_t nsys, stride, offSol;
+    int in_lo, in_hi, out_lo, out_hi;     // keep-ranges, clipped to [0, N)
+    int icl, ich, ocl, och;               // their full 16-byte chunks
+    bool in_edge_lo, in_edge_hi, out_edge_lo, out_edge_hi;   // odd range ends
+    int kl, ku;
+    bool fwd, b1, b2, apply_scale;
+    double row0_scale;                    // scaleC on lane 0, 1 elsewhere
+};
+
+__device__ __forceinline__ jw_pass64 make_pass64(double* rhs, int64_t nsys, int64_t stride, int64_t offSol,
+                                                 int in_lo, int in_hi, int out_lo, int out_hi, int kl, int ku,
+                                                 bool fwd, bool b1, bool b2, bool apply_scale, double scale,
+                                                 int lane)
+{
+    jw_pass64 g;
+    g.rhs = rhs;
+    g.nsys = nsys;
+    g.stride = stride;
+    g.offSol = offSol;
+    g.in_lo = in_lo;
+    g.in_hi = in_hi;
+    g.out_lo = out_lo;
+    g.out_hi = out_hi;
+    g.icl = (in_lo + 1) >> 1;
+    g.ich = in_hi >> 1;
+    g.ocl = (out_lo + 1) >> 1;
+    g.och = out_hi >> 1;
+    g.in_edge_lo = (in_lo & 1) && in_lo < in_hi;
+    g.in_edge_hi = (in_hi & 1) && in_hi - 1 >= in_lo;   // never the row of in_edge_lo: lo odd => lo+1 even
+    g.out_edge_lo = (out_lo & 1) && out_lo < out_hi;
+    g.out_edge_hi = (out_hi & 1) && out_hi - 1 >= out_lo;
+    g.kl = kl;
+    g.ku = ku;
+    g.fwd = fwd;
+    g.b1 = b1;
+    g.b2 = b2;
+    g.apply_scale = apply_scale;
+    g.row0_scale = lane == 0 ? scale : 1.0;
+    return g;
+}
+
 template <int R>
-__global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc d)
+__device__ __forceinline__ void zero_staging(double2* stg, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < R; k++) stg[64 * k + lane] = make_double2(0.0, 0.0);
+}
+
+// One pass: systems sys0 and sys0+1 (if it exists) through the whole step.
+// tab: the matrix set's table in LDS; stg: this wavefront's staging (2 systems),
+// whose slots outside the input keep-range hold zeros on entry and on exit.
+// ALL3: all three recurrences present (the fused step), decided at compile time.
+template <int R, bool ALL3>
+__device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int lane, const jw_pass64& g,
+                                          int64_t sys0)
 {
     constexpr int H = R / 2;                  // 16-byte chunks per lane per system
-    constexpr int NTAB = JW_TAB64_ELEMS(R);
     constexpr int SYSC = 64 * H;              // chunks per system region
     constexpr int SC = 8 * R * 64;            // start of the scan multipliers
+    const bool ok1 = sys0 + 1 < g.nsys;
+    const double* src0 = g.rhs + sys0 * g.stride + g.offSol;
+
+    // ---- 1. input ----
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (s == 1 && !ok1) continue;
+        const double* src = src0 + s * g.stride;
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            const int c = swz<H>(64 * k + lane);
+            if (c >= g.icl && c < g.ich)
+                __builtin_amdgcn_global_load_lds(
+                    src + 2 * c, (__attribute__((address_space(3))) void*)(stg + s * SYSC + 64 * k), 16, 0, 0);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the DMA writes have landed in LDS
+    compiler_fence();
+
+    double x0[R], x1[R];
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+        const double2 v0 = stg[swz<H>(lane * H + m)];
+        const double2 v1 = stg[SYSC + swz<H>(lane * H + m)];
+        x0[2 * m] = v0.x;
+        x0[2 * m + 1] = v0.y;
+        x1[2 * m] = v1.x;
+        x1[2 * m + 1] = v1.y;
+    }
+    if (g.in_edge_lo || g.in_edge_hi) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const bool on = e == 0 ? g.in_edge_lo : g.in_edge_hi;
+            const int r = e == 0 ? g.in_lo : g.in_hi - 1;
+            if (on && lane == r / R) {
+                const double v0 = src0[r];
+                const double v1 = ok1 ? src0[g.stride + r] : 0.0;
+#pragma unroll
+                for (int k = 0; k < R; k++)
+                    if (k == r % R) {
+                        x0[k] = v0;
+                        x1[k] = v1;
+                    }
+            }
+        }
+    }
+
+    // ---- 2. banded matvec (coefficients carry the folded row scaling) ----
+    {
+        double m1a = 0.0, m2a = 0.0, p1a = 0.0, p2a = 0.0;
+        double m1b = 0.0, m2b = 0.0, p1b = 0.0, p2b = 0.0;
+        if (g.kl >= 1) { m1a = dpp_d<DPP_WAVE_SHR1>(x0[R - 1]); m1b = dpp_d<DPP_WAVE_SHR1>(x1[R - 1]); }
+        if (g.kl >= 2) { m2a = dpp_d<DPP_WAVE_SHR1>(x0[R - 2]); m2b = dpp_d<DPP_WAVE_SHR1>(x1[R - 2]); }
+        if (g.ku >= 1) { p1a = dpp_d<DPP_WAVE_SHL1>(x0[0]); p1b = dpp_d<DPP_WAVE_SHL1>(x1[0]); }
+        if (g.ku >= 2) { p2a = dpp_d<DPP_WAVE_SHL1>(x0[1]); p2b = dpp_d<DPP_WAVE_SHL1>(x1[1]); }
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const double2 c0 = *reinterpret_cast<const double2*>(tab + tab_coef64(0, i, lane, R));
+            const double2 c1 = *reinterpret_cast<const double2*>(tab + tab_coef64(1, i, lane, R));
+            const double vp2 = tab[tab_coef64(2, i, lane, R)];
+            const double ia = x0[i], ib = x1[i];
+            const double n1a = (i + 1 < R) ? x0[(i + 1) % R] : p1a;
+            const double n1b = (i + 1 < R) ? x1[(i + 1) % R] : p1b;
+            const double n2a = (i + 2 < R) ? x0[(i + 2) % R] : ((i + 2 == R) ? p1a : p2a);
+            const double n2b = (i + 2 < R) ? x1[(i + 2) % R] : ((i + 2 == R) ? p1b : p2b);
+            double ua = c1.x * ia, ub = c1.x * ib;
+            ua = __builtin_fma(c0.y, m1a, ua);
+            ub = __builtin_fma(c0.y, m1b, ub);
+            ua = __builtin_fma(c0.x, m2a, ua);
+            ub = __builtin_fma(c0.x, m2b, ub);
+            ua = __builtin_fma(c1.y, n1a, ua);
+            ub = __builtin_fma(c1.y, n1b, ub);
+            ua = __builtin_fma(vp2, n2a, ua);
+            ub = __builtin_fma(vp2, n2b, ub);
+            m2a = m1a;
+            m2b = m1b;
+            m1a = ia;
+            m1b = ib;
+            x0[i] = ua;
+            x1[i] = ub;
+        }
+    }
+
+    // ---- 3a. forward recurrence W_r = u'_r - l'_r W_{r-1} ----
+    if (ALL3 || g.fwd) {
+        double lr[R];
+#pragma unroll
+        for (int i = 0; i < R; i++) lr[i] = tab[tab_coef64(2, i, lane, R) + 1];
+        double a = x0[0], b = x1[0];
+#pragma unroll
+        for (int i = 1; i < R; i++) {
+            a = __builtin_fma(-lr[i], a, x0[i]);
+            b = __builtin_fma(-lr[i], b, x1[i]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double f = tab[SC + k * 64 + lane];
+            double sa, sb;
+            switch (k) {
+            case 0: sa = dpp_d<DPP_ROW_SHR(1)>(a); sb = dpp_d<DPP_ROW_SHR(1)>(b); break;
+            case 1: sa = dpp_d<DPP_ROW_SHR(2)>(a); sb = dpp_d<DPP_ROW_SHR(2)>(b); break;
+            case 2: sa = dpp_d<DPP_ROW_SHR(4)>(a); sb = dpp_d<DPP_ROW_SHR(4)>(b); break;
+            default: sa = dpp_d<DPP_ROW_SHR(8)>(a); sb = dpp_d<DPP_ROW_SHR(8)>(b); break;
+            }
+            a = __builtin_fma(f, sa, a);
+            b = __builtin_fma(f, sb, b);
+        }
+        {
+            // rows 1 and 3 take the ends of rows 0 and 2; rows 2 and 3 then take lane 31
+            const double fp = tab[SC + 4 * 64 + lane], fp2 = tab[SC + 5 * 64 + lane];
+            const double sa = dpp_rm<DPP_ROW_BCAST15, 0xa>(a), sb = dpp_rm<DPP_ROW_BCAST15, 0xa>(b);
+            a = __builtin_fma(fp, sa, a);
+            b = __builtin_fma(fp, sb, b);
+            const double ta = dpp_rm<DPP_ROW_BCAST31, 0xc>(a), tb = dpp_rm<DPP_ROW_BCAST31, 0xc>(b);
+            a = __builtin_fma(fp2, ta, a);
+            b = __builtin_fma(fp2, tb, b);
+        }
+        double wa = dpp_d<DPP_WAVE_SHR1>(a), wb = dpp_d<DPP_WAVE_SHR1>(b);
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            wa = __builtin_fma(-lr[i], wa, x0[i]);
+            wb = __builtin_fma(-lr[i], wb, x1[i]);
+            x0[i] = wa;
+            x1[i] = wb;
+        }
+    }
+
+    // ---- 3b/3c. backward recurrences Z_r = W_r - h'_r Z_{r+1}, y_r = Z_r - E_r y_{r+1} ----
+#pragma unroll
+    for (int stage = 0; stage < 2; stage++) {
+            if (!ALL3 && stage == 0 && !g.b1) continue;
+        if (!ALL3 && stage == 1 && !g.b2) continue;
+        double hh[R];
+#pragma unroll
+        for (int i = 0; i < R; i++) hh[i] = tab[tab_coef64(3, i, lane, R) + stage];
+        const int j = 1 + stage;
+        double a = x0[R - 1], b = x1[R - 1];
+#pragma unroll
+        for (int i = R - 2; i >= 0; i--) {
+            a = __builtin_fma(-hh[i], a, x0[i]);
+            b = __builtin_fma(-hh[i], b, x1[i]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double f = tab[SC + (j * 8 + k) * 64 + lane];
+            double sa, sb;
+            switch (k) {
+            case 0: sa = dpp_d<DPP_ROW_SHL(1)>(a); sb = dpp_d<DPP_ROW_SHL(1)>(b); break;
+            case 1: sa = dpp_d<DPP_ROW_SHL(2)>(a); sb = dpp_d<DPP_ROW_SHL(2)>(b); break;
+            case 2: sa = dpp_d<DPP_ROW_SHL(4)>(a); sb = dpp_d<DPP_ROW_SHL(4)>(b); break;
+            default: sa = dpp_d<DPP_ROW_SHL(8)>(a); sb = dpp_d<DPP_ROW_SHL(8)>(b); break;
+            }
+            a = __builtin_fma(f, sa, a);
+            b = __builtin_fma(f, sb, b);
+        }
+        {
+            // every row adds the in-row results of the later row starts (lanes 16, 32, 48),
+            // weighted by matrix-only multipliers (zero for rows at or after that start)
+            const double c1 = tab[SC + (j * 8 + 4) * 64 + lane];
+            const double c2 = tab[SC + (j * 8 + 5) * 64 + lane];
+            const double c3 = tab[SC + (j * 8 + 6) * 64 + lane];
+            const double a1 = readlane_d(a, 16), a2 = readlane_d(a, 32), a3 = readlane_d(a, 48);
+            const double b1 = readlane_d(b, 16), b2 = readlane_d(b, 32), b3 = readlane_d(b, 48);
+            a = __builtin_fma(c1, a1, a);
+            b = __builtin_fma(c1, b1, b);
+            a = __builtin_fma(c2, a2, a);
+            b = __builtin_fma(c2, b2, b);
+            a = __builtin_fma(c3, a3, a);
+            b = __builtin_fma(c3, b3, b);
+        }
+        double za = dpp_d<DPP_WAVE_SHL1>(a), zb = dpp_d<DPP_WAVE_SHL1>(b);
+#pragma unroll
+        for (int i = R - 1; i >= 0; i--) {
+            za = __builtin_fma(-hh[i], za, x0[i]);
+            zb = __builtin_fma(-hh[i], zb, x1[i]);
+            x0[i] = za;
+            x1[i] = zb;
+        }
+    }
+
+    // ---- row-0 scaleC (pfSolve_TridiagonalSolve.h:1918-1927) ----
+    if (g.apply_scale) {
+        x0[0] *= g.row0_scale;
+        x1[0] *= g.row0_scale;
+    }
+
+    // ---- 4. output: blocked -> LDS -> contiguous 16-byte stores ----
+    compiler_fence();
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+        stg[swz<H>(lane * H + m)] = make_double2(x0[2 * m], x0[2 * m + 1]);
+        stg[SYSC + swz<H>(lane * H + m)] = make_double2(x1[2 * m], x1[2 * m + 1]);
+    }
+    compiler_fence();
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (s == 1 && !ok1) continue;
+        // the host only selects this kernel when sys*stride + offSol is even and rhs is 16-byte aligned
+        double2* dst2 = reinterpret_cast<double2*>(g.rhs) + (((sys0 + s) * g.stride + g.offSol) >> 1);
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            const int c = 64 * k + lane;
+            if (c >= g.ocl && c < g.och) dst2[c] = stg[s * SYSC + swz<H>(c)];
+        }
+    }
+    if (g.out_edge_lo || g.out_edge_hi) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const bool on = e == 0 ? g.out_edge_lo : g.out_edge_hi;
+            const int r = e == 0 ? g.out_lo : g.out_hi - 1;
+            if (on && lane == r / R) {
+                double* dst = g.rhs + sys0 * g.stride + g.offSol + r;
+                dst[0] = pick_row<R>(x0, r % R);
+                if (ok1) dst[g.stride] = pick_row<R>(x1, r % R);
+            }
+        }
+    }
+    // restore the zeros of the slots the input DMA skips
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            const int c = swz<H>(64 * k + lane);
+            if (!(c >= g.icl && c < g.ich)) stg[s * SYSC + 64 * k + lane] = make_double2(0.0, 0.0);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): LDS traffic done before the next DMA lands
+    compiler_fence();
+}
+
+template <int R, bool ALL3>
+__global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc d)
+{
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
     extern __shared__ __align__(16) unsigned char smem[];
     const double* tab = reinterpret_cast<const double*>(smem);
-
     {
         const uint4* src = reinterpret_cast<const uint4*>(d.table);
         uint4* dst = reinterpret_cast<uint4*>(smem);
         constexpr int n16 = NTAB * 8 / 16;
         for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
     }
-
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = blockDim.x >> 6;
-    double2* stg = reinterpret_cast<double2*>(smem + NTAB * 8) + (size_t)wave * 2 * SYSC;
-    double* rhs = reinterpret_cast<double*>(d.rhs);
-    const int64_t npass = (d.nsys + 1) >> 1;
-
-    // full chunks of the keep-ranges and their odd ends (32-bit row arithmetic)
-    const int in_lo = (int)d.in_lo, in_hi = (int)d.in_hi;
-    const int out_lo = (int)d.out_lo, out_hi = (int)d.out_hi;
-    const int icl = (in_lo + 1) >> 1, ich = in_hi >> 1;
-    const int ocl = (out_lo + 1) >> 1, och = out_hi >> 1;
-    const bool in_edge_lo = (in_lo & 1) && in_lo < in_hi;
-    const bool in_edge_hi = (in_hi & 1) && in_hi - 1 >= in_lo;   // never the row of in_edge_lo: lo odd => lo+1 even
-    const bool out_edge_lo = (out_lo & 1) && out_lo < out_hi;
-    const bool out_edge_hi = (out_hi & 1) && out_hi - 1 >= out_lo;
-
-    const double row0_scale = lane == 0 ? d.scale_hi : 1.0;
-
-    // Staging slots that the input DMA never fills (chunks outside the input
-    // keep-range) hold zeros, so the blocked read-back needs no masking.
-    // The output phase overwrites every slot and re-zeroes those afterwards.
-#pragma unroll
-    for (int k = 0; k < 2 * H; k++) stg[64 * k + lane] = make_double2(0.0, 0.0);
+    double2* stg = reinterpret_cast<double2*>(smem + NTAB * 8) + (size_t)wave * 64 * R;
+    zero_staging<R>(stg, lane);
     __syncthreads();
-
+    const int64_t npass = (d.nsys + 1) >> 1;
     for (int64_t q = (int64_t)blockIdx.x * nwaves + wave; q < npass; q += (int64_t)gridDim.x * nwaves) {
-        const int64_t sys0 = 2 * q;
-        const bool ok1 = sys0 + 1 < d.nsys;
-        const double* src0 = rhs + sys0 * d.stride + d.offSol;
+        // rebuilt every pass from the kernel arguments, so that nothing derived from
+        // them stays live in registers across the loop
+        const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
+                                        (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
+                                        d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+        jw_pair64<R, ALL3>(tab, stg, lane, g, 2 * q);
+    }
+}
 
-        // ---- 1. input ----
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            if (s == 1 && !ok1) continue;
-            const double* src = src0 + s * d.stride;
-#pragma unroll
-            for (int k = 0; k < H; k++) {
-                const int c = swz<H>(64 * k + lane);
-                if (c >= icl && c < ich)
-                    __builtin_amdgcn_global_load_lds(
-                        src + 2 * c, (__attribute__((address_space(3))) void*)(stg + s * SYSC + 64 * k), 16, 0, 0);
-            }
+// Uneven-size batch: a persistent grid drains a queue of work units (a matrix
+// set and a range of its system pairs), in the order the host sorted them
+// (largest N first).  A workgroup takes the next unit with one atomic add on
+// the queue counter, stages the set's table in LDS when the set changes and
+// spreads the unit's pairs over its wavefronts.  Every workgroup leaves the
+// loop once the counter passes the last unit.
+template <int R>
+__global__ __launch_bounds__(max_threads<R>()) void jw_batch64_kernel(const jw_batch_item* items,
+                                                                      const jw_batch_unit* units, int nunits,
+                                                                      int* counter)
+{
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_unit;
+    const double* tab = reinterpret_cast<const double*>(smem);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* stg = reinterpret_cast<double2*>(smem + NTAB * 8) + (size_t)wave * 64 * R;
+    int cur = -1;
+    for (;;) {
+        __syncthreads();   // the previous unit is finished: s_unit and the table may change
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (u >= nunits) break;
+        const jw_batch_unit un = units[u];
+        const jw_batch_item& it = items[un.item];
+        if (un.item != cur) {
+            const uint4* src = reinterpret_cast<const uint4*>(it.table);
+            uint4* dst = reinterpret_cast<uint4*>(smem);
+            constexpr int n16 = NTAB * 8 / 16;
+            for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
+            zero_staging<R>(stg, lane);   // the new set's keep-range may be narrower
+            __syncthreads();
+            cur = un.item;
         }
-        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the DMA writes have landed in LDS
-        compiler_fence();
-
-        double x0[R], x1[R];
-#pragma unroll
-        for (int m = 0; m < H; m++) {
-            const double2 v0 = stg[swz<H>(lane * H + m)];
-            const double2 v1 = stg[SYSC + swz<H>(lane * H + m)];
-            x0[2 * m] = v0.x;
-            x0[2 * m + 1] = v0.y;
-            x1[2 * m] = v1.x;
-            x1[2 * m + 1] = v1.y;
+        for (int q = un.pair0 + wave; q < un.pair1; q += nwaves) {
+            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(it.rhs), it.nsys, it.stride, it.offSol, 0,
+                                            it.N, 0, it.N, it.kl, it.ku, true, true, true, it.apply_scale != 0,
+                                            it.scale, lane);
+            jw_pair64<R, true>(tab, stg, lane, g, 2 * (int64_t)q);
         }
-        if (in_edge_lo || in_edge_hi) {
-#pragma unroll
-            for (int e = 0; e < 2; e++) {
-                const bool on = e == 0 ? in_edge_lo : in_edge_hi;
-                const int r = e == 0 ? in_lo : in_hi - 1;
-                if (on && lane == r / R) {
-                    const double v0 = src0[r];
-                    const double v1 = ok1 ? src0[d.stride + r] : 0.0;
-#pragma unroll
-                    for (int k = 0; k < R; k++)
-                        if (k == r % R) {
-                            x0[k] = v0;
-                            x1[k] = v1;
-                        }
-                }
-            }
-        }
-
-        // ---- 2. banded matvec (coefficients carry the folded row scaling) ----
-        {
-            double m1a = 0.0, m2a = 0.0, p1a = 0.0, p2a = 0.0;
-            double m1b = 0.0, m2b = 0.0, p1b = 0.0, p2b = 0.0;
-            if (d.kl >= 1) { m1a = dpp_d<DPP_WAVE_SHR1>(x0[R - 1]); m1b = dpp_d<DPP_WAVE_SHR1>(x1[R - 1]); }
-            if (d.kl >= 2) { m2a = dpp_d<DPP_WAVE_SHR1>(x0[R - 2]); m2b = dpp_d<DPP_WAVE_SHR1>(x1[R - 2]); }
-            if (d.ku >= 1) { p1a = dpp_d<DPP_WAVE_SHL1>(x0[0]); p1b = dpp_d<DPP_WAVE_SHL1>(x1[0]); }
-            if (d.ku >= 2) { p2a = dpp_d<DPP_WAVE_SHL1>(x0[1]); p2b = dpp_d<DPP_WAVE_SHL1>(x1[1]); }
-#pragma unroll
-            for (int i = 0; i < R; i++) {
-                const double2 c0 = *reinterpret_cast<const double2*>(tab + tab_coef64(0, i, lane, R));
-                const double2 c1 = *reinterpret_cast<const double2*>(tab + tab_coef64(1, i, lane, R));
-                const double vp2 = tab[tab_coef64(2, i, lane, R)];
-                const double ia = x0[i], ib = x1[i];
-                const double n1a = (i + 1 < R) ? x0[(i + 1) % R] : p1a;
-                const double n1b = (i + 1 < R) ? x1[(i + 1) % R] : p1b;
-                const double n2a = (i + 2 < R) ? x0[(i + 2) % R] : ((i + 2 == R) ? p1a : p2a);
-                const double n2b = (i + 2 < R) ? x1[(i + 2) % R] : ((i + 2 == R) ? p1b : p2b);
-                double ua = c1.x * ia, ub = c1.x * ib;
-                ua = __builtin_fma(c0.y, m1a, ua);
-                ub = __builtin_fma(c0.y, m1b, ub);
-                ua = __builtin_fma(c0.x, m2a, ua);
-                ub = __builtin_fma(c0.x, m2b, ub);
-                ua = __builtin_fma(c1.y, n1a, ua);
-                ub = __builtin_fma(c1.y, n1b, ub);
-                ua = __builtin_fma(vp2, n2a, ua);
-                ub = __builtin_fma(vp2, n2b, ub);
-                m2a = m1a;
-                m2b = m1b;
-                m1a = ia;
-                m1b = ib;
-                x0[i] = ua;
-                x1[i] = ub;
-            }
-        }
-
-        // ---- 3a. forward recurrence W_r = u'_r - l'_r W_{r-1} ----
-        if (d.fwd) {
-            double lr[R];
-#pragma unroll
-            for (int i = 0; i < R; i++) lr[i] = tab[tab_coef64(2, i, lane, R) + 1];
-            double a = x0[0], b = x1[0];
-#pragma unroll
-            for (int i = 1; i < R; i++) {
-                a = __builtin_fma(-lr[i], a, x0[i]);
-                b = __builtin_fma(-lr[i], b, x1[i]);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const double f = tab[SC + k * 64 + lane];
-                double sa, sb;
-                switch (k) {
-                case 0: sa = dpp_d<DPP_ROW_SHR(1)>(a); sb = dpp_d<DPP_ROW_SHR(1)>(b); break;
-                case 1: sa = dpp_d<DPP_ROW_SHR(2)>(a); sb = dpp_d<DPP_ROW_SHR(2)>(b); break;
-                case 2: sa = dpp_d<DPP_ROW_SHR(4)>(a); sb = dpp_d<DPP_ROW_SHR(4)>(b); break;
-                default: sa = dpp_d<DPP_ROW_SHR(8)>(a); sb = dpp_d<DPP_ROW_SHR(8)>(b); break;
-                }
-                a = __builtin_fma(f, sa, a);
-                b = __builtin_fma(f, sb, b);
-            }
-            {
-                // rows 1 and 3 take the ends of rows 0 and 2; rows 2 and 3 then take lane 31
-                const double fp = tab[SC + 4 * 64 + lane], fp2 = tab[SC + 5 * 64 + lane];
-                const double sa = dpp_rm<DPP_ROW_BCAST15, 0xa>(a), sb = dpp_rm<DPP_ROW_BCAST15, 0xa>(b);
-                a = __builtin_fma(fp, sa, a);
-                b = __builtin_fma(fp, sb, b);
-                const double ta = dpp_rm<DPP_ROW_BCAST31, 0xc>(a), tb = dpp_rm<DPP_ROW_BCAST31, 0xc>(b);
-                a = __builtin_fma(fp2, ta, a);
-                b = __builtin_fma(fp2, tb, b);
-            }
-            double wa = dpp_d<DPP_WAVE_SHR1>(a), wb = dpp_d<DPP_WAVE_SHR1>(b);
-#pragma unroll
-            for (int i = 0; i < R; i++) {
-                wa = __builtin_fma(-lr[i], wa, x0[i]);
-                wb = __builtin_fma(-lr[i], wb, x1[i]);
-                x0[i] = wa;
-                x1[i] = wb;
-            }
-        }
-
-        // ---- 3b/3c. backward recurrences Z_r = W_r - h'_r Z_{r+1}, y_r = Z_r - E_r y_{r+1} ----
-#pragma unroll
-        for (int stage = 0; stage < 2; stage++) {
-            if (stage == 0 && !d.b1) continue;
-            if (stage == 1 && !d.b2) continue;
-            double hh[R];
-#pragma unroll
-            for (int i = 0; i < R; i++) hh[i] = tab[tab_coef64(3, i, lane, R) + stage];
-            const int j = 1 + stage;
-            double a = x0[R - 1], b = x1[R - 1];
-#pragma unroll
-            for (int i = R - 2; i >= 0; i--) {
-                a = __builtin_fma(-hh[i], a, x0[i]);
-                b = __builtin_fma(-hh[i], b, x1[i]);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const double f = tab[SC + (j * 8 + k) * 64 + lane];
-                double sa, sb;
-                switch (k) {
-                case 0: sa = dpp_d<DPP_ROW_SHL(1)>(a); sb = dpp_d<DPP_ROW_SHL(1)>(b); break;
-                case 1: sa = dpp_d<DPP_ROW_SHL(2)>(a); sb = dpp_d<DPP_ROW_SHL(2)>(b); break;
-                case 2: sa = dpp_d<DPP_ROW_SHL(4)>(a); sb = dpp_d<DPP_ROW_SHL(4)>(b); break;
-                default: sa = dpp_d<DPP_ROW_SHL(8)>(a); sb = dpp_d<DPP_ROW_SHL(8)>(b); break;
-                }
-                a = __builtin_fma(f, sa, a);
-                b = __builtin_fma(f, sb, b);
-            }
-            {
-                // every row adds the in-row results of the later row starts (lanes 16, 32, 48),
-                // weighted by matrix-only multipliers (zero for rows at or after that start)
-                const double c1 = tab[SC + (j * 8 + 4) * 64 + lane];
-                const double c2 = tab[SC + (j * 8 + 5) * 64 + lane];
-                const double c3 = tab[SC + (j * 8 + 6) * 64 + lane];
-                const double a1 = readlane_d(a, 16), a2 = readlane_d(a, 32), a3 = readlane_d(a, 48);
-                const double b1 = readlane_d(b, 16), b2 = readlane_d(b, 32), b3 = readlane_d(b, 48);
-                a = __builtin_fma(c1, a1, a);
-                b = __builtin_fma(c1, b1, b);
-                a = __builtin_fma(c2, a2, a);
-                b = __builtin_fma(c2, b2, b);
-                a = __builtin_fma(c3, a3, a);
-                b = __builtin_fma(c3, b3, b);
-            }
-            double za = dpp_d<DPP_WAVE_SHL1>(a), zb = dpp_d<DPP_WAVE_SHL1>(b);
-#pragma unroll
-            for (int i = R - 1; i >= 0; i--) {
-                za = __builtin_fma(-hh[i], za, x0[i]);
-                zb = __builtin_fma(-hh[i], zb, x1[i]);
-                x0[i] = za;
-                x1[i] = zb;
-            }
-        }
-
-        // ---- row-0 scaleC (pfSolve_TridiagonalSolve.h:1918-1927) ----
-        if (d.apply_scale) {
-            x0[0] *= row0_scale;
-            x1[0] *= row0_scale;
-        }
-
-        // ---- 4. output: blocked -> LDS -> contiguous 16-byte stores ----
-        compiler_fence();
-#pragma unroll
-        for (int m = 0; m < H; m++) {
-            stg[swz<H>(lane * H + m)] = make_double2(x0[2 * m], x0[2 * m + 1]);
-            stg[SYSC + swz<H>(lane * H + m)] = make_double2(x1[2 * m], x1[2 * m + 1]);
-        }
-        compiler_fence();
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            if (s == 1 && !ok1) continue;
-            // the host only selects this kernel when sys*stride + offSol is even and rhs is 16-byte aligned
-            double2* dst2 = reinterpret_cast<double2*>(rhs) + (((sys0 + s) * d.stride + d.offSol) >> 1);
-#pragma unroll
-            for (int k = 0; k < H; k++) {
-                const int c = 64 * k + lane;
-                if (c >= ocl && c < och) dst2[c] = stg[s * SYSC + swz<H>(c)];
-            }
-        }
-        if (out_edge_lo || out_edge_hi) {
-#pragma unroll
-            for (int e = 0; e < 2; e++) {
-                const bool on = e == 0 ? out_edge_lo : out_edge_hi;
-                const int r = e == 0 ? out_lo : out_hi - 1;
-                if (on && lane == r / R) {
-                    double* dst = rhs + sys0 * d.stride + d.offSol + r;
-                    dst[0] = pick_row<R>(x0, r % R);
-                    if (ok1) dst[d.stride] = pick_row<R>(x1, r % R);
-                }
-            }
-        }
-        // restore the zeros of the slots the input DMA skips
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-#pragma unroll
-            for (int k = 0; k < H; k++) {
-                const int c = swz<H>(64 * k + lane);
-                if (!(c >= icl && c < ich)) stg[s * SYSC + 64 * k + lane] = make_double2(0.0, 0.0);
-            }
-        }
-        __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): LDS traffic done before the next DMA lands
-        compiler_fence();
     }
 }
 
 template <int R>
-const void* kernel_ptr()
+const void* kernel_ptr(bool all3)
 {
-    return reinterpret_cast<const void*>(&jw_main64_kernel<R>);
+    return all3 ? reinterpret_cast<const void*>(&jw_main64_kernel<R, true>)
+                : reinterpret_cast<const void*>(&jw_main64_kernel<R, false>);
 }
 
-const void* pick64(int R)
+const void* pick64(int R, bool all3 = true)
 {
     switch (R) {
-    case 2: return kernel_ptr<2>();
-    case 4: return kernel_ptr<4>();
-    case 8: return kernel_ptr<8>();
-    case 16: return kernel_ptr<16>();
+    case 2: return kernel_ptr<2>(all3);
+    case 4: return kernel_ptr<4>(all3);
+    case 8: return kernel_ptr<8>(all3);
+    case 16: return kernel_ptr<16>(all3);
+    default: return nullptr;
+    }
+}
+
+template <int R>
+const void* batch_ptr()
+{
+    return reinterpret_cast<const void*>(&jw_batch64_kernel<R>);
+}
+
+const void* pick_batch64(int R)
+{
+    switch (R) {
+    case 2: return batch_ptr<2>();
+    case 4: return batch_ptr<4>();
+    case 8: return batch_ptr<8>();
+    case 16: return batch_ptr<16>();
     default: return nullptr;
     }
 }
@@ -387,25 +507,58 @@ int64_t jwdev_main64_lds_bytes(int R, int waves)
     return jwdev_table64_elems(R) * 8 + 2LL * 64 * R * 8 * waves;
 }
 
+/* workgroups per CU, the smaller of the two variants (all recurrences / some) */
 int jwdev_main64_occupancy(int R, int waves)
 {
-    const void* f = pick64(R);
-    if (!f) return 0;
+    int best = -1;
+    for (int v = 0; v < 2; v++) {
+        const void* f = pick64(R, v == 0);
+        if (!f) return 0;
+        const int lds = (int)jwdev_main64_lds_bytes(R, waves);
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * waves, lds) != hipSuccess) return 0;
+        if (best < 0 || nb < best) best = nb;
+    }
+    return best;
+}
+
+int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream)
+{
+    const void* f = pick64(R, r->fwd && r->b1 && r->b2);
+    if (!f) { snprintf(g_err64, sizeof(g_err64), "no 64-lane kernel for R=%d", R); return -1; }
+    const size_t lds = (size_t)jwdev_main64_lds_bytes(R, waves);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return -2;
+    void* args[] = {const_cast<jw_run_desc*>(r)};
+    e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? 0 : -3;
+}
+
+int jwdev_batch64_occupancy(int R, int waves)
+{
+    const void* f = pick_batch64(R);
+    if (!f || waves < 1 || waves > max_waves64(R)) return 0;
     const int lds = (int)jwdev_main64_lds_bytes(R, waves);
+    if (lds > kLdsBytesMax) return 0;
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * waves, lds) != hipSuccess) return 0;
     return nb;
 }
 
-int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream)
+int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
+                  int waves, int grid, void* stream)
 {
-    const void* f = pick64(R);
-    if (!f) { snprintf(g_err64, sizeof(g_err64), "no 64-lane kernel for R=%d", R); return -1; }
+    const void* f = pick_batch64(R);
+    if (!f) { snprintf(g_err64, sizeof(g_err64), "no batch kernel for R=%d", R); return -1; }
+    if (waves < 1 || waves > max_waves64(R) || grid < 1) { snprintf(g_err64, sizeof(g_err64), "bad batch launch"); return -1; }
     const size_t lds = (size_t)jwdev_main64_lds_bytes(R, waves);
+    if (lds > (size_t)kLdsBytesMax) { snprintf(g_err64, sizeof(g_err64), "batch LDS %zu too large", lds); return -1; }
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -2;
-    void* args[] = {const_cast<jw_run_desc*>(r)};
+    void* args[] = {const_cast<jw_batch_item**>(&d_items), const_cast<jw_batch_unit**>(&d_units), &nunits,
+                    &d_counter};
     e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : -3;
 }
