@@ -117,11 +117,19 @@ typedef struct pfsolve_batch pfsolve_batch;
 PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set* sets, int64_t nsets,
                                    const pfsolve_batch_tuning* tuning);
 PfSolveResult pfsolve_batch_run(pfsolve_batch* b, hipStream_t stream);
+/* run = prepare + solve.  prepare writes every set's coefficient table from its
+ * matrices; solve applies the resident tables to the right-hand sides.  A
+ * caller whose matrices do not change between steps prepares once and then
+ * only solves. */
+PfSolveResult pfsolve_batch_prepare(pfsolve_batch* b, hipStream_t stream);
+PfSolveResult pfsolve_batch_solve(pfsolve_batch* b, hipStream_t stream);
 PfSolveResult pfsolve_batch_run_plain(pfsolve_batch* b, hipStream_t stream);
 /* Records pfsolve_batch_run on `stream` (not the null stream) into a hipGraph,
  * then replays it with pfsolve_batch_launch_graph.  The queue counters are
  * reset by a kernel inside the graph, so every replay is a full step. */
 PfSolveResult pfsolve_batch_capture_graph(pfsolve_batch* b, hipStream_t stream);
+/* solve_only = 1: the graph holds pfsolve_batch_solve (tables prepared beforehand) */
+PfSolveResult pfsolve_batch_capture_graph_mode(pfsolve_batch* b, hipStream_t stream, int solve_only);
 PfSolveResult pfsolve_batch_launch_graph(pfsolve_batch* b, hipStream_t stream);
 /* Autotuner: times every candidate (waves, unit size, grid, rows per lane,
  * queue order) on scratch right-hand sides of the batch's shapes (the

@@ -178,11 +178,18 @@ class Batch:
     def run(self, stream: int = 0) -> None:
         check(lib().pfsolve_batch_run(self.handle, stream), "pfsolve_batch_run")
 
+    def prepare(self, stream: int = 0) -> None:
+        check(lib().pfsolve_batch_prepare(self.handle, stream), "pfsolve_batch_prepare")
+
+    def solve(self, stream: int = 0) -> None:
+        check(lib().pfsolve_batch_solve(self.handle, stream), "pfsolve_batch_solve")
+
     def run_plain(self, stream: int = 0) -> None:
         check(lib().pfsolve_batch_run_plain(self.handle, stream), "pfsolve_batch_run_plain")
 
-    def capture_graph(self, stream: int) -> None:
-        check(lib().pfsolve_batch_capture_graph(self.handle, stream), "pfsolve_batch_capture_graph")
+    def capture_graph(self, stream: int, solve_only: bool = False) -> None:
+        check(lib().pfsolve_batch_capture_graph_mode(self.handle, stream, int(solve_only)),
+              "pfsolve_batch_capture_graph")
 
     def launch_graph(self, stream: int) -> None:
         check(lib().pfsolve_batch_launch_graph(self.handle, stream), "pfsolve_batch_launch_graph")

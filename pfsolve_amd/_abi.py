@@ -315,10 +315,12 @@ def declare_ext(lib: C.CDLL) -> None:
     lib.pfsolve_batch_create.argtypes = [P(vp), P(PfSolveBatchSet), i64, P(PfSolveBatchTuning)]
     lib.pfsolve_batch_create.restype = C.c_int
     for name in ("pfsolve_batch_run", "pfsolve_batch_run_plain", "pfsolve_batch_capture_graph",
-                 "pfsolve_batch_launch_graph"):
+                 "pfsolve_batch_launch_graph", "pfsolve_batch_prepare", "pfsolve_batch_solve"):
         f = getattr(lib, name)
         f.argtypes = [vp, vp]
         f.restype = C.c_int
+    lib.pfsolve_batch_capture_graph_mode.argtypes = [vp, vp, C.c_int]
+    lib.pfsolve_batch_capture_graph_mode.restype = C.c_int
     lib.pfsolve_batch_tune.argtypes = [vp, vp, C.c_int, P(PfSolveBatchTuning)]
     lib.pfsolve_batch_tune.restype = C.c_int
     lib.pfsolve_batch_set_tuning.argtypes = [vp, P(PfSolveBatchTuning)]

@@ -456,6 +456,21 @@ PfSolveResult pfsolve_batch_run(pfsolve_batch* b, hipStream_t stream)
     return launch_step(b, b->d_items, stream, -1, 1);
 }
 
+PfSolveResult pfsolve_batch_prepare(pfsolve_batch* b, hipStream_t stream)
+{
+    if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    return jwdev_prologue_batch(b->d_jobs, (int)b->nsets, b->max_RL, JW_PREC_F64, b->d_counters, NCLS, stream) == 0
+               ? PFSOLVE_SUCCESS
+               : PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+}
+
+PfSolveResult pfsolve_batch_solve(pfsolve_batch* b, hipStream_t stream)
+{
+    if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    if (jwdev_zero_counters(b->d_counters, NCLS, stream) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    return launch_step(b, b->d_items, stream, -1, 0);
+}
+
 PfSolveResult pfsolve_batch_run_plain(pfsolve_batch* b, hipStream_t stream)
 {
     if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
@@ -500,13 +515,18 @@ PfSolveResult pfsolve_batch_run_plain(pfsolve_batch* b, hipStream_t stream)
 
 PfSolveResult pfsolve_batch_capture_graph(pfsolve_batch* b, hipStream_t stream)
 {
+    return pfsolve_batch_capture_graph_mode(b, stream, 0);
+}
+
+PfSolveResult pfsolve_batch_capture_graph_mode(pfsolve_batch* b, hipStream_t stream, int solve_only)
+{
     if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     if (!stream) return PFSOLVE_ERROR_INVALID_QUEUE;
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = NULL; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = NULL; }
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) != hipSuccess)
         return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
-    PfSolveResult r = launch_step(b, b->d_items, stream, -1, 1);
+    PfSolveResult r = solve_only ? pfsolve_batch_solve(b, stream) : launch_step(b, b->d_items, stream, -1, 1);
     hipGraph_t g = NULL;
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (r != PFSOLVE_SUCCESS) { if (g) hipGraphDestroy(g); return r; }

@@ -1,12 +1,13 @@
 // jw_device.hip -- gfx950 (CDNA4, wave64) kernels of the Jones-Worland step.
 //
 // Kernels
-//   jw_prologue_kernel  one wavefront per matrix set: reads the caller's
+//   jw_prologue_kernel  one workgroup per matrix set: reads the caller's
 //                       matrices in the reference storage (any JW mode or the
 //                       fused V/T/M step), factors the unit-diagonal solve
 //                       (LU of I+L+U, pivots by a Moebius-map scan across
 //                       lanes), and writes the per-row coefficient table and
-//                       the per-lane scan multipliers.
+//                       the per-lane scan multipliers.  jw_prologue_batch_kernel
+//                       does the same for many sets, one workgroup each.
 //   jw_main_kernel<T,R,L>
 //                       persistent: each workgroup stages one table in LDS,
 //                       each wavefront owns 64/L groups of L lanes, each group
@@ -74,9 +75,49 @@ __host__ __device__ __forceinline__ int ilog2_rt(int v)
 // ------------------------------------------------------------------------
 constexpr int kPrologueThreads = 256;
 
-// LDS of prologue_body: 12 arrays of R*L elements (v[-2..2], ld, ud, D, E, l, g, h)
-// followed by 3 x 64 per-lane multipliers
-static inline size_t prologue_lds_bytes(int RL, size_t elt) { return (12 * (size_t)RL + 192) * elt; }
+// LDS of prologue_body: 5 arrays of R*L elements (ld -> l -> l', ud -> h -> h', D, E, g)
+static inline size_t prologue_lds_bytes(int RL, size_t elt) { return 5 * (size_t)RL * elt; }
+
+// the band coefficients v_{-2..2} of row r of the matvec (zero outside the
+// matrix), P = storage position of row r in the permuted diagonals
+template <typename T>
+__device__ __forceinline__ void row_band(const jw_source_desc& d, const void* mat, int64_t r, int64_t N, int64_t P,
+                                         T (&v)[5])
+{
+    using A = Ar<T>;
+    const int mf = d.mem_f64;
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[k] = A::zero();
+    switch (d.band_kind) {
+    case JW_BAND_DIAG:
+        v[2] = mem_load<T>(mat, d.mdA + P, mf);
+        break;
+    case JW_BAND_UPPER:
+        v[2] = mem_load<T>(mat, d.mdA + P, mf);
+        if (r + 1 < N) v[3] = mem_load<T>(mat, d.udA + P, mf);
+        break;
+    case JW_BAND_LOWER:
+        v[2] = mem_load<T>(mat, d.mdA + P, mf);
+        if (r >= 1) v[1] = mem_load<T>(mat, d.ldA + P, mf);
+        break;
+    case JW_BAND_GB:
+#pragma unroll
+        for (int off = -2; off <= 2; off++) {
+            const int64_t col = r + off;
+            if (off < -d.KL || off > d.KU || col < 0 || col >= N) continue;
+            v[2 + off] = mem_load<T>(mat, d.offV + (int64_t)(d.KU - off) * N + col, mf);
+        }
+        break;
+    default:
+        v[2] = A::one();
+        break;
+    }
+    if (d.mdS >= 0) {
+        const T sc = mem_load<T>(mat, d.mdS + P, mf);
+#pragma unroll
+        for (int k = 0; k < 5; k++) v[k] = A::mul(v[k], sc);
+    }
+}
 
 template <typename T>
 __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table, int L, int R, int format,
@@ -84,14 +125,11 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
 {
     using A = Ar<T>;
     const int RL = R * L;
-    T* sv = reinterpret_cast<T*>(smem_raw);   // 5 arrays: v_{-2..2}
-    T* sld = sv + 5 * RL;                      // raw ld, later l' (format 2)
-    T* sud = sld + RL;                         // raw ud, later h' (format 2)
-    T* sD = sud + RL;
+    T* sl = reinterpret_cast<T*>(smem_raw);    // raw ld, then l, then (format 2) l'
+    T* sh = sl + RL;                           // raw ud, then h, then (format 2) h'
+    T* sD = sh + RL;
     T* sE = sD + RL;
-    T* sl = sE + RL;
-    T* sg = sl + RL;
-    T* sh = sg + RL;
+    T* sg = sE + RL;
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int64_t N = d.N;
@@ -99,43 +137,16 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
     const bool has_b1 = d.solve_kind == JW_SOLVE_UPPER || d.solve_kind == JW_SOLVE_TRI;
     const bool has_f = d.solve_kind == JW_SOLVE_LOWER || d.solve_kind == JW_SOLVE_TRI;
 
-    // phase 1: raw coefficients per row, zeros outside the matrix
+    // phase 1: raw solve coefficients per row (zeros outside the matrix); format 1
+    // also takes the band coefficients straight into the table
     for (int r = tid; r < RL; r += nthr) {
+        T ldv = A::zero(), udv = A::zero(), Dv = A::zero(), Ev = A::zero();
         T v[5];
 #pragma unroll
         for (int k = 0; k < 5; k++) v[k] = A::zero();
-        T ldv = A::zero(), udv = A::zero(), Dv = A::zero(), Ev = A::zero();
         if (r < N) {
             const int64_t P = d.permuted ? reorder_pos(r, N) : (int64_t)r;
-            switch (d.band_kind) {
-            case JW_BAND_DIAG:
-                v[2] = mem_load<T>(mat, d.mdA + P, mf);
-                break;
-            case JW_BAND_UPPER:
-                v[2] = mem_load<T>(mat, d.mdA + P, mf);
-                if (r + 1 < N) v[3] = mem_load<T>(mat, d.udA + P, mf);
-                break;
-            case JW_BAND_LOWER:
-                v[2] = mem_load<T>(mat, d.mdA + P, mf);
-                if (r >= 1) v[1] = mem_load<T>(mat, d.ldA + P, mf);
-                break;
-            case JW_BAND_GB:
-#pragma unroll
-                for (int off = -2; off <= 2; off++) {
-                    const int64_t col = r + off;
-                    if (off < -d.KL || off > d.KU || col < 0 || col >= N) continue;
-                    v[2 + off] = mem_load<T>(mat, d.offV + (int64_t)(d.KU - off) * N + col, mf);
-                }
-                break;
-            default:
-                v[2] = A::one();
-                break;
-            }
-            if (d.mdS >= 0) {
-                T s = mem_load<T>(mat, d.mdS + P, mf);
-#pragma unroll
-                for (int k = 0; k < 5; k++) v[k] = A::mul(v[k], s);
-            }
+            if (format == 1) row_band<T>(d, mat, r, N, P, v);
             if (has_f && r >= 1) ldv = mem_load<T>(mat, d.ldS + P, mf);
             if (has_b1 && r + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);
             if (d.has_m) {
@@ -145,12 +156,18 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
                 Dv = A::one();
             }
         }
-#pragma unroll
-        for (int k = 0; k < 5; k++) sv[k * RL + r] = v[k];
-        sld[r] = ldv;
-        sud[r] = udv;
+        sl[r] = ldv;
+        sh[r] = udv;
         sD[r] = Dv;
         sE[r] = Ev;
+        if (format == 1) {
+            const int t = r / R, i = r % R;
+            table[tab_coef(0, i, t, 0, R, L)] = v[0];
+            table[tab_coef(0, i, t, 1, R, L)] = v[1];
+            table[tab_coef(1, i, t, 0, R, L)] = v[2];
+            table[tab_coef(1, i, t, 1, R, L)] = v[3];
+            table[tab_coef(2, i, t, 0, R, L)] = v[4];
+        }
     }
     __syncthreads();
 
@@ -166,7 +183,7 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
             if (active) {
                 for (int i = 0; i < R; i++) {
                     const int r = t * R + i;
-                    T m = (r >= 1 && r < N) ? A::mul(sld[r], sud[r - 1]) : A::zero();
+                    T m = (r >= 1 && r < N) ? A::mul(sl[r], sh[r - 1]) : A::zero();
                     T na = A::msub(a, m, c), nb = A::msub(b, m, e);
                     c = a;
                     e = b;
@@ -202,20 +219,25 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
             // exclusive: composition of the maps of lanes 0..t-1 applied to (1,1)
             const int src = (t >= 1) ? t - 1 : t;
             T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
-            T pprev = A::one();
-            if (t >= 1) pprev = A::div(A::add(pa, pb), A::add(pc, pe));
+            // g_{r-1} = 1 / p_{r-1} of the row before this lane's first
+            T gprev = A::one();
+            if (t >= 1) gprev = A::div(A::add(pc, pe), A::add(pa, pb));
+            // ud of that row is read before any lane overwrites its ud with h
+            T udprev = (active && t >= 1) ? sh[t * R - 1] : A::zero();
             if (active) {
                 for (int i = 0; i < R; i++) {
                     const int r = t * R + i;
+                    const T ldr = sl[r], udr = sh[r];
                     T l = A::zero(), g = A::zero(), h = A::zero();
                     if (r < N) {
-                        T m = (r >= 1) ? A::mul(sld[r], sud[r - 1]) : A::zero();
-                        l = (r >= 1) ? A::div(sld[r], pprev) : A::zero();
-                        T p = A::sub(A::one(), A::div(m, pprev));
+                        // l_r = ld_r / p_{r-1},  p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r,  h_r = ud_r g_r
+                        l = (r >= 1) ? A::mul(ldr, gprev) : A::zero();
+                        const T p = A::sub(A::one(), A::mul(l, udprev));
                         g = A::div(A::one(), p);
-                        h = A::mul(sud[r], g);
-                        pprev = p;
+                        h = A::mul(udr, g);
+                        gprev = g;
                     }
+                    udprev = udr;
                     sl[r] = l;
                     sg[r] = g;
                     sh[r] = h;
@@ -225,21 +247,25 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
     } else {
         for (int r = tid; r < RL; r += nthr) {
             const bool in = r < N;
-            sl[r] = (in && d.solve_kind == JW_SOLVE_LOWER) ? sld[r] : A::zero();
+            if (!(in && d.solve_kind == JW_SOLVE_LOWER)) sl[r] = A::zero();
             sg[r] = in ? A::one() : A::zero();
-            sh[r] = (in && d.solve_kind == JW_SOLVE_UPPER) ? sud[r] : A::zero();
+            if (!(in && d.solve_kind == JW_SOLVE_UPPER)) sh[r] = A::zero();
         }
     }
     __syncthreads();
 
     if (format == 2) {
-        // fold the scalings (row-parallel); l' and h' go to the raw ld/ud arrays
+        // fold the scalings (row-parallel); l' and h' replace l and h in LDS
         for (int r = tid; r < RL; r += nthr) {
             const int t = r / R, i = r % R;
             const T ar = A::mul(sD[r], sg[r]);
             T cf[8];
+            T v[5];
 #pragma unroll
-            for (int k = 0; k < 5; k++) cf[k] = A::mul(ar, sv[k * RL + r]);
+            for (int k = 0; k < 5; k++) v[k] = A::zero();
+            if (r < N) row_band<T>(d, mat, r, N, d.permuted ? reorder_pos(r, N) : (int64_t)r, v);
+#pragma unroll
+            for (int k = 0; k < 5; k++) cf[k] = A::mul(ar, v[k]);
             T lp = A::zero(), hp = A::zero();
             if (r >= 1 && r < N) lp = A::div(A::mul(ar, sl[r]), A::mul(sD[r - 1], sg[r - 1]));
             if (r + 1 < N) hp = A::div(A::mul(sD[r], sh[r]), sD[r + 1]);
@@ -251,20 +277,17 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
                 table[tab_coef64(p, i, t, R)] = cf[2 * p];
                 table[tab_coef64(p, i, t, R) + 1] = cf[2 * p + 1];
             }
+            sl[r] = lp;
+            sh[r] = hp;
         }
         __syncthreads();
-        // l', h' recomputed into LDS by the owning lanes for the per-lane products
         if (tid < 64) {
             const int t = tid;
             T bf = A::one(), bb1 = A::one(), bb2 = A::one();
             for (int i = 0; i < R; i++) {
                 const int r = t * R + i;
-                const T ar = A::mul(sD[r], sg[r]);
-                T lp = A::zero(), hp = A::zero();
-                if (r >= 1 && r < N) lp = A::div(A::mul(ar, sl[r]), A::mul(sD[r - 1], sg[r - 1]));
-                if (r + 1 < N) hp = A::div(A::mul(sD[r], sh[r]), sD[r + 1]);
-                bf = A::mul(bf, A::neg(lp));
-                bb1 = A::mul(bb1, A::neg(hp));
+                bf = A::mul(bf, A::neg(sl[r]));
+                bb1 = A::mul(bb1, A::neg(sh[r]));
                 bb2 = A::mul(bb2, A::neg(sE[r]));
             }
             // in-row Hillis-Steele: before step k the value covers the 2^k lanes a
@@ -320,13 +343,9 @@ __device__ void prologue_body(const jw_source_desc& d, const void* mat, T* table
     }
 
     // format 1: pairs (vm2,vm1) (v0,vp1) (vp2,l) (g,h) (D,E) + Hillis-Steele multipliers
+    // (the band pairs were written in phase 1)
     for (int r = tid; r < RL; r += nthr) {
         const int t = r / R, i = r % R;
-        table[tab_coef(0, i, t, 0, R, L)] = sv[0 * RL + r];
-        table[tab_coef(0, i, t, 1, R, L)] = sv[1 * RL + r];
-        table[tab_coef(1, i, t, 0, R, L)] = sv[2 * RL + r];
-        table[tab_coef(1, i, t, 1, R, L)] = sv[3 * RL + r];
-        table[tab_coef(2, i, t, 0, R, L)] = sv[4 * RL + r];
         table[tab_coef(2, i, t, 1, R, L)] = sl[r];
         table[tab_coef(3, i, t, 0, R, L)] = sg[r];
         table[tab_coef(3, i, t, 1, R, L)] = sh[r];

@@ -132,6 +132,7 @@ int jwdev_main64_occupancy(int R, int waves);
 int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream);
 /* persistent uneven-batch kernel over a unit queue; *d_counter must be 0 at launch */
 int jwdev_batch64_occupancy(int R, int waves);
+int jwdev_zero_counters(int* d_counters, int n, void* stream);
 int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
                   int waves, int grid, void* stream);
 int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream);

@@ -463,6 +463,12 @@ const void* pick64(int R, bool all3 = true)
     }
 }
 
+// clears the work-queue counters (a kernel, so that it can sit in a captured graph)
+__global__ void jw_zero_counters_kernel(int* p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 template <int R>
 const void* batch_ptr()
 {
@@ -561,6 +567,14 @@ int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, in
                     &d_counter};
     e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : -3;
+}
+
+int jwdev_zero_counters(int* d_counters, int n, void* stream)
+{
+    if (n < 1 || n > 64) return -1;
+    hipLaunchKernelGGL(jw_zero_counters_kernel, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
+                       d_counters, n);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 }  // extern "C"

@@ -251,3 +251,24 @@ def test_batch_harmonic_degrees_full_size():
     d.x = x1 + 2.0 * x2
     y12 = run_batch(d)
     assert O.relerr_norm(y12, a + 2.0 * y2) <= 1e-13
+
+
+def test_batch_prepare_then_solve_matches_run():
+    """tables prepared once and kept resident: every solve equals a full run"""
+    t = torch()
+    d = BatchData(SMALL_SETS, seed=7)
+    ref = run_batch(d)
+    stream = t.cuda.current_stream().cuda_stream
+    with pf.Batch(d.sets) as b:
+        b.prepare(stream)
+        for _ in range(2):
+            d.reset()
+            b.solve(stream)
+            np.testing.assert_array_equal(d.result(), ref)
+        s = t.cuda.Stream()
+        b.capture_graph(s.cuda_stream, solve_only=True)
+        for rep in range(3):
+            d.reset()
+            b.launch_graph(s.cuda_stream)
+            s.synchronize()
+            np.testing.assert_array_equal(d.result(), ref, err_msg=f"replay {rep}")
