@@ -247,8 +247,12 @@ def main() -> int:
     plan.app.configuration.outputBufferSize = C.cast(osize, pf._abi.u64p)
     geo = plan.geometry()
     fast = geo.fast_rows_per_lane > 0
-    kname = (f"jw_main64_kernel<{geo.fast_rows_per_lane}>" if fast
-             else f"jw_main_kernel<double,{geo.rows_per_lane},{geo.lanes}>")
+    if fast and geo.fast_store_waves:
+        kname = f"jw_main64s_kernel<{geo.fast_rows_per_lane}, true>"
+    elif fast:
+        kname = f"jw_main64_kernel<{geo.fast_rows_per_lane}, true>"
+    else:
+        kname = f"jw_main_kernel<double,{geo.rows_per_lane},{geo.lanes}>"
     launch = dict(scaleC=SCALE_C, inputZeropad=list(ZP))
 
     for _ in range(args.warmup):
@@ -316,6 +320,7 @@ def main() -> int:
             "lanes": 64 if fast else geo.lanes,
             "rows_per_lane": geo.fast_rows_per_lane if fast else geo.rows_per_lane,
             "waves_per_workgroup": geo.fast_waves if fast else geo.waves,
+            "store_waves_per_workgroup": geo.fast_store_waves if fast else 0,
         },
         "p50_ms": float(np.percentile(steps_ms, 50)),
         "p99_ms": float(np.percentile(steps_ms, 99)),

@@ -42,7 +42,9 @@ typedef struct pfsolve_geometry {
     int64_t table_bytes;      /* coefficient table per matrix set */
     int64_t lds_bytes;        /* LDS per workgroup */
     int32_t fast_rows_per_lane;   /* wave-per-system FP64 kernel: R (0 = not used) */
-    int32_t fast_waves;           /* ... and its waves per workgroup */
+    int32_t fast_waves;           /* ... and its computing waves per workgroup */
+    int32_t fast_store_waves;     /* ... storing waves per workgroup (0: each wave stores its own) */
+    int32_t pad;
 } pfsolve_geometry;
 
 /* Validates a configuration as initializePfSolve does, without touching the

@@ -224,6 +224,8 @@ class PfSolveGeometry(C.Structure):
         ("lds_bytes", i64),
         ("fast_rows_per_lane", C.c_int32),
         ("fast_waves", C.c_int32),
+        ("fast_store_waves", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 

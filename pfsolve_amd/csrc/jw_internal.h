@@ -130,6 +130,12 @@ int jwdev_main64_waves(int R);
 int64_t jwdev_main64_lds_bytes(int R, int waves);
 int jwdev_main64_occupancy(int R, int waves);
 int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream);
+/* the same with compute / store split (R <= 8): compute wavefronts never store,
+ * storer wavefronts write their results; *d_err is raised if a bounded wait ran out */
+int jwdev_main64s_shape(int R, int* ncomp, int* nstore);
+int64_t jwdev_main64s_lds_bytes(int R, int ncomp);
+int jwdev_main64s_occupancy(int R);
+int jwdev_main64s(const jw_run_desc* r, int R, int grid, int* d_err, void* stream);
 /* persistent uneven-batch kernel over a unit queue; *d_counter must be 0 at launch */
 int jwdev_batch64_occupancy(int R, int waves);
 int jwdev_zero_counters(int* d_counters, int n, void* stream);

@@ -129,21 +129,18 @@ __device__ __forceinline__ void zero_staging(double2* stg, int lane)
     for (int k = 0; k < R; k++) stg[64 * k + lane] = make_double2(0.0, 0.0);
 }
 
-// One pass: systems sys0 and sys0+1 (if it exists) through the whole step.
-// tab: the matrix set's table in LDS; stg: this wavefront's staging (2 systems),
-// whose slots outside the input keep-range hold zeros on entry and on exit.
-// ALL3: all three recurrences present (the fused step), decided at compile time.
-template <int R, bool ALL3>
-__device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int lane, const jw_pass64& g,
-                                          int64_t sys0)
+// The pieces of one pass over systems sys0 and sys0+1 (if it exists).  stg: a
+// wavefront's staging for 2 systems (64 R double2); the input slots outside the
+// keep-range hold zeros whenever the input DMA is issued.
+
+// issue the input DMA (global -> LDS) of the pair; completion is vmcnt(0)
+template <int R>
+__device__ __forceinline__ void dma_pair(double2* stg, int lane, const jw_pass64& g, int64_t sys0)
 {
-    constexpr int H = R / 2;                  // 16-byte chunks per lane per system
-    constexpr int SYSC = 64 * H;              // chunks per system region
-    constexpr int SC = 8 * R * 64;            // start of the scan multipliers
+    constexpr int H = R / 2;
+    constexpr int SYSC = 64 * H;
     const bool ok1 = sys0 + 1 < g.nsys;
     const double* src0 = g.rhs + sys0 * g.stride + g.offSol;
-
-    // ---- 1. input ----
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         if (s == 1 && !ok1) continue;
@@ -156,10 +153,18 @@ __device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int l
                     src + 2 * c, (__attribute__((address_space(3))) void*)(stg + s * SYSC + 64 * k), 16, 0, 0);
         }
     }
-    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the DMA writes have landed in LDS
-    compiler_fence();
+}
 
-    double x0[R], x1[R];
+// blocked read-back of the landed input (lane t: rows t R .. t R + R-1), odd range
+// ends read from global memory by their owning lane
+template <int R>
+__device__ __forceinline__ void read_pair(const double2* stg, int lane, const jw_pass64& g, int64_t sys0,
+                                          double (&x0)[R], double (&x1)[R])
+{
+    constexpr int H = R / 2;
+    constexpr int SYSC = 64 * H;
+    const bool ok1 = sys0 + 1 < g.nsys;
+    const double* src0 = g.rhs + sys0 * g.stride + g.offSol;
 #pragma unroll
     for (int m = 0; m < H; m++) {
         const double2 v0 = stg[swz<H>(lane * H + m)];
@@ -187,6 +192,15 @@ __device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int l
         }
     }
 
+}
+
+// banded matvec, the three recurrences and row-0 scaleC, in registers.
+// ALL3: all three recurrences present (the fused step), decided at compile time.
+template <int R, bool ALL3>
+__device__ __forceinline__ void compute_pair(const double* tab, int lane, const jw_pass64& g, double (&x0)[R],
+                                             double (&x1)[R])
+{
+    constexpr int SC = 8 * R * 64;            // start of the scan multipliers
     // ---- 2. banded matvec (coefficients carry the folded row scaling) ----
     {
         double m1a = 0.0, m2a = 0.0, p1a = 0.0, p2a = 0.0;
@@ -326,13 +340,30 @@ __device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int l
         x1[0] *= g.row0_scale;
     }
 
-    // ---- 4. output: blocked -> LDS -> contiguous 16-byte stores ----
+}
+
+// results written blocked into the staging
+template <int R>
+__device__ __forceinline__ void write_blocked(double2* stg, int lane, const double (&x0)[R], const double (&x1)[R])
+{
+    constexpr int H = R / 2;
+    constexpr int SYSC = 64 * H;
     compiler_fence();
 #pragma unroll
     for (int m = 0; m < H; m++) {
         stg[swz<H>(lane * H + m)] = make_double2(x0[2 * m], x0[2 * m + 1]);
         stg[SYSC + swz<H>(lane * H + m)] = make_double2(x1[2 * m], x1[2 * m + 1]);
     }
+}
+
+// contiguous read of the staging and 16-byte stores of the output keep-range;
+// odd range ends stored by lane 0
+template <int R>
+__device__ __forceinline__ void store_pair(const double2* stg, int lane, const jw_pass64& g, int64_t sys0)
+{
+    constexpr int H = R / 2;
+    constexpr int SYSC = 64 * H;
+    const bool ok1 = sys0 + 1 < g.nsys;
     compiler_fence();
 #pragma unroll
     for (int s = 0; s < 2; s++) {
@@ -350,14 +381,25 @@ __device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int l
         for (int e = 0; e < 2; e++) {
             const bool on = e == 0 ? g.out_edge_lo : g.out_edge_hi;
             const int r = e == 0 ? g.out_lo : g.out_hi - 1;
-            if (on && lane == r / R) {
+            if (on && lane == 0) {
                 double* dst = g.rhs + sys0 * g.stride + g.offSol + r;
-                dst[0] = pick_row<R>(x0, r % R);
-                if (ok1) dst[g.stride] = pick_row<R>(x1, r % R);
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    if (s == 1 && !ok1) continue;
+                    const double2 v = stg[s * SYSC + swz<H>(r >> 1)];
+                    dst[s * g.stride] = (r & 1) ? v.y : v.x;
+                }
             }
         }
     }
-    // restore the zeros of the slots the input DMA skips
+}
+
+// zeros back into the slots the input DMA skips
+template <int R>
+__device__ __forceinline__ void rezero_input(double2* stg, int lane, const jw_pass64& g)
+{
+    constexpr int H = R / 2;
+    constexpr int SYSC = 64 * H;
 #pragma unroll
     for (int s = 0; s < 2; s++) {
 #pragma unroll
@@ -366,6 +408,22 @@ __device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int l
             if (!(c >= g.icl && c < g.ich)) stg[s * SYSC + 64 * k + lane] = make_double2(0.0, 0.0);
         }
     }
+}
+
+// One whole pass in one wavefront (input, compute, output through one staging).
+template <int R, bool ALL3>
+__device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int lane, const jw_pass64& g,
+                                          int64_t sys0)
+{
+    dma_pair<R>(stg, lane, g, sys0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the DMA writes have landed in LDS
+    compiler_fence();
+    double x0[R], x1[R];
+    read_pair<R>(stg, lane, g, sys0, x0, x1);
+    compute_pair<R, ALL3>(tab, lane, g, x0, x1);
+    write_blocked<R>(stg, lane, x0, x1);
+    store_pair<R>(stg, lane, g, sys0);
+    rezero_input<R>(stg, lane, g);
     __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): LDS traffic done before the next DMA lands
     compiler_fence();
 }
@@ -396,6 +454,189 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc
                                         (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
                                         d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
         jw_pair64<R, ALL3>(tab, stg, lane, g, 2 * q);
+    }
+}
+
+// input of a pair fetched into registers, in the layout of the DMA (slot 64k + lane
+// of system s holds chunk swz(64k + lane)); chunks outside the keep-range are zero
+template <int R>
+__device__ __forceinline__ void load_pair(double2 (&pf)[2][R / 2], int lane, const jw_pass64& g, int64_t sys0)
+{
+    constexpr int H = R / 2;
+    const bool ok1 = sys0 + 1 < g.nsys;
+    const double2* src0 = reinterpret_cast<const double2*>(g.rhs) + ((sys0 * g.stride + g.offSol) >> 1);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const double2* src = src0 + s * (g.stride >> 1);
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            const int c = swz<H>(64 * k + lane);
+            double2 v = make_double2(0.0, 0.0);
+            if (c >= g.icl && c < g.ich && (s == 0 || ok1)) v = src[c];
+            pf[s][k] = v;
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void stage_pair(double2* stg, int lane, const double2 (&pf)[2][R / 2])
+{
+    constexpr int H = R / 2;
+    constexpr int SYSC = 64 * H;
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int k = 0; k < H; k++) stg[s * SYSC + 64 * k + lane] = pf[s][k];
+}
+
+// Compute / store split.  The first `ncomp` wavefronts of a workgroup compute,
+// the others store.  A compute wavefront owns an input and an output staging;
+// per pass it moves the input chunks it fetched into registers one pass
+// earlier through its input staging into the blocked layout, issues the loads
+// of its next pair at once, computes, waits until a storer has emptied its
+// output staging, writes the results there and raises its `full` flag.  Storer wavefronts poll the flags of their
+// compute wavefronts, store full stagings to global memory and lower the flag.
+// Compute wavefronts issue no global stores, so their vmcnt waits never wait
+// for write acknowledgements; stores stay in flight while compute goes on.
+// Every wait on a flag is bounded (kSpinLimit polls without progress); a
+// wavefront that runs out stops and raises *err, so the grid always drains.
+constexpr int kSpinLimit = 1 << 22;
+
+// Flags and pair indices are relaxed workgroup-scope atomics in LDS: they
+// compile to plain LDS accesses with no wait on outstanding global loads (a
+// volatile access would wait for them and stall the prefetch).  The writer
+// waits lgkmcnt(0) for its staging writes before it raises a flag.
+__device__ __forceinline__ int lds_flag(int* f)
+{
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+__device__ __forceinline__ void lds_set(int* f, int v)
+{
+    __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// waits until *f == v; false when the limit ran out
+__device__ __forceinline__ bool wait_flag(int* f, int v)
+{
+    for (int it = 0; it < kSpinLimit; it++) {
+        if (lds_flag(f) == v) return true;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return false;
+}
+
+template <int R>
+constexpr int split_threads()
+{
+    return R <= 4 ? 1024 : (R == 8 ? 576 : 320);
+}
+
+template <int R, bool ALL3>
+__global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_desc d, int ncomp, int* err)
+{
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_full[16], s_done[16];
+    __shared__ long long s_pair[16];
+    const double* tab = reinterpret_cast<const double*>(smem);
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(d.table);
+        uint4* dst = reinterpret_cast<uint4*>(smem);
+        constexpr int n16 = NTAB * 8 / 16;
+        for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* in_base = reinterpret_cast<double2*>(smem + NTAB * 8);
+    double2* out_base = in_base + (size_t)ncomp * 64 * R;
+    if (wave < ncomp) {
+        if (lane == 0) {
+            lds_set(&s_full[wave], 0);
+            lds_set(&s_done[wave], 0);
+        }
+    }
+    __syncthreads();
+    const int64_t npass = (d.nsys + 1) >> 1;
+    int* vfull = s_full;
+    int* vdone = s_done;
+
+    if (wave < ncomp) {
+        double2* sin = in_base + (size_t)wave * 64 * R;
+        double2* sout = out_base + (size_t)wave * 64 * R;
+        const int64_t step = (int64_t)gridDim.x * ncomp;
+        int64_t q = (int64_t)blockIdx.x * ncomp + wave;
+        bool ok = true;
+        double2 pf[2][R / 2];
+        if (q < npass) {
+            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
+                                            (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
+                                            d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+            load_pair<R>(pf, lane, g, 2 * q);
+        }
+        while (q < npass) {
+            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
+                                            (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
+                                            d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+            // the prefetched chunks (loaded a whole pass ago) through LDS into the blocked layout
+            stage_pair<R>(sin, lane, pf);
+            double x0[R], x1[R];
+            read_pair<R>(sin, lane, g, 2 * q, x0, x1);
+            const int64_t qn = q + step;
+            if (qn < npass) load_pair<R>(pf, lane, g, 2 * qn);
+            compute_pair<R, ALL3>(tab, lane, g, x0, x1);
+            if (!wait_flag(vfull + wave, 0)) { ok = false; break; }
+            write_blocked<R>(sout, lane, x0, x1);
+            if (lane == 0) __hip_atomic_store(&s_pair[wave], (long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): staging and pair index written
+            compiler_fence();
+            if (lane == 0) lds_set(vfull + wave, 1);
+            q = qn;
+        }
+        if (ok) ok = wait_flag(vfull + wave, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        compiler_fence();
+        if (lane == 0) {
+            lds_set(vdone + wave, 1);
+            if (!ok) *err = 1;
+        }
+        return;
+    }
+
+    // storer wavefront j serves compute wavefronts j, j + nstore, ...
+    const int nstore = nwaves - ncomp;
+    const int j = wave - ncomp;
+    int idle = 0;
+    for (;;) {
+        bool all_done = true, progress = false;
+        for (int w = j; w < ncomp; w += nstore) {
+            if (lds_flag(vfull + w)) {
+                const long long qq = __hip_atomic_load(&s_pair[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const int64_t q = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)qq) |
+                                  ((int64_t)__builtin_amdgcn_readfirstlane((int)(qq >> 32)) << 32);
+                const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
+                                                (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl,
+                                                d.ku, d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+                store_pair<R>(out_base + (size_t)w * 64 * R, lane, g, 2 * q);
+                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the staging has been read
+                compiler_fence();
+                if (lane == 0) lds_set(vfull + w, 0);
+                progress = true;
+                all_done = false;
+            } else if (!lds_flag(vdone + w)) {
+                all_done = false;   // the flag is read before `done`: a raised done with an empty staging is final
+            }
+        }
+        if (all_done) break;
+        if (progress) {
+            idle = 0;
+        } else {
+            if (++idle >= kSpinLimit) {
+                if (lane == 0) *err = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
     }
 }
 
@@ -484,6 +725,41 @@ const void* pick_batch64(int R)
     case 16: return batch_ptr<16>();
     default: return nullptr;
     }
+}
+
+template <int R>
+const void* split_ptr(bool all3)
+{
+    return all3 ? reinterpret_cast<const void*>(&jw_main64s_kernel<R, true>)
+                : reinterpret_cast<const void*>(&jw_main64s_kernel<R, false>);
+}
+
+const void* pick_split(int R, bool all3)
+{
+    switch (R) {
+    case 2: return split_ptr<2>(all3);
+    case 4: return split_ptr<4>(all3);
+    case 8: return split_ptr<8>(all3);
+    default: return nullptr;
+    }
+}
+
+// compute / storer wavefronts of the split kernel: as many compute wavefronts as
+// LDS holds (table + input and output staging each), within the launch bound
+bool split_shape(int R, int* ncomp, int* nstore)
+{
+    int maxw = R <= 4 ? 16 : (R == 8 ? 9 : 5);
+    const int64_t table = jwdev_table64_elems(R) * 8;
+    const int64_t per = 2LL * 2 * 64 * R * 8;   // input + output staging of 2 systems
+    const int64_t avail = kLdsBytesMax - 256 - table;
+    int c = (int)(avail / per);
+    int st = c >= 12 ? 2 : (c >= 6 ? 2 : 1);
+    if (c + st > maxw) c = maxw - st;
+    if (c > 16) c = 16;
+    if (c < 1) return false;
+    *ncomp = c;
+    *nstore = st;
+    return true;
 }
 
 int max_waves64(int R) { return R <= 4 ? 16 : (R == 8 ? 12 : 5); }
@@ -575,6 +851,47 @@ int jwdev_zero_counters(int* d_counters, int n, void* stream)
     hipLaunchKernelGGL(jw_zero_counters_kernel, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
                        d_counters, n);
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+/* compute / store split kernel (R <= 8): shape, occupancy, launch */
+int jwdev_main64s_shape(int R, int* ncomp, int* nstore)
+{
+    if (!pick_split(R, true)) return 0;
+    return split_shape(R, ncomp, nstore) ? 1 : 0;
+}
+
+int64_t jwdev_main64s_lds_bytes(int R, int ncomp)
+{
+    return jwdev_table64_elems(R) * 8 + 2LL * 2 * 64 * R * 8 * ncomp;
+}
+
+int jwdev_main64s_occupancy(int R)
+{
+    int c = 0, st = 0;
+    if (!jwdev_main64s_shape(R, &c, &st)) return 0;
+    int best = -1;
+    for (int v = 0; v < 2; v++) {
+        const void* f = pick_split(R, v == 0);
+        const int lds = (int)jwdev_main64s_lds_bytes(R, c);
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * (c + st), lds) != hipSuccess) return 0;
+        if (best < 0 || nb < best) best = nb;
+    }
+    return best;
+}
+
+int jwdev_main64s(const jw_run_desc* r, int R, int grid, int* d_err, void* stream)
+{
+    int c = 0, st = 0;
+    if (!jwdev_main64s_shape(R, &c, &st)) { snprintf(g_err64, sizeof(g_err64), "no split kernel for R=%d", R); return -1; }
+    const void* f = pick_split(R, r->fwd && r->b1 && r->b2);
+    const size_t lds = (size_t)jwdev_main64s_lds_bytes(R, c);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return -2;
+    void* args[] = {const_cast<jw_run_desc*>(r), &c, &d_err};
+    e = hipLaunchKernel(f, dim3(grid), dim3(64 * (c + st)), args, lds, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? 0 : -3;
 }
 
 }  // extern "C"

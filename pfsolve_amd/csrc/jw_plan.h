@@ -25,6 +25,10 @@ struct pfsolve_plan {
     /* wave-per-system FP64 path (table format 2): rows per lane (0 = not available),
      * waves per workgroup and workgroups per CU */
     int R64, waves64, occupancy64;
+    /* compute / store split kernel for R64 <= 8 (0 = not used), its workgroups per
+     * CU, and the device flag it raises when a bounded wait runs out */
+    int split64, occupancy_split;
+    int* d_err;
     hipEvent_t ev_start, ev_stop;   /* optional timing events around the main kernel */
 };
 

@@ -60,6 +60,7 @@ void deletePfSolve(PfSolveApplication* app)
     struct pfsolve_plan* p = app->localFFTPlan;
     if (p) {
         if (p->table) hipFree(p->table);
+        if (p->d_err) hipFree(p->d_err);
         free(p);
     }
     app->localFFTPlan = NULL;
@@ -121,6 +122,16 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
             }
         }
         if (cfg->kernelVariant == 2 && !p->R64) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
+        /* compute / store split form of the wave-per-system kernel (R <= 8) */
+        if (p->R64 && p->R64 <= 8) {
+            int c = 0, st = 0;
+            if (jwdev_main64s_shape(p->R64, &c, &st)) {
+                p->occupancy_split = jwdev_main64s_occupancy(p->R64);
+                if (p->occupancy_split >= 1) p->split64 = c;
+            }
+        }
+        if (hipMalloc((void**)&p->d_err, 256) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
+        if (hipMemset(p->d_err, 0, 256) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_COPY; }
         if (hipMalloc(&p->table, p->table_bytes) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
         p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
         if (p->occupancy < 1) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY; }
@@ -185,12 +196,17 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
                 return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             if (nsys <= 0) return PFSOLVE_SUCCESS;
             const int64_t npass = (nsys + 1) / 2;
-            int64_t need = (npass + p->waves64 - 1) / p->waves64;
-            int64_t grid = (int64_t)p->cu_count * p->occupancy64;
+            const int per_wg = p->split64 ? p->split64 : p->waves64;
+            int64_t need = (npass + per_wg - 1) / per_wg;
+            int64_t grid = (int64_t)p->cu_count * (p->split64 ? p->occupancy_split : p->occupancy64);
             if (grid > need) grid = need;
             if (grid < 1) grid = 1;
             if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
-            if (jwdev_main64(&run, p->R64, p->waves64, (int)grid, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            if (p->split64) {
+                if (jwdev_main64s(&run, p->R64, (int)grid, p->d_err, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            } else if (jwdev_main64(&run, p->R64, p->waves64, (int)grid, s) != 0) {
+                return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            }
             if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
             return PFSOLVE_SUCCESS;
         }
@@ -258,11 +274,23 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
     return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
 }
 
-/* With one stream per plan there is nothing to join (pfSolve_RunApp.h:28-57 waits on
- * per-stream events only when num_streams > 1). */
+/* Waits for the plan's stream (pfSolve_RunApp.h:28-57 joins the plan's streams) and
+ * reports a raised device error flag (a bounded wait inside a kernel ran out). */
 PfSolveResult PfSolveSync(PfSolveApplication* app)
 {
     if (!app) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    struct pfsolve_plan* p = app->localFFTPlan;
+    if (!p) return PFSOLVE_SUCCESS;   /* as the reference: nothing to join */
+    if (hipStreamSynchronize(plan_stream(app)) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+    if (p->d_err) {
+        int e = 0;
+        if (hipMemcpy(&e, p->d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+            return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+        if (e) {
+            (void)hipMemset(p->d_err, 0, sizeof(int));
+            return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+        }
+    }
     return PFSOLVE_SUCCESS;
 }
 
@@ -440,7 +468,11 @@ PfSolveResult pfsolve_plan_geometry(const PfSolveApplication* app, pfsolve_geome
     g->table_bytes = (int64_t)p->table_bytes;
     g->lds_bytes = p->lds_bytes;
     g->fast_rows_per_lane = p->R64;
-    g->fast_waves = p->waves64;
+    g->fast_waves = p->split64 ? p->split64 : p->waves64;
+    if (p->split64) {
+        int c = 0, st = 0;
+        if (jwdev_main64s_shape(p->R64, &c, &st)) g->fast_store_waves = st;
+    }
     return PFSOLVE_SUCCESS;
 }
 

@@ -355,10 +355,34 @@ def test_wave_per_system_kernel_is_selected_for_aligned_fp64():
     with pf.Plan(dmat.data_ptr(), x.data_ptr(), size=[N, B, 1], jw_type=14, doublePrecision=1,
                  offsetV=offV, offsetT=offT, offsetM=offM) as plan:
         g = plan.geometry()
-        assert g.fast_rows_per_lane == 8 and g.fast_waves >= 4
+        # compute / store split: computing and storing wavefronts
+        assert g.fast_rows_per_lane == 8 and g.fast_waves >= 4 and g.fast_store_waves >= 1
+        plan.append()
+        assert pf.PfSolveSync(plan.app) == 0   # no bounded wait ran out
     with pf.Plan(dmat.data_ptr(), x.data_ptr(), size=[N, B, 1], jw_type=14, doublePrecision=1,
                  offsetV=offV, offsetT=offT, offsetM=offM, kernelVariant=1) as plan:
         assert plan.geometry().fast_rows_per_lane == 0
+
+
+def test_split_kernel_many_passes_per_wavefront():
+    """enough pairs that every computing wavefront hands many outputs to its storer:
+    N = 256 (R = 4) and N = 100 (R = 2), 40000 real systems, a sample of them against
+    the oracle"""
+    import torch
+    for N in (256, 100):
+        B = 40000
+        raw, buf, (offV, offT, offM) = fused_buffers(N, 11, 1)
+        x = O.uniform(17 + N, -1, 1, B * N)
+        cfg = dict(size=[N, B, 1], jw_type=14, doublePrecision=1, scaleC=1.0, offsetV=offV, offsetT=offT,
+                   offsetM=offM)
+        got = run_plan(cfg, buf, x)
+        idx = list(range(0, B, 997)) + [B - 2, B - 1]
+        sub = np.concatenate([x[b * N:(b + 1) * N] for b in idx])
+        p = O.problem(jw_type=14, N=N, B=len(idx), KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                      cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+        want = O.jw_f128(p, buf, sub)
+        for j, b in enumerate(idx):
+            assert O.relerr_norm(got[b * N:(b + 1) * N], want[j * N:(j + 1) * N]) <= TOL64, (N, b)
 
 
 @pytest.mark.parametrize("jw_type,ub", MODES)
