@@ -365,6 +365,17 @@ __device__ __forceinline__ void store_pair(const double2* stg, int lane, const j
     constexpr int SYSC = 64 * H;
     const bool ok1 = sys0 + 1 < g.nsys;
     compiler_fence();
+    // all LDS reads first (always in bounds), so that they overlap, then the stores
+    double2 v[2][H];
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int k = 0; k < H; k++) v[s][k] = stg[s * SYSC + swz<H>(64 * k + lane)];
+    // keeps the compiler from sinking each read into its store's branch
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int k = 0; k < H; k++) asm volatile("" : "+v"(v[s][k].x), "+v"(v[s][k].y));
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         if (s == 1 && !ok1) continue;
@@ -373,7 +384,7 @@ __device__ __forceinline__ void store_pair(const double2* stg, int lane, const j
 #pragma unroll
         for (int k = 0; k < H; k++) {
             const int c = 64 * k + lane;
-            if (c >= g.ocl && c < g.och) dst2[c] = stg[s * SYSC + swz<H>(c)];
+            if (c >= g.ocl && c < g.och) dst2[c] = v[s][k];
         }
     }
     if (g.out_edge_lo || g.out_edge_hi) {
@@ -528,7 +539,7 @@ __device__ __forceinline__ bool wait_flag(int* f, int v)
 template <int R>
 constexpr int split_threads()
 {
-    return R <= 4 ? 1024 : (R == 8 ? 576 : 320);
+    return R <= 4 ? 1024 : (R == 8 ? 768 : 320);
 }
 
 template <int R, bool ALL3>
@@ -744,21 +755,23 @@ const void* pick_split(int R, bool all3)
     }
 }
 
-// compute / storer wavefronts of the split kernel: as many compute wavefronts as
-// LDS holds (table + input and output staging each), within the launch bound
+// compute / storer wavefronts of the split kernel.  A storer wavefront moves one
+// pair's output (64 R 16-byte stores) in about the time a compute wavefront
+// needs for R/2 of them, so the workgroup is as many compute wavefronts as LDS
+// holds (table + input and output staging each, at most 7 / 10) and storers
+// for the rest of the launch bound (12 wavefronts for R = 8, 16 below).
 bool split_shape(int R, int* ncomp, int* nstore)
 {
-    int maxw = R <= 4 ? 16 : (R == 8 ? 9 : 5);
+    const int maxw = R <= 4 ? 16 : (R == 8 ? 12 : 5);
+    const int cmax = R <= 4 ? 10 : (R == 8 ? 7 : 2);
     const int64_t table = jwdev_table64_elems(R) * 8;
     const int64_t per = 2LL * 2 * 64 * R * 8;   // input + output staging of 2 systems
     const int64_t avail = kLdsBytesMax - 256 - table;
     int c = (int)(avail / per);
-    int st = c >= 12 ? 2 : (c >= 6 ? 2 : 1);
-    if (c + st > maxw) c = maxw - st;
-    if (c > 16) c = 16;
-    if (c < 1) return false;
+    if (c > cmax) c = cmax;
+    if (c < 1 || c >= maxw) return false;
     *ncomp = c;
-    *nstore = st;
+    *nstore = maxw - c;
     return true;
 }
 
