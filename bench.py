@@ -185,6 +185,94 @@ def bench_batch(torch, pf, steps: int, warmup: int) -> dict:
     return out
 
 
+def to_dd(q):
+    """exact rational -> (hi, lo) double-double, round to nearest"""
+    from fractions import Fraction
+    hi = float(q)
+    lo = float(q - Fraction(hi))
+    return hi, lo
+
+
+def gen_fused_matrices_dd(N: int, seed: int, reorder):
+    """The throughput workload's distributions as double-double: FP64 draws with a low
+    word of U(-1,1) 2^-60 |hi|; the stored T block [a/b | 1/b | c/b] and M block
+    [1/d | e/d] are formed exactly (rationals) and rounded to double-double."""
+    from fractions import Fraction
+    rng = np.random.default_rng(seed)
+    lo_rng = np.random.default_rng(seed + 1)
+
+    def draw(a, b, n):
+        hi = rng.uniform(a, b, n)
+        lo = lo_rng.uniform(-1, 1, n) * 2.0 ** -60 * np.abs(hi)
+        return [Fraction(float(h)) + Fraction(float(l)) for h, l in zip(hi, lo)]
+
+    V = [draw(-1, 1, N) for _ in range(5)]
+    a, b, c = draw(-0.5, 0.5, N), draw(2, 3, N), draw(-0.5, 0.5, N)
+    d, e = draw(1, 2, N), draw(-0.5, 0.5, N)
+    a[0] = Fraction(0)
+    c[-1] = Fraction(0)
+    e[-1] = Fraction(0)
+    P = [reorder(i, N, N, 0) for i in range(N)]
+    offV, offT, offM = 0, 5 * N, 8 * N
+    buf = np.zeros((10 * N, 2))
+    for j in range(5):
+        for col in range(N):
+            r = col - 2 + j
+            if 0 <= r < N:
+                buf[offV + j * N + col] = to_dd(V[j][col])
+    for i in range(N):
+        buf[offT + P[i]] = to_dd(a[i] / b[i])
+        buf[offT + N + P[i]] = to_dd(1 / b[i])
+        buf[offT + 2 * N + P[i]] = to_dd(c[i] / b[i])
+        buf[offM + P[i]] = to_dd(1 / d[i])
+        buf[offM + N + P[i]] = to_dd(e[i] / d[i])
+    return buf, (offV, offT, offM)
+
+
+def bench_dd(torch, pf, steps: int, warmup: int, N: int = 256, B: int = 16384) -> dict:
+    """Double-double fused step, N = 256, 16384 real systems, DD storage (16 B per value)."""
+    lib = pf.lib()
+    buf, (offV, offT, offM) = gen_fused_matrices_dd(N, 7, lib.pfsolve_reorder_i0)
+    dmat = torch.from_numpy(buf.ravel()).cuda()
+    g = torch.Generator(device="cuda").manual_seed(3)
+    xh = torch.rand(B * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    xl = (torch.rand(B * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1) * 2.0 ** -60 * xh.abs()
+    x = torch.stack([xh, xl], -1).contiguous()
+    stream = torch.cuda.Stream()
+    plan = pf.Plan(dmat.data_ptr(), x.data_ptr(), stream=stream.cuda_stream, size=[N, B, 1], jw_type=14,
+                   quadDoubleDoublePrecision=1, scaleC=1.0, offsetV=offV, offsetT=offT, offsetM=offM)
+    geo = plan.geometry()
+    for _ in range(warmup):
+        plan.append()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    kst = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ken = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for e in evs + kst + ken:
+        e.record(stream)
+    stream.synchronize()
+    with torch.cuda.stream(stream):
+        for i in range(steps):
+            evs[i].record(stream)
+            lib.pfsolve_set_kernel_events(C.byref(plan.app), C.c_void_p(kst[i].cuda_event),
+                                          C.c_void_p(ken[i].cuda_event))
+            plan.append()
+        evs[steps].record(stream)
+    stream.synchronize()
+    lib.pfsolve_set_kernel_events(C.byref(plan.app), None, None)
+    plan.close()
+    assert torch.isfinite(x).all().item(), "non-finite DD output"
+    ms = evs[0].elapsed_time(evs[steps]) / steps
+    kms = float(np.mean([kst[i].elapsed_time(ken[i]) for i in range(steps)]))
+    alg = B * 16 * (N + N) + 10 * N * 16
+    return {
+        "workload": f"double-double fused step, N={N}, {B} real systems, DD storage, matrices FP64 draws "
+                    "with low words U(-1,1) 2^-60 |hi|",
+        "value": B / (ms * 1e-3), "unit": "systems/s", "ms_per_step": ms,
+        "kernel": f"jw_main_kernel<dd,{geo.rows_per_lane},{geo.lanes}>", "kernel_ms": kms,
+        "algorithmic_bytes": alg, "hbm_frac": alg / (kms * 1e-3) / HBM_PEAK,
+    }
+
+
 def stream_reference(torch, nbytes: int, reps: int = 20) -> dict:
     """What plain streaming kernels reach on this GPU for the same byte count: copy_
     (half read, half written) and fill_ (written only)."""
@@ -214,6 +302,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-size batch measurement")
+    ap.add_argument("--no-dd", action="store_true", help="skip the double-double measurement")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time torch copy_ / fill_ over the same byte count (streaming reference)")
     args = ap.parse_args()
@@ -339,6 +428,8 @@ def main() -> int:
     }
     if not args.no_batch:
         result["batch"] = bench_batch(torch, pf, max(10, K), max(3, args.warmup))
+    if not args.no_dd:
+        result["dd"] = bench_dd(torch, pf, max(5, K // 2), 2)
     if args.calibrate:
         result["stream_reference"] = stream_reference(torch, alg)
     if not args.no_cpu:

@@ -69,13 +69,13 @@ PfSolveResult pfsolve_set_kernel_events(PfSolveApplication* app, hipEvent_t star
  * offsetM, T and M in reorder_i0 order unless naturalOrder) and `nsys` real
  * systems solved in place.  pfsolve_batch_run turns every set's matrices into
  * its coefficient table in one launch, then solves all systems in one
- * persistent launch per row class (N <= 128, 256, 512, 1024): workgroups take
+ * persistent launch per row class (N <= 128, 256, 512, 1024, 2048): workgroups take
  * work units (a set and a range of its system pairs) from a queue ordered
  * largest N first.  pfsolve_batch_run_plain does the same work with one
  * plain plan launch pair per set, for comparison.  No counterpart in the
  * reference, which launches one plan per size. */
 typedef struct pfsolve_batch_set {
-    int64_t N;                /* rows, 2 <= N <= 1024 */
+    int64_t N;                /* rows, 2 <= N <= 2048 */
     int64_t nsys;             /* real systems sharing the set's matrices (>= 0) */
     const void* matrices;     /* device pointer, FP64 */
     int64_t offsetV, offsetT, offsetM;
@@ -88,7 +88,7 @@ typedef struct pfsolve_batch_set {
     double scaleC;            /* multiplies solution row 0 (1.0: none) */
 } pfsolve_batch_set;
 
-#define PFSOLVE_BATCH_CLASSES 4   /* row classes R = 2, 4, 8, 16 rows per lane */
+#define PFSOLVE_BATCH_CLASSES 5   /* row classes R = 2, 4, 8, 16, 32 rows per lane */
 
 /* What the autotuner chooses, per row class c (R = 2 << c). 0 = default. */
 typedef struct pfsolve_batch_tuning {
@@ -98,6 +98,7 @@ typedef struct pfsolve_batch_tuning {
     int32_t rows_per_lane_shift[PFSOLVE_BATCH_CLASSES]; /* 1: sets of this class run with 2R rows
                                                            per lane (half the lanes, one scan level
                                                            fewer), when 2R <= 16 */
+    /* unit_pairs counts systems, not pairs, for the R = 32 class (one system per pass) */
     int32_t order;            /* 0: largest N first, 1: sets as given */
     int32_t pad;
 } pfsolve_batch_tuning;

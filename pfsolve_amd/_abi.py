@@ -229,7 +229,7 @@ class PfSolveGeometry(C.Structure):
     ]
 
 
-BATCH_CLASSES = 4
+BATCH_CLASSES = 5
 
 
 class PfSolveBatchSet(C.Structure):

@@ -149,10 +149,19 @@ static int set_class(int64_t N)
     return c;   /* N <= 64 * (2 << c) */
 }
 
+/* the doubled rows-per-lane variant exists for R = 2, 4, 8 (to 4, 8, 16) */
+#define NSHIFT 3
+
 static int eff_group(const pfsolve_batch* b, int64_t s)
 {
-    int g = b->cls[s] + (b->tuning.rows_per_lane_shift[b->cls[s]] ? 1 : 0);
-    return g < NCLS ? g : NCLS - 1;
+    const int c = b->cls[s];
+    return c + (c < NSHIFT && b->tuning.rows_per_lane_shift[c] ? 1 : 0);
+}
+
+/* wavefront passes of a set: pairs of systems, or systems for R = 32 */
+static int64_t set_passes(const pfsolve_batch* b, int64_t s)
+{
+    return eff_group(b, s) == NCLS - 1 ? b->sets[s].nsys : (b->sets[s].nsys + 1) / 2;
 }
 
 static void source_of(const pfsolve_batch_set* st, jw_source_desc* src)
@@ -225,7 +234,7 @@ static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
     for (int64_t k = 0; k < S; k++) {
         const int64_t s = order[k];
         const int g = eff_group(b, s);
-        const int64_t pairs = (b->sets[s].nsys + 1) / 2;
+        const int64_t pairs = set_passes(b, s);
         int64_t up = b->tuning.unit_pairs[b->cls[s]];
         if (up <= 0) up = pairs > 0 ? pairs : 1;
         cnt[g] += (pairs + up - 1) / up;
@@ -244,7 +253,7 @@ static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
     for (int64_t k = 0; k < S; k++) {
         const int64_t s = order[k];
         const int g = eff_group(b, s);
-        const int64_t pairs = (b->sets[s].nsys + 1) / 2;
+        const int64_t pairs = set_passes(b, s);
         int64_t up = b->tuning.unit_pairs[b->cls[s]];
         if (up <= 0) up = pairs > 0 ? pairs : 1;
         for (int64_t p = 0; p < pairs; p += up) {
@@ -265,8 +274,8 @@ static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
         /* the tuning of the native class that feeds this group wins; default otherwise */
         int w = 0, gpc = 0;
         for (int c = 0; c < NCLS; c++) {
-            const int ge = c + (b->tuning.rows_per_lane_shift[c] ? 1 : 0);
-            if ((ge < NCLS ? ge : NCLS - 1) != g) continue;
+            const int ge = c + (c < NSHIFT && b->tuning.rows_per_lane_shift[c] ? 1 : 0);
+            if (ge != g) continue;
             if (b->tuning.waves[c] > w) w = b->tuning.waves[c];
             if (b->tuning.grid_per_cu[c] > gpc) gpc = b->tuning.grid_per_cu[c];
         }
@@ -346,7 +355,7 @@ void pfsolve_batch_destroy(pfsolve_batch* b)
 
 static PfSolveResult check_set(const pfsolve_batch_set* s)
 {
-    if (s->N < 2 || s->N > 1024) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+    if (s->N < 2 || s->N > 2048) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
     if (s->nsys < 0) return PFSOLVE_ERROR_EMPTY_size;
     if (s->KL < 0 || s->KL > 2 || s->KU < 0 || s->KU > 2) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
     if (!s->matrices) return PFSOLVE_ERROR_EMPTY_buffer;
@@ -385,10 +394,10 @@ PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set*
         b->cls[s] = set_class(sets[s].N);
         b->table_off[s] = off;
         int rmax = 2 << b->cls[s];
-        if (rmax < 16) rmax *= 2;   /* room for the doubled rows-per-lane variant */
+        if (b->cls[s] < NSHIFT) rmax *= 2;   /* room for the doubled rows-per-lane variant */
         off += ((int64_t)jwdev_table64_elems(rmax) * 8 + 255) / 256 * 256;
         b->total_systems += sets[s].nsys;
-        pairs_cls[b->cls[s]] += (sets[s].nsys + 1) / 2;
+        pairs_cls[b->cls[s]] += b->cls[s] == NCLS - 1 ? sets[s].nsys : (sets[s].nsys + 1) / 2;
     }
     b->table_bytes = off;
     if (hipMalloc(&b->tables, (size_t)off) != hipSuccess ||
@@ -413,8 +422,7 @@ PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set*
             b->tuning.order = e->order;
         }
     }
-    for (int c = 0; c < NCLS; c++)
-        if (b->tuning.rows_per_lane_shift[c] && c == NCLS - 1) b->tuning.rows_per_lane_shift[c] = 0;
+    for (int c = NSHIFT; c < NCLS; c++) b->tuning.rows_per_lane_shift[c] = 0;
     PfSolveResult r = upload_queue(b);
     if (r == PFSOLVE_SUCCESS) r = upload_items(b, b->d_items, NULL);
     if (r != PFSOLVE_SUCCESS) { pfsolve_batch_destroy(b); return r; }
@@ -431,7 +439,7 @@ PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tun
             return PFSOLVE_ERROR_EMPTY_size;
     if (t->order < 0 || t->order > 1) return PFSOLVE_ERROR_EMPTY_size;
     b->tuning = *t;
-    b->tuning.rows_per_lane_shift[NCLS - 1] = 0;
+    for (int c = NSHIFT; c < NCLS; c++) b->tuning.rows_per_lane_shift[c] = 0;
     return upload_queue(b);
 }
 
@@ -505,7 +513,7 @@ PfSolveResult pfsolve_batch_run_plain(pfsolve_batch* b, hipStream_t stream)
         const int waves = jwdev_main64_waves(R);
         const int occ = jwdev_main64_occupancy(R, waves);
         if (waves < 1 || occ < 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
-        const int64_t npass = (st->nsys + 1) / 2;
+        const int64_t npass = R == 32 ? st->nsys : (st->nsys + 1) / 2;
         int64_t grid = (npass + waves - 1) / waves;
         if (grid > (int64_t)b->cu_count * occ) grid = (int64_t)b->cu_count * occ;
         if (jwdev_main64(&run, R, waves, (int)grid, stream) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
@@ -631,13 +639,14 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
         const pfsolve_batch_tuning start = b->tuning;
         pfsolve_batch_tuning cur = start;
         int64_t pairs_cls[NCLS] = {0};
-        for (int64_t s = 0; s < b->nsets; s++) pairs_cls[b->cls[s]] += (b->sets[s].nsys + 1) / 2;
+        for (int64_t s = 0; s < b->nsets; s++)
+            pairs_cls[b->cls[s]] += b->cls[s] == NCLS - 1 ? b->sets[s].nsys : (b->sets[s].nsys + 1) / 2;
         static const int units_try[] = {0, 1, 2, 4, 8, 16, 32, 64};
         for (int c = 0; c < NCLS && res == PFSOLVE_SUCCESS; c++) {
             if (!pairs_cls[c]) continue;
             float best_ms = 1e30f;
             pfsolve_batch_tuning best = cur;
-            const int shifts = c < NCLS - 1 ? 2 : 1;
+            const int shifts = c < NSHIFT ? 2 : 1;
             for (int sh = 0; sh < shifts && res == PFSOLVE_SUCCESS; sh++) {
                 const int g = c + sh;
                 const int wmax = group_waves_default(g);

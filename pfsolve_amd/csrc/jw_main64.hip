@@ -468,6 +468,192 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc
     }
 }
 
+// ---------------------------------------------------------------------------
+// Long systems (R = 32 rows per lane, N <= 2048): one system per wavefront pass.
+// The 140 KiB table stays in global memory (every wavefront of the grid reads
+// it, so it is served from L2) and coefficients are streamed row by row instead
+// of held in registers; LDS holds the 16 KiB staging of each wavefront.
+// ---------------------------------------------------------------------------
+// table reads through a buffer resource: one VGPR lane offset, the row offset in
+// an SGPR, no 64-bit address per coefficient
+typedef double jw_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 tb2(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    const jw_d2v d = __builtin_bit_cast(jw_d2v, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    return make_double2(d.x, d.y);
+}
+__device__ __forceinline__ double tb1(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+template <int R, bool ALL3>
+__device__ __forceinline__ void compute_single(__amdgpu_buffer_rsrc_t tr, int lane, const jw_pass64& g,
+                                               double (&x)[R])
+{
+    constexpr int SC = 8 * R * 64 * 8;        // byte offset of the scan multipliers
+    const int lp = lane * 16;                 // a lane's coefficient pair within a row block
+    const int ls = lane * 8;                  // a lane's scan multiplier
+    // coefficient pair p of row-in-lane i: byte offset ((p R + i) 64 + lane) 16
+    {
+        double m1 = 0.0, m2 = 0.0, p1 = 0.0, p2 = 0.0;
+        if (g.kl >= 1) m1 = dpp_d<DPP_WAVE_SHR1>(x[R - 1]);
+        if (g.kl >= 2) m2 = dpp_d<DPP_WAVE_SHR1>(x[R - 2]);
+        if (g.ku >= 1) p1 = dpp_d<DPP_WAVE_SHL1>(x[0]);
+        if (g.ku >= 2) p2 = dpp_d<DPP_WAVE_SHL1>(x[1]);
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const double2 c0 = tb2(tr, lp, (0 * R + i) * 1024);
+            const double2 c1 = tb2(tr, lp, (1 * R + i) * 1024);
+            const double vp2 = tb1(tr, lp, (2 * R + i) * 1024);
+            const double xi = x[i];
+            const double n1 = (i + 1 < R) ? x[(i + 1) % R] : p1;
+            const double n2 = (i + 2 < R) ? x[(i + 2) % R] : ((i + 2 == R) ? p1 : p2);
+            double u = c1.x * xi;
+            u = __builtin_fma(c0.y, m1, u);
+            u = __builtin_fma(c0.x, m2, u);
+            u = __builtin_fma(c1.y, n1, u);
+            u = __builtin_fma(vp2, n2, u);
+            m2 = m1;
+            m1 = xi;
+            x[i] = u;
+        }
+    }
+    if (ALL3 || g.fwd) {
+        double a = x[0];
+#pragma unroll
+        for (int i = 1; i < R; i++) a = __builtin_fma(-tb1(tr, lp + 8, (2 * R + i) * 1024), a, x[i]);
+        a = __builtin_fma(tb1(tr, ls, SC + 0 * 512), dpp_d<DPP_ROW_SHR(1)>(a), a);
+        a = __builtin_fma(tb1(tr, ls, SC + 1 * 512), dpp_d<DPP_ROW_SHR(2)>(a), a);
+        a = __builtin_fma(tb1(tr, ls, SC + 2 * 512), dpp_d<DPP_ROW_SHR(4)>(a), a);
+        a = __builtin_fma(tb1(tr, ls, SC + 3 * 512), dpp_d<DPP_ROW_SHR(8)>(a), a);
+        a = __builtin_fma(tb1(tr, ls, SC + 4 * 512), dpp_rm<DPP_ROW_BCAST15, 0xa>(a), a);
+        a = __builtin_fma(tb1(tr, ls, SC + 5 * 512), dpp_rm<DPP_ROW_BCAST31, 0xc>(a), a);
+        double w = dpp_d<DPP_WAVE_SHR1>(a);
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            w = __builtin_fma(-tb1(tr, lp + 8, (2 * R + i) * 1024), w, x[i]);
+            x[i] = w;
+        }
+    }
+#pragma unroll
+    for (int stage = 0; stage < 2; stage++) {
+        if (!ALL3 && stage == 0 && !g.b1) continue;
+        if (!ALL3 && stage == 1 && !g.b2) continue;
+        const int j = 1 + stage;
+        double a = x[R - 1];
+#pragma unroll
+        for (int i = R - 2; i >= 0; i--) a = __builtin_fma(-tb1(tr, lp + 8 * stage, (3 * R + i) * 1024), a, x[i]);
+        a = __builtin_fma(tb1(tr, ls, SC + (j * 8 + 0) * 512), dpp_d<DPP_ROW_SHL(1)>(a), a);
+        a = __builtin_fma(tb1(tr, ls, SC + (j * 8 + 1) * 512), dpp_d<DPP_ROW_SHL(2)>(a), a);
+        a = __builtin_fma(tb1(tr, ls, SC + (j * 8 + 2) * 512), dpp_d<DPP_ROW_SHL(4)>(a), a);
+        a = __builtin_fma(tb1(tr, ls, SC + (j * 8 + 3) * 512), dpp_d<DPP_ROW_SHL(8)>(a), a);
+        {
+            const double s1 = readlane_d(a, 16), s2 = readlane_d(a, 32), s3 = readlane_d(a, 48);
+            a = __builtin_fma(tb1(tr, ls, SC + (j * 8 + 4) * 512), s1, a);
+            a = __builtin_fma(tb1(tr, ls, SC + (j * 8 + 5) * 512), s2, a);
+            a = __builtin_fma(tb1(tr, ls, SC + (j * 8 + 6) * 512), s3, a);
+        }
+        double z = dpp_d<DPP_WAVE_SHL1>(a);
+#pragma unroll
+        for (int i = R - 1; i >= 0; i--) {
+            z = __builtin_fma(-tb1(tr, lp + 8 * stage, (3 * R + i) * 1024), z, x[i]);
+            x[i] = z;
+        }
+    }
+    if (g.apply_scale) x[0] *= g.row0_scale;
+}
+
+// one system per pass: stg holds 64 R / 2 chunks (the first system region of a pair staging)
+template <int R, bool ALL3>
+__device__ __forceinline__ void jw_single64(__amdgpu_buffer_rsrc_t tab, double2* stg, int lane, const jw_pass64& g,
+                                            int64_t sys)
+{
+    constexpr int H = R / 2;
+    const double* src = g.rhs + sys * g.stride + g.offSol;
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        const int c = swz<H>(64 * k + lane);
+        if (c >= g.icl && c < g.ich)
+            __builtin_amdgcn_global_load_lds(src + 2 * c, (__attribute__((address_space(3))) void*)(stg + 64 * k),
+                                             16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the DMA writes have landed in LDS
+    compiler_fence();
+    double x[R];
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+        const double2 v = stg[swz<H>(lane * H + m)];
+        x[2 * m] = v.x;
+        x[2 * m + 1] = v.y;
+    }
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const bool on = e == 0 ? g.in_edge_lo : g.in_edge_hi;
+        const int r = e == 0 ? g.in_lo : g.in_hi - 1;
+        if (on && lane == r / R) {
+            const double v = src[r];
+#pragma unroll
+            for (int k = 0; k < R; k++)
+                if (k == r % R) x[k] = v;
+        }
+    }
+    compute_single<R, ALL3>(tab, lane, g, x);
+    compiler_fence();
+#pragma unroll
+    for (int m = 0; m < H; m++) stg[swz<H>(lane * H + m)] = make_double2(x[2 * m], x[2 * m + 1]);
+    compiler_fence();
+    double2 v[H];
+#pragma unroll
+    for (int k = 0; k < H; k++) v[k] = stg[swz<H>(64 * k + lane)];
+#pragma unroll
+    for (int k = 0; k < H; k++) asm volatile("" : "+v"(v[k].x), "+v"(v[k].y));
+    double2* dst2 = reinterpret_cast<double2*>(g.rhs) + ((sys * g.stride + g.offSol) >> 1);
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        const int c = 64 * k + lane;
+        if (c >= g.ocl && c < g.och) dst2[c] = v[k];
+    }
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const bool on = e == 0 ? g.out_edge_lo : g.out_edge_hi;
+        const int r = e == 0 ? g.out_lo : g.out_hi - 1;
+        if (on && lane == 0) {
+            const double2 w = stg[swz<H>(r >> 1)];
+            g.rhs[sys * g.stride + g.offSol + r] = (r & 1) ? w.y : w.x;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        const int c = swz<H>(64 * k + lane);
+        if (!(c >= g.icl && c < g.ich)) stg[64 * k + lane] = make_double2(0.0, 0.0);
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): LDS traffic done before the next DMA lands
+    compiler_fence();
+}
+
+template <int R, bool ALL3>
+__global__ __launch_bounds__(256) void jw_main64g_kernel(jw_run_desc d)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    // word 3 of the buffer resource: the gfx9 raw-buffer default (data format 32)
+    const __amdgpu_buffer_rsrc_t tab = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(d.table), 0,
+                                                                         JW_TAB64_ELEMS(R) * 8, 0x00020000);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* stg = reinterpret_cast<double2*>(smem) + (size_t)wave * 32 * R;
+#pragma unroll
+    for (int k = 0; k < R / 2; k++) stg[64 * k + lane] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int64_t q = (int64_t)blockIdx.x * nwaves + wave; q < d.nsys; q += (int64_t)gridDim.x * nwaves) {
+        const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
+                                        (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
+                                        d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+        jw_single64<R, ALL3>(tab, stg, lane, g, q);
+    }
+}
+
 // input of a pair fetched into registers, in the layout of the DMA (slot 64k + lane
 // of system s holds chunk swz(64k + lane)); chunks outside the keep-range are zero
 template <int R>
@@ -697,6 +883,43 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_batch64_kernel(const jw_b
     }
 }
 
+// The same queue for long systems (R = 32): one system per wavefront pass, table
+// read through a buffer resource; the unit ranges count systems, not pairs.
+__global__ __launch_bounds__(256) void jw_batch64g_kernel(const jw_batch_item* items, const jw_batch_unit* units,
+                                                          int nunits, int* counter)
+{
+    constexpr int R = 32;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_unit;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* stg = reinterpret_cast<double2*>(smem) + (size_t)wave * 32 * R;
+    int cur = -1;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (u >= nunits) break;
+        const jw_batch_unit un = units[u];
+        const jw_batch_item& it = items[un.item];
+        if (un.item != cur) {
+#pragma unroll
+            for (int k = 0; k < R / 2; k++) stg[64 * k + lane] = make_double2(0.0, 0.0);
+            cur = un.item;
+        }
+        const __amdgpu_buffer_rsrc_t tab = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<void*>(it.table), 0, JW_TAB64_ELEMS(R) * 8, 0x00020000);
+        for (int q = un.pair0 + wave; q < un.pair1; q += nwaves) {
+            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(it.rhs), it.nsys, it.stride, it.offSol, 0,
+                                            it.N, 0, it.N, it.kl, it.ku, true, true, true, it.apply_scale != 0,
+                                            it.scale, lane);
+            jw_single64<R, true>(tab, stg, lane, g, q);
+        }
+    }
+}
+
 template <int R>
 const void* kernel_ptr(bool all3)
 {
@@ -707,6 +930,8 @@ const void* kernel_ptr(bool all3)
 const void* pick64(int R, bool all3 = true)
 {
     switch (R) {
+    case 32: return all3 ? reinterpret_cast<const void*>(&jw_main64g_kernel<32, true>)
+                         : reinterpret_cast<const void*>(&jw_main64g_kernel<32, false>);
     case 2: return kernel_ptr<2>(all3);
     case 4: return kernel_ptr<4>(all3);
     case 8: return kernel_ptr<8>(all3);
@@ -730,6 +955,7 @@ const void* batch_ptr()
 const void* pick_batch64(int R)
 {
     switch (R) {
+    case 32: return reinterpret_cast<const void*>(&jw_batch64g_kernel);
     case 2: return batch_ptr<2>();
     case 4: return batch_ptr<4>();
     case 8: return batch_ptr<8>();
@@ -775,7 +1001,7 @@ bool split_shape(int R, int* ncomp, int* nstore)
     return true;
 }
 
-int max_waves64(int R) { return R <= 4 ? 16 : (R == 8 ? 12 : 5); }
+int max_waves64(int R) { return R <= 4 ? 16 : (R == 8 ? 12 : (R == 16 ? 5 : 4)); }
 
 thread_local char g_err64[256];
 
@@ -787,10 +1013,14 @@ int64_t jwdev_table64_elems(int R) { return JW_TAB64_ELEMS(R); }
 
 /* waves per workgroup: as many as LDS (table + 2 systems of staging per wave)
  * and the kernel's launch bound allow */
+/* LDS copy of the table: R <= 16 (R = 32 reads it from global memory) */
+static int64_t table_lds_bytes(int R) { return R <= 16 ? jwdev_table64_elems(R) * 8 : 0; }
+
 int jwdev_main64_waves(int R)
 {
     if (!pick64(R)) return 0;
-    const int64_t table = jwdev_table64_elems(R) * 8;
+    if (R == 32) return 4;
+    const int64_t table = table_lds_bytes(R);
     const int64_t per_wave = 2LL * 64 * R * 8;
     int w = (int)((kLdsBytesMax - table) / per_wave);
     if (w > max_waves64(R)) w = max_waves64(R);
@@ -799,7 +1029,8 @@ int jwdev_main64_waves(int R)
 
 int64_t jwdev_main64_lds_bytes(int R, int waves)
 {
-    return jwdev_table64_elems(R) * 8 + 2LL * 64 * R * 8 * waves;
+    if (R == 32) return 64LL * R * 8 * waves;   /* one system per wavefront */
+    return table_lds_bytes(R) + 2LL * 64 * R * 8 * waves;
 }
 
 /* workgroups per CU, the smaller of the two variants (all recurrences / some) */

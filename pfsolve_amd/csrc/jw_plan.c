@@ -7,7 +7,7 @@
 
 #include <string.h>
 
-#define JW_MAX_N_F64 1024
+#define JW_MAX_N_F64 2048   /* N > 1024: wave-per-system long kernel only */
 #define JW_MAX_N_DD 512
 
 int jw_plan_kind(const PfSolveConfiguration* cfg)
@@ -97,6 +97,7 @@ PfSolveResult jw_select_geometry(int64_t N, int precision, int* L, int* R)
     if (N > maxN) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
     *L = 64;
     *R = (int)((N + 63) / 64);
+    if (*R > 16) *R = 32;   /* the general kernel stops at 16 rows per lane */
     return PFSOLVE_SUCCESS;
 }
 

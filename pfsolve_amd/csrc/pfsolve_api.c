@@ -30,6 +30,14 @@ static PfSolveResult fill_geometry(const PfSolveConfiguration* cfg, pfsolve_geom
     int L = 0, R = 0;
     PfSolveResult r = jw_select_geometry(cfg->M_size, g->precision, &L, &R);
     if (r != PFSOLVE_SUCCESS) return r;
+    if (R > 16) {
+        /* long systems: only the wave-per-system kernel (no general-kernel geometry) */
+        g->lanes = 0;
+        g->rows_per_lane = 0;
+        g->table_bytes = jwdev_table64_elems(R) * 8;
+        g->lds_bytes = 0;
+        return PFSOLVE_SUCCESS;
+    }
     g->lanes = L;
     g->rows_per_lane = R;
     g->table_bytes = jwdev_table_elems(L, R) * (g->precision == JW_PREC_F64 ? 8 : 16);
@@ -48,7 +56,10 @@ PfSolveResult pfsolve_validate_configuration(const PfSolveConfiguration* cfg, pf
     if (r != PFSOLVE_SUCCESS) return r;
     if (jw_plan_kind(&c) == PF_KIND_JW) {
         const pfsolve_geometry* gg = g ? g : &tmp;
-        if (!jwdev_supported(gg->lanes, gg->rows_per_lane, gg->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD))
+        const int long_n = gg->lanes == 0;
+        if (long_n && c.kernelVariant == 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        if (!long_n &&
+            !jwdev_supported(gg->lanes, gg->rows_per_lane, gg->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD))
             return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
     }
     return PFSOLVE_SUCCESS;
@@ -105,12 +116,14 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
         p->lds_bytes = g.lds_bytes;
         p->table_bytes = (size_t)g.table_bytes;
         const int kp = p->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD;
-        if (!jwdev_supported(p->L, p->R, kp)) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
-        /* wave-per-system path: FP64, 64 lanes, R a power of two in [2, 16] */
+        const int long_n = p->L == 0;   /* N > 1024: the wave-per-system long kernel only */
+        if (long_n && cfg->kernelVariant == 1) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
+        if (!long_n && !jwdev_supported(p->L, p->R, kp)) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
+        /* wave-per-system path: FP64, 64 lanes, R a power of two in [2, 32] */
         if (p->precision == JW_PREC_F64 && cfg->kernelVariant != 1) {
             int r64 = 2;
             while (r64 * 64 < p->N) r64 *= 2;
-            if (r64 <= 16) {
+            if (r64 <= 32) {
                 p->R64 = r64;
                 p->waves64 = jwdev_main64_waves(r64);
                 p->occupancy64 = p->waves64 ? jwdev_main64_occupancy(r64, p->waves64) : 0;
@@ -121,7 +134,7 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
                 if (b2 > p->table_bytes) p->table_bytes = b2;
             }
         }
-        if (cfg->kernelVariant == 2 && !p->R64) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
+        if ((cfg->kernelVariant == 2 || long_n) && !p->R64) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
         /* compute / store split form of the wave-per-system kernel (R <= 8) */
         if (p->R64 && p->R64 <= 8) {
             int c = 0, st = 0;
@@ -133,8 +146,10 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
         if (hipMalloc((void**)&p->d_err, 256) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
         if (hipMemset(p->d_err, 0, 256) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_COPY; }
         if (hipMalloc(&p->table, p->table_bytes) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
-        p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
-        if (p->occupancy < 1) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY; }
+        if (!long_n) {
+            p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
+            if (p->occupancy < 1) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY; }
+        }
     } else if (p->kind == PF_KIND_BLOCK) {
         snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_block_%llu_%llu_%d_%d_%lld",
                  (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->block, cfg->lshift,
@@ -195,7 +210,8 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
             if (jwdev_prologue(&src, cfg->buffer[0], p->table, 64, p->R64, p->precision, 2, s) != 0)
                 return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             if (nsys <= 0) return PFSOLVE_SUCCESS;
-            const int64_t npass = (nsys + 1) / 2;
+            /* wavefront passes: pairs of systems, single systems for R = 32 */
+            const int64_t npass = p->R64 == 32 ? nsys : (nsys + 1) / 2;
             const int per_wg = p->split64 ? p->split64 : p->waves64;
             int64_t need = (npass + per_wg - 1) / per_wg;
             int64_t grid = (int64_t)p->cu_count * (p->split64 ? p->occupancy_split : p->occupancy64);
@@ -210,6 +226,9 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
             if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
             return PFSOLVE_SUCCESS;
         }
+        /* long systems run only on the wave-per-system kernel, which needs 16-byte
+         * aligned systems (even stride and offset, aligned buffer) */
+        if (p->L == 0) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         if (jwdev_prologue(&src, cfg->buffer[0], p->table, p->L, p->R, p->precision, 1, s) != 0)
             return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
         if (nsys <= 0) return PFSOLVE_SUCCESS;

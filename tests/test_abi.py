@@ -130,6 +130,16 @@ def test_geometry_covers_every_row(N):
     assert g.lds_bytes <= 160 * 1024
 
 
+@pytest.mark.parametrize("N", [1025, 1500, 2048])
+def test_long_systems_validate_for_the_wave_per_system_kernel_only(N):
+    """1024 < N <= 2048 (FP64): accepted with no general-kernel geometry; the general
+    kernel alone (kernelVariant 1) and N > 2048 are rejected"""
+    rc, g = validate(cfg(N=N, jw_type=14))
+    assert rc == 0 and g.lanes == 0 and g.table_bytes > 0
+    assert validate(cfg(N=N, jw_type=14, kernelVariant=1))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(N=2049, jw_type=14))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+
+
 def test_dd_geometry_and_limits():
     rc, g = validate(cfg(N=256, jw_type=14, doublePrecision=0, quadDoubleDoublePrecision=1))
     assert rc == 0 and g.precision == 2 and g.lanes * g.rows_per_lane >= 256

@@ -20,7 +20,7 @@ TOL64 = 1e-13
 SMALL_SETS = [(2, 3, 1, 1.0), (3, 1, 0, 0.5), (17, 4, 1, 1.0), (64, 5, 1, 2.0), (65, 2, 0, 1.0),
               (100, 6, 1, 1.0), (128, 1, 1, 0.75), (129, 3, 1, 1.0), (256, 8, 0, 1.0), (257, 2, 1, 1.0),
               (300, 5, 1, -1.5), (512, 4, 1, 1.0), (513, 3, 0, 1.0), (700, 2, 1, 1.0), (1000, 6, 1, 0.25),
-              (1024, 7, 1, 1.0), (511, 0, 1, 1.0)]
+              (1024, 7, 1, 1.0), (511, 0, 1, 1.0), (1025, 3, 1, 1.0), (1600, 4, 0, 0.5), (2048, 5, 1, 1.0)]
 
 
 def torch():
@@ -31,7 +31,7 @@ def torch():
 def row_class(N):
     """row class c of a set: R = 2 << c rows per lane, N <= 64 R"""
     c = 0
-    while c < 3 and (128 << c) < N:
+    while c < 4 and (128 << c) < N:
         c += 1
     return c
 
@@ -153,8 +153,8 @@ def tuning(**kw):
     dict(unit_pairs=2, waves=1),
     dict(waves=2, grid_per_cu=1),
     dict(order=1, unit_pairs=3),
-    dict(rows_per_lane_shift=[1, 1, 1, 0]),
-    dict(rows_per_lane_shift=[1, 0, 1, 0], unit_pairs=1, waves=1),
+    dict(rows_per_lane_shift=[1, 1, 1, 0, 0]),
+    dict(rows_per_lane_shift=[1, 0, 1, 0, 0], unit_pairs=1, waves=1),
 ])
 def test_batch_tunings(kw):
     """every tuning gives the oracle's results; without a rows-per-lane change the

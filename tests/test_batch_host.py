@@ -26,7 +26,7 @@ def make_sets(**over):
 
 @pytest.mark.parametrize("over,code", [
     (dict(N=1), "PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH"),
-    (dict(N=1025), "PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH"),
+    (dict(N=2049), "PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH"),
     (dict(nsys=-1), "PFSOLVE_ERROR_EMPTY_size"),
     (dict(KL=3), "PFSOLVE_ERROR_UNSUPPORTED_RADIX"),
     (dict(KU=-1), "PFSOLVE_ERROR_UNSUPPORTED_RADIX"),

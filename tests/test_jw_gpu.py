@@ -385,8 +385,42 @@ def test_split_kernel_many_passes_per_wavefront():
             assert O.relerr_norm(got[b * N:(b + 1) * N], want[j * N:(j + 1) * N]) <= TOL64, (N, b)
 
 
+@pytest.mark.parametrize("N", [1025, 1300, 2047, 2048])
+@pytest.mark.parametrize("permuted", [1, 0])
+def test_fused_step_long_systems(N, permuted):
+    """1024 < N <= 2048: one system per wavefront, table read from global memory;
+    odd batch, runtime keep-ranges with odd ends, row-0 scaleC"""
+    B = 5
+    S = N + 4 + (N % 2)
+    raw, buf, (offV, offT, offM) = fused_buffers(N, N + 7, permuted)
+    x = O.uniform(50 + N, -1, 1, B * S)
+    jt = 14 if permuted else 1014
+    mask = (_abi.RUNTIME_OUTPUTBUFFERSTRIDE | _abi.RUNTIME_SCALEC | _abi.RUNTIME_INPUTZEROPAD
+            | _abi.RUNTIME_OUTPUTZEROPAD)
+    cfg = dict(size=[N, B, 1], jw_type=jt, doublePrecision=1, scaleC=1.0, KL=2, KU=2, jw_control_bitmask=mask,
+               offsetV=offV, offsetT=offT, offsetM=offM)
+    launch = dict(outputBufferStride=S, scaleC=0.5, inputZeropad=[3, N - 1], outputZeropad=[1, N - 2])
+    got = run_plan(cfg, buf, x, launch)
+    p = O.problem(jw_type=jt, N=N, B=B, KL=2, KU=2, mask=mask, cfg_outStride=N, run_outStride=S,
+                  cfg_scaleC=1.0, run_scaleC=0.5, run_izp=[3, N - 1], run_ozp=[1, N - 2],
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    want = O.jw_f128(p, buf, x)
+    check_systems(got, want, x, B, S, 0, N, 1, N - 2, TOL64)
+
+
+def test_long_systems_need_aligned_layout():
+    """the long kernel moves 16-byte chunks: an odd stride is refused, not run"""
+    N, B = 1500, 3
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 3, 1)
+    x = O.uniform(4, -1, 1, B * (N + 1))
+    cfg = dict(size=[N, B, 1], jw_type=14, doublePrecision=1, offsetV=offV, offsetT=offT, offsetM=offM,
+               jw_control_bitmask=_abi.RUNTIME_OUTPUTBUFFERSTRIDE)
+    code = run_plan(cfg, buf, x, dict(outputBufferStride=N + 1), expect=1)
+    assert code == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+
+
 @pytest.mark.parametrize("jw_type,ub", MODES)
-@pytest.mark.parametrize("N", [2, 64, 130, 512, 1024])
+@pytest.mark.parametrize("N", [2, 64, 130, 512, 1024, 1800])
 def test_reference_modes_wave_per_system(jw_type, ub, N):
     """aligned batches of the reference modes run on the wave-per-system kernel"""
     jw = jw_type % 10
