@@ -32,6 +32,8 @@ size_t pfsolve_sizeof_application(void);
 /* byte offset of a named field, -1 if unknown */
 int64_t pfsolve_offsetof_configuration(const char* field);
 int64_t pfsolve_offsetof_launch_params(const char* field);
+/* size of an extension struct of this header by type name, 0 if unknown */
+size_t pfsolve_sizeof_ext(const char* type_name);
 
 /* ---- plan introspection (no device needed) ---- */
 typedef struct pfsolve_geometry {
@@ -143,6 +145,58 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
 PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tuning* t);
 PfSolveResult pfsolve_batch_get_info(const pfsolve_batch* b, pfsolve_batch_info* info);
 void pfsolve_batch_destroy(pfsolve_batch* b);
+
+/* ---- serving loop ----
+ *
+ * pfsolve_serve runs a request loop over `reqs` (sorted by arrival time): each
+ * request is a bundle of `nsys` real systems of size N (host input rhs_in and
+ * output rhs_out, systems contiguous) whose fused-step matrices (the jw_type
+ * 14 / 1014 layout, host memory) are identified by matrix_key.  Requests are
+ * admitted in arrival order while device memory (one arena of hbm_budget
+ * bytes for bundles in flight and resident matrices) and max_inflight allow;
+ * matrices and their coefficient table stay resident per key for the whole
+ * run; every admitted request waiting for launch joins the next launch on a
+ * free stream slot (up to max_batch requests: copies in, one persistent
+ * queue kernel per row class, copies out), so new requests are launched while
+ * earlier launches are still on the device.  realtime = 1 admits a request no
+ * earlier than its arrival_s (seconds from the start, wall clock); realtime =
+ * 0 treats every request as arrived at the start.  Host buffers should be
+ * pinned for the copies to overlap.  A request that cannot fit the budget even
+ * alone is rejected (status PFSOLVE_ERROR_FAILED_TO_ALLOCATE), as is a key
+ * reused with another N.  No counterpart in the reference. */
+typedef struct pfsolve_serve_config {
+    int64_t hbm_budget;       /* bytes of device memory the loop may hold */
+    int32_t max_inflight;     /* requests admitted and not finished, at most */
+    int32_t streams;          /* stream slots (launches in flight), 1..16 */
+    int32_t max_batch;        /* requests per launch */
+    int32_t realtime;
+} pfsolve_serve_config;
+
+typedef struct pfsolve_serve_request {
+    int64_t N, nsys;
+    uint64_t matrix_key;
+    const double* matrices;   /* host */
+    int64_t offsetV, offsetT, offsetM;
+    int32_t naturalOrder, KL, KU, pad;
+    double scaleC;
+    const double* rhs_in;     /* host, nsys * N */
+    double* rhs_out;          /* host, nsys * N */
+    double arrival_s;
+    double admit_s, done_s;   /* filled in: seconds from the start of the run */
+    int32_t status, pad2;     /* filled in: PfSolveResult of the request */
+} pfsolve_serve_request;
+
+typedef struct pfsolve_serve_stats {
+    int64_t requests, rejected, systems, launches;
+    double wall_s, systems_per_s;
+    double p50_ms, p99_ms, max_ms;    /* completion - arrival (realtime) or - start */
+    double mean_batch;                /* requests per launch */
+    int64_t peak_bytes;               /* arena bytes in use at the peak */
+    int64_t resident_keys, resident_bytes;
+} pfsolve_serve_stats;
+
+PfSolveResult pfsolve_serve(const pfsolve_serve_config* cfg, pfsolve_serve_request* reqs, int64_t nreq,
+                            pfsolve_serve_stats* stats);
 
 /* Tuning table on disk (text, one line per row class and batch size bucket):
  * pfsolve_batch_tune results are added to the process-wide table, which

@@ -223,3 +223,20 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def serve(requests, hbm_budget: int, max_inflight: int = 256, streams: int = 4, max_batch: int = 64,
+          realtime: bool = False):
+    """Runs the serving loop (pfsolve_serve) over `requests`, dicts with the
+    pfsolve_serve_request fields (host pointers as ints), sorted by arrival_s.
+    Returns (stats, per-request (admit_s, done_s, status) list)."""
+    arr = (_abi.PfSolveServeRequest * max(1, len(requests)))()
+    for i, q in enumerate(requests):
+        d = {"KL": 2, "KU": 2, "scaleC": 1.0}
+        d.update(q)
+        for k, v in d.items():
+            setattr(arr[i], k, v)
+    cfg = _abi.PfSolveServeConfig(hbm_budget, max_inflight, streams, max_batch, int(realtime))
+    st = _abi.PfSolveServeStats()
+    check(lib().pfsolve_serve(C.byref(cfg), arr, len(requests), C.byref(st)), "pfsolve_serve")
+    return st, [(arr[i].admit_s, arr[i].done_s, arr[i].status) for i in range(len(requests))]

@@ -277,6 +277,31 @@ class PfSolveBatchInfo(C.Structure):
     ]
 
 
+class PfSolveServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", i64), ("max_inflight", C.c_int32), ("streams", C.c_int32),
+                ("max_batch", C.c_int32), ("realtime", C.c_int32)]
+
+
+class PfSolveServeRequest(C.Structure):
+    _fields_ = [
+        ("N", i64), ("nsys", i64), ("matrix_key", u64), ("matrices", vp),
+        ("offsetV", i64), ("offsetT", i64), ("offsetM", i64),
+        ("naturalOrder", C.c_int32), ("KL", C.c_int32), ("KU", C.c_int32), ("pad", C.c_int32),
+        ("scaleC", C.c_double), ("rhs_in", vp), ("rhs_out", vp),
+        ("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+        ("status", C.c_int32), ("pad2", C.c_int32),
+    ]
+
+
+class PfSolveServeStats(C.Structure):
+    _fields_ = [
+        ("requests", i64), ("rejected", i64), ("systems", i64), ("launches", i64),
+        ("wall_s", C.c_double), ("systems_per_s", C.c_double),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double), ("mean_batch", C.c_double),
+        ("peak_bytes", i64), ("resident_keys", i64), ("resident_bytes", i64),
+    ]
+
+
 def declare(lib: C.CDLL) -> None:
     """Argument and result types of every exported entry point."""
     P = C.POINTER
@@ -337,3 +362,7 @@ def declare_ext(lib: C.CDLL) -> None:
     lib.pfsolve_tuning_load.restype = C.c_int
     lib.pfsolve_tuning_clear.argtypes = []
     lib.pfsolve_tuning_clear.restype = None
+    lib.pfsolve_sizeof_ext.argtypes = [C.c_char_p]
+    lib.pfsolve_sizeof_ext.restype = C.c_size_t
+    lib.pfsolve_serve.argtypes = [P(PfSolveServeConfig), P(PfSolveServeRequest), i64, P(PfSolveServeStats)]
+    lib.pfsolve_serve.restype = C.c_int

@@ -316,6 +316,23 @@ PfSolveResult PfSolveSync(PfSolveApplication* app)
 /* The reference returns the literal 00102, an octal constant (pfSolve.h:101-103). */
 int PfSolveGetVersion(void) { return 00102; }
 
+size_t pfsolve_sizeof_ext(const char* type_name)
+{
+    static const struct { const char* n; size_t s; } k[] = {
+        {"pfsolve_geometry", sizeof(pfsolve_geometry)},
+        {"pfsolve_batch_set", sizeof(pfsolve_batch_set)},
+        {"pfsolve_batch_tuning", sizeof(pfsolve_batch_tuning)},
+        {"pfsolve_batch_info", sizeof(pfsolve_batch_info)},
+        {"pfsolve_serve_config", sizeof(pfsolve_serve_config)},
+        {"pfsolve_serve_request", sizeof(pfsolve_serve_request)},
+        {"pfsolve_serve_stats", sizeof(pfsolve_serve_stats)},
+    };
+    if (!type_name) return 0;
+    for (size_t i = 0; i < sizeof(k) / sizeof(k[0]); i++)
+        if (strcmp(k[i].n, type_name) == 0) return k[i].s;
+    return 0;
+}
+
 typedef struct { int code; const char* name; } err_name;
 static const err_name k_err_names[] = {
     {0, "PFSOLVE_SUCCESS"},
