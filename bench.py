@@ -45,11 +45,9 @@ def gen_fused_matrices(N: int, seed: int, reorder) -> np.ndarray:
     reference's permuted storage; returns the buffer and offsets."""
     rng = np.random.default_rng(seed)
     V = rng.uniform(-1, 1, (5, N))
-    for j in range(5):
-        for col in range(N):
-            r = col - 2 + j
-            if not (0 <= r < N):
-                V[j, col] = 0.0
+    # band-major: V[j, col] holds row col - 2 + j; entries outside the matrix are zero
+    rows = np.arange(N)[None, :] - 2 + np.arange(5)[:, None]
+    V[(rows < 0) | (rows >= N)] = 0.0
     a, b, c = rng.uniform(-0.5, 0.5, N), rng.uniform(2, 3, N), rng.uniform(-0.5, 0.5, N)
     d, e = rng.uniform(1, 2, N), rng.uniform(-0.5, 0.5, N)
     a[0] = 0.0
@@ -273,6 +271,48 @@ def bench_dd(torch, pf, steps: int, warmup: int, N: int = 256, B: int = 16384) -
     }
 
 
+def bench_serve(torch, pf, seconds: float, rate: float = 2000.0, budget: int = 200 * 10**9, seed: int = 11) -> dict:
+    """Serving stream: Poisson arrivals at `rate` requests/s for `seconds`; each request a
+    bundle of log-uniform [64, 4096] real systems of N log-uniform in [32, 2048].  Matrices
+    are drawn per N (seed = N) and stay resident per N; inputs are windows of one pinned
+    random pool chosen by request id; outputs go to one pinned scratch buffer (the loop's
+    results are checked by the tests, the bench measures the loop)."""
+    lib = pf.lib()
+    rng = np.random.default_rng(seed)
+    n = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    Ns = np.exp(rng.uniform(np.log(32), np.log(2048), n)).astype(np.int64)
+    sizes = np.exp(rng.uniform(np.log(64), np.log(4096), n)).astype(np.int64)
+    mats = {}
+    for N in np.unique(Ns):
+        buf, offs = gen_fused_matrices(int(N), int(N), lib.pfsolve_reorder_i0)
+        mats[int(N)] = (np.ascontiguousarray(buf), offs)
+    max_elems = 4096 * 2048
+    pool = torch.empty(2 * max_elems, dtype=torch.float64).pin_memory()
+    pool.uniform_(-1.0, 1.0)
+    out = torch.empty(max_elems, dtype=torch.float64).pin_memory()
+    base_in, base_out = pool.data_ptr(), out.data_ptr()
+    reqs = []
+    for i in range(n):
+        N = int(Ns[i])
+        buf, (offV, offT, offM) = mats[N]
+        start = (i * 7919) % max_elems
+        reqs.append(dict(N=N, nsys=int(sizes[i]), matrix_key=N, matrices=buf.ctypes.data, offsetV=offV,
+                         offsetT=offT, offsetM=offM, naturalOrder=0, rhs_in=base_in + 8 * start, rhs_out=base_out,
+                         arrival_s=float(arrivals[i])))
+    st, res = pf.serve(reqs, hbm_budget=budget, max_inflight=512, streams=8, max_batch=64, realtime=True)
+    bad = sum(1 for _, _, c in res if c != 0)
+    return {
+        "workload": f"Poisson {rate:.0f} requests/s for {seconds:g} s ({n} requests), N log-U[32,2048], "
+                    "bundle log-U[64,4096] real systems, FP64 fused step, matrices resident per N, "
+                    f"HBM budget {budget / 1e9:.0f} GB",
+        "requests": int(st.requests), "rejected": int(st.rejected), "failed": bad,
+        "systems_per_s": st.systems_per_s, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms, "max_ms": st.max_ms,
+        "launches": int(st.launches), "mean_requests_per_launch": st.mean_batch,
+        "peak_device_bytes": int(st.peak_bytes), "resident_keys": int(st.resident_keys), "wall_s": st.wall_s,
+    }
+
+
 def stream_reference(torch, nbytes: int, reps: int = 20) -> dict:
     """What plain streaming kernels reach on this GPU for the same byte count: copy_
     (half read, half written) and fill_ (written only)."""
@@ -303,6 +343,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-size batch measurement")
     ap.add_argument("--no-dd", action="store_true", help="skip the double-double measurement")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the serving stream (60 = the full stream; 0 skips it)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time torch copy_ / fill_ over the same byte count (streaming reference)")
     args = ap.parse_args()
@@ -430,6 +472,8 @@ def main() -> int:
         result["batch"] = bench_batch(torch, pf, max(10, K), max(3, args.warmup))
     if not args.no_dd:
         result["dd"] = bench_dd(torch, pf, max(5, K // 2), 2)
+    if args.serve_seconds > 0:
+        result["serve"] = bench_serve(torch, pf, args.serve_seconds)
     if args.calibrate:
         result["stream_reference"] = stream_reference(torch, alg)
     if not args.no_cpu:
