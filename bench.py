@@ -271,6 +271,77 @@ def bench_dd(torch, pf, steps: int, warmup: int, N: int = 256, B: int = 16384) -
     }
 
 
+def bench_fused_blocks(torch, pf, steps: int, warmup: int, N: int = 512, ncx: int = 65536) -> dict:
+    """Block operators fused around the step: complex systems read through block
+    layout 4 with operator ScaleD (204), the step, written through layout 7 with
+    accumulate (027), one plan and one launch; against the same work as three plans
+    (block 204 -> step on a packed real buffer -> block 027)."""
+    lib = pf.lib()
+    buf, (offV, offT, offM) = gen_fused_matrices(N, 0, lib.pfsolve_reorder_i0)
+    dmat = torch.from_numpy(buf).cuda()
+    g = torch.Generator(device="cuda").manual_seed(4)
+    xin = torch.rand(2 * ncx * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    out = torch.zeros(2 * ncx * N, dtype=torch.float64, device="cuda")
+    tmp = torch.zeros(2 * ncx * N, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.Stream()
+    sptr = stream.cuda_stream
+    jw = dict(size=[N, 2 * ncx, 1], jw_type=14, doublePrecision=1, scaleC=1.0, offsetV=offV, offsetT=offT,
+              offsetM=offM)
+    fused = pf.Plan(dmat.data_ptr(), out.data_ptr(), stream=sptr, inputBlock=204, outputBlock=27,
+                    inputBlockLshift=1, inputBlockScale=0.5, **jw)
+    b_in = pf.Plan(xin.data_ptr(), tmp.data_ptr(), stream=sptr, size=[N, ncx, 1], doublePrecision=1, block=204,
+                   lshift=1, scaleC=0.5)
+    step = pf.Plan(dmat.data_ptr(), tmp.data_ptr(), stream=sptr, **jw)
+    b_out = pf.Plan(tmp.data_ptr(), out.data_ptr(), stream=sptr, size=[N, ncx, 1], doublePrecision=1, block=27)
+
+    def run_fused():
+        fused.append(inputBuffer=xin.data_ptr())
+
+    def run_chain():
+        b_in.append()
+        step.append()
+        b_out.append()
+
+    def timed(fn, kernel_events=False):
+        for _ in range(warmup):
+            fn()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        kst = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ken = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        for e in evs + kst + ken:
+            e.record(stream)
+        stream.synchronize()
+        with torch.cuda.stream(stream):
+            for i in range(steps):
+                evs[i].record(stream)
+                if kernel_events:
+                    lib.pfsolve_set_kernel_events(C.byref(fused.app), C.c_void_p(kst[i].cuda_event),
+                                                  C.c_void_p(ken[i].cuda_event))
+                fn()
+            evs[steps].record(stream)
+        stream.synchronize()
+        if kernel_events:
+            lib.pfsolve_set_kernel_events(C.byref(fused.app), None, None)
+        ms = evs[0].elapsed_time(evs[steps]) / steps
+        kms = float(np.mean([kst[i].elapsed_time(ken[i]) for i in range(steps)])) if kernel_events else None
+        return ms, kms
+
+    ms_f, kms_f = timed(run_fused, kernel_events=True)
+    ms_c, _ = timed(run_chain)
+    for p in (fused, b_in, step, b_out):
+        p.close()
+    assert torch.isfinite(out).all().item(), "non-finite fused-block output"
+    # input read, previous output read (accumulate), output written: 48 B per complex row
+    alg = ncx * N * 48 + 10 * N * 8
+    return {
+        "workload": f"block 204 -> fused JW step -> block 027 (accumulate), N={N}, {ncx} complex systems, FP64",
+        "value": ncx / (ms_f * 1e-3), "unit": "systems/s", "ms_per_step": ms_f,
+        "kernel": "jw_main64s_kernel<8, true, 1>", "kernel_ms": kms_f,
+        "algorithmic_bytes": alg, "hbm_frac": alg / (kms_f * 1e-3) / HBM_PEAK,
+        "chain_ms_per_step": ms_c, "speedup_vs_chain": ms_c / ms_f,
+    }
+
+
 def bench_serve(torch, pf, seconds: float, rate: float = 2000.0, budget: int = 200 * 10**9, seed: int = 11) -> dict:
     """Serving stream: Poisson arrivals at `rate` requests/s for `seconds`; each request a
     bundle of log-uniform [64, 4096] real systems of N log-uniform in [32, 2048].  Matrices
@@ -343,6 +414,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-size batch measurement")
     ap.add_argument("--no-dd", action="store_true", help="skip the double-double measurement")
+    ap.add_argument("--no-fused-blocks", action="store_true", help="skip the fused block I/O measurement")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the serving stream (60 = the full stream; 0 skips it)")
     ap.add_argument("--calibrate", action="store_true",
@@ -472,6 +544,8 @@ def main() -> int:
         result["batch"] = bench_batch(torch, pf, max(10, K), max(3, args.warmup))
     if not args.no_dd:
         result["dd"] = bench_dd(torch, pf, max(5, K // 2), 2)
+    if not args.no_fused_blocks:
+        result["fused_blocks"] = bench_fused_blocks(torch, pf, max(10, K // 2), 3)
     if args.serve_seconds > 0:
         result["serve"] = bench_serve(torch, pf, args.serve_seconds)
     if args.calibrate:

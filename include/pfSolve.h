@@ -215,6 +215,20 @@ typedef struct {
     int64_t offsetT;             /* fused step (jw_type 14): tridiagonal block T */
     int64_t autotune;            /* 0: built-in table, 1: run the autotuner at plan time */
     int64_t kernelVariant;       /* 0: automatic, 1: general kernel, 2: wave-per-system FP64 kernel */
+    /* Fused block operators around a JW step (block codes as in `block`: layout +
+     * 20 for accumulate + 100/200/300/400 for ScaleC/ScaleD/SphLaplA/SphLaplB).
+     * inputBlock (layout 4) reads packed complex systems from inputBuffer and
+     * feeds their re/im halves to the step; outputBlock (layout 7) writes the
+     * result as packed complex systems to outputBuffer.  Both or neither; 0 = none.
+     * Strides count complex elements (input/outputBufferStride); input system 0
+     * starts at inputBlockOffset, output system 0 at offsetSolution. */
+    int64_t inputBlock;
+    int64_t outputBlock;
+    int64_t inputBlockLshift;
+    int64_t outputBlockLshift;
+    double inputBlockScale;      /* the input operator's s */
+    double outputBlockScale;     /* the output operator's s */
+    int64_t inputBlockOffset;
 } PfSolveConfiguration;
 
 typedef struct {

@@ -117,7 +117,7 @@ class Plan:
     def append(self, **launch) -> None:
         lp = _abi.PfSolveLaunchParams()
         for k, v in launch.items():
-            if k in ("buffer", "outputBuffer"):
+            if k in ("buffer", "outputBuffer", "inputBuffer"):
                 arr = (C.c_void_p * 1)(v)
                 self._keep.append(arr)
                 setattr(lp, k, C.cast(arr, _abi.vpp))

@@ -175,6 +175,13 @@ class PfSolveConfiguration(C.Structure):
         ("offsetT", i64),
         ("autotune", i64),
         ("kernelVariant", i64),
+        ("inputBlock", i64),
+        ("outputBlock", i64),
+        ("inputBlockLshift", i64),
+        ("outputBlockLshift", i64),
+        ("inputBlockScale", C.c_double),
+        ("outputBlockScale", C.c_double),
+        ("inputBlockOffset", i64),
     ]
 
 

@@ -208,4 +208,19 @@ __host__ __device__ __forceinline__ int tab_coef64(int p, int i, int t, int R)
     return ((p * R + i) * 64 + t) * 2;
 }
 
+// block scale operators of row n (PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_RegistersScaleC.h:27-201,
+// runtime-scaleC branches): 1 ScaleC, 2 ScaleD, 3 SphLaplA, 4 SphLaplB; anything else 1
+__device__ __forceinline__ double jw_block_scale(int op, int64_t n, int lshift, double s)
+{
+    const double x = (double)n, l = (double)lshift;
+    switch (op) {
+    case 1: return s;
+    case 2: return 2.0 * (x + l + 1.0) * sqrt((x + 1.0) / (x + s + l + 1.5));
+    case 3: return 4.0 * (x + l + 2.0) * (x + l + 3.0) *
+                   sqrt((x + 1.0) * (x + 2.0) / ((x + s + l + 2.5) * (x + s + l + 3.5)));
+    case 4: return (x + l + 1.0) * (4.0 * l + 6.0) * sqrt((x + 1.0) / (x + s + l + 1.5));
+    default: return 1.0;
+    }
+}
+
 #endif

@@ -669,19 +669,6 @@ __global__ __launch_bounds__(512) void jw_main_kernel(jw_run_desc d)
 // ------------------------------------------------------------------------
 // block plan kernel (FP64): PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel2/pfSolve_block.h:37-115
 // ------------------------------------------------------------------------
-__device__ __forceinline__ double block_scale(int op, int64_t n, int lshift, double s)
-{
-    const double x = (double)n, l = (double)lshift;
-    switch (op) {
-    case 1: return s;
-    case 2: return 2.0 * (x + l + 1.0) * sqrt((x + 1.0) / (x + s + l + 1.5));
-    case 3: return 4.0 * (x + l + 2.0) * (x + l + 3.0) *
-                   sqrt((x + 1.0) * (x + 2.0) / ((x + s + l + 2.5) * (x + s + l + 3.5)));
-    case 4: return (x + l + 1.0) * (4.0 * l + 6.0) * sqrt((x + 1.0) / (x + s + l + 1.5));
-    default: return 1.0;
-    }
-}
-
 __global__ __launch_bounds__(256) void jw_block_kernel(jw_block_desc d)
 {
     const double* in = reinterpret_cast<const double*>(d.in);
@@ -711,7 +698,7 @@ __global__ __launch_bounds__(256) void jw_block_kernel(jw_block_desc d)
             }
         }
         if (!d.skip_read && d.op >= 1 && d.op <= 4) {
-            const double f = block_scale(d.op, xr, d.lshift, d.scale);
+            const double f = jw_block_scale(d.op, xr, d.lshift, d.scale);
             re *= f;
             im *= f;
         }

@@ -72,6 +72,14 @@ typedef struct jw_run_desc {
     double scale_hi, scale_lo;
     int32_t vec2;             /* FP64: 16-byte aligned pairs may use 128-bit access */
     int32_t mem_f64;          /* DD compute with FP64 storage */
+    /* fused block operators (pair = re/im of one packed complex system) */
+    int32_t fused_io;         /* 1: read in_c, write out_c; rhs unused */
+    int32_t in_op, out_op, out_acc;
+    int32_t in_lshift, out_lshift;
+    double in_scale, out_scale;
+    const void* in_c;         /* packed complex input */
+    void* out_c;              /* packed complex output */
+    int64_t in_stride_c, in_off_c, out_stride_c, out_off_c;   /* complex elements */
 } jw_run_desc;
 
 /* One matrix set for the batched prologue (one workgroup each). */

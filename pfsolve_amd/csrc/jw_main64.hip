@@ -686,6 +686,90 @@ __device__ __forceinline__ void stage_pair(double2* stg, int lane, const double2
         for (int k = 0; k < H; k++) stg[s * SYSC + 64 * k + lane] = pf[s][k];
 }
 
+// Fused block operators (compute / store split kernel, IOMODE 1): the pair of a
+// pass is the re/im of complex system q of a packed (XIXI) array.
+// Input (block layout 4): each row scaled by the input operator, rows >= N zero;
+// output (block layout 7): each row scaled by the output operator, then added to
+// the packed output when accumulating.  The arithmetic per element is the block
+// kernel's (jw_block_kernel), so the fused step equals the three-plan chain.
+template <int R>
+__device__ __forceinline__ void load_pair_packed(double2 (&pf)[2][R / 2], int lane, const jw_run_desc& d, int64_t q)
+{
+    constexpr int H = R / 2;
+    const double2* src = reinterpret_cast<const double2*>(d.in_c) + (q * d.in_stride_c + d.in_off_c);
+    const bool zero_in = d.in_op == 1 && d.in_scale == 0.0;   // baked ScaleC of 0: zeros, input not read
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        const int c = swz<H>(64 * k + lane);
+        const int r0 = 2 * c, r1 = 2 * c + 1;
+        double2 e0 = make_double2(0.0, 0.0), e1 = make_double2(0.0, 0.0);
+        if (!zero_in && r0 < d.N) e0 = src[r0];
+        if (!zero_in && r1 < d.N) e1 = src[r1];
+        pf[0][k] = make_double2(e0.x, e1.x);
+        pf[1][k] = make_double2(e0.y, e1.y);
+    }
+}
+
+// the input operator on a lane's rows, once they are in registers (rows >= N are zero)
+template <int R>
+__device__ __forceinline__ void scale_input(double (&x0)[R], double (&x1)[R], int lane, const jw_run_desc& d)
+{
+    if (d.in_op < 1 || d.in_op > 4) return;
+    // recomputed every pass: hoisting the R factors out of the pass loop would hold
+    // them in registers for the whole kernel
+    asm volatile("" : "+v"(lane));
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        // rows >= N stay exact zeros (a factor there may be NaN for some scales)
+        const int r = lane * R + i;
+        const double f = r < d.N ? jw_block_scale(d.in_op, r, d.in_lshift, d.in_scale) : 1.0;
+        x0[i] *= f;
+        x1[i] *= f;
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void store_pair_packed(const double2* stg, int lane, const jw_run_desc& d, int64_t q)
+{
+    constexpr int H = R / 2;
+    constexpr int SYSC = 64 * H;
+    double2* dst = reinterpret_cast<double2*>(d.out_c) + (q * d.out_stride_c + d.out_off_c);
+    const double* s0 = reinterpret_cast<const double*>(stg);
+    const double* s1 = reinterpret_cast<const double*>(stg + SYSC);
+    const bool zero_out = d.out_op == 1 && d.out_scale == 0.0;
+    asm volatile("" : "+v"(lane));   // factors recomputed per pair, not held across the loop
+    double re[R], im[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int r = 64 * k + lane;   // complex row r: chunk r / 2, half r % 2, of both systems
+        const int e = 2 * swz<H>(r >> 1) + (r & 1);
+        re[k] = s0[e];
+        im[k] = s1[e];
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) asm volatile("" : "+v"(re[k]), "+v"(im[k]));
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int r = 64 * k + lane;
+        if (r >= d.N) continue;
+        double a = re[k], b = im[k];
+        if (zero_out) {
+            a = 0.0;
+            b = 0.0;
+        } else if (d.out_op >= 1 && d.out_op <= 4) {
+            const double f = jw_block_scale(d.out_op, r, d.out_lshift, d.out_scale);
+            a *= f;
+            b *= f;
+        }
+        if (d.out_acc) {
+            const double2 prev = dst[r];
+            a += prev.x;
+            b += prev.y;
+        }
+        dst[r] = make_double2(a, b);
+    }
+}
+
 // Compute / store split.  The first `ncomp` wavefronts of a workgroup compute,
 // the others store.  A compute wavefront owns an input and an output staging;
 // per pass it moves the input chunks it fetched into registers one pass
@@ -728,7 +812,7 @@ constexpr int split_threads()
     return R <= 4 ? 1024 : (R == 8 ? 768 : 320);
 }
 
-template <int R, bool ALL3>
+template <int R, bool ALL3, int IOMODE>
 __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_desc d, int ncomp, int* err)
 {
     constexpr int NTAB = JW_TAB64_ELEMS(R);
@@ -766,21 +850,33 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
         bool ok = true;
         double2 pf[2][R / 2];
         if (q < npass) {
-            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
-                                            (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
-                                            d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
-            load_pair<R>(pf, lane, g, 2 * q);
+            if (IOMODE) {
+                load_pair_packed<R>(pf, lane, d, q);
+            } else {
+                const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
+                                                (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl,
+                                                d.ku, d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+                load_pair<R>(pf, lane, g, 2 * q);
+            }
         }
+        // fused block input: whole systems, zero-filled past N by the packed loads, so
+        // the keep-range is [0, N rounded up to even) and read_pair has no edge rows
+        const int in_lo = IOMODE ? 0 : (int)d.in_lo;
+        const int in_hi = IOMODE ? (int)(d.N + (d.N & 1)) : (int)d.in_hi;
         while (q < npass) {
-            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
-                                            (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
-                                            d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol, in_lo,
+                                            in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku, d.fwd, d.b1, d.b2,
+                                            d.apply_scale, d.scale_hi, lane);
             // the prefetched chunks (loaded a whole pass ago) through LDS into the blocked layout
             stage_pair<R>(sin, lane, pf);
             double x0[R], x1[R];
             read_pair<R>(sin, lane, g, 2 * q, x0, x1);
+            if (IOMODE) scale_input<R>(x0, x1, lane, d);
             const int64_t qn = q + step;
-            if (qn < npass) load_pair<R>(pf, lane, g, 2 * qn);
+            if (qn < npass) {
+                if (IOMODE) load_pair_packed<R>(pf, lane, d, qn);
+                else load_pair<R>(pf, lane, g, 2 * qn);
+            }
             compute_pair<R, ALL3>(tab, lane, g, x0, x1);
             if (!wait_flag(vfull + wave, 0)) { ok = false; break; }
             write_blocked<R>(sout, lane, x0, x1);
@@ -814,7 +910,8 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
                 const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
                                                 (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl,
                                                 d.ku, d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
-                store_pair<R>(out_base + (size_t)w * 64 * R, lane, g, 2 * q);
+                if (IOMODE) store_pair_packed<R>(out_base + (size_t)w * 64 * R, lane, d, q);
+                else store_pair<R>(out_base + (size_t)w * 64 * R, lane, g, 2 * q);
                 __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the staging has been read
                 compiler_fence();
                 if (lane == 0) lds_set(vfull + w, 0);
@@ -965,18 +1062,21 @@ const void* pick_batch64(int R)
 }
 
 template <int R>
-const void* split_ptr(bool all3)
+const void* split_ptr(bool all3, bool io)
 {
-    return all3 ? reinterpret_cast<const void*>(&jw_main64s_kernel<R, true>)
-                : reinterpret_cast<const void*>(&jw_main64s_kernel<R, false>);
+    if (io)
+        return all3 ? reinterpret_cast<const void*>(&jw_main64s_kernel<R, true, 1>)
+                    : reinterpret_cast<const void*>(&jw_main64s_kernel<R, false, 1>);
+    return all3 ? reinterpret_cast<const void*>(&jw_main64s_kernel<R, true, 0>)
+                : reinterpret_cast<const void*>(&jw_main64s_kernel<R, false, 0>);
 }
 
-const void* pick_split(int R, bool all3)
+const void* pick_split(int R, bool all3, bool io = false)
 {
     switch (R) {
-    case 2: return split_ptr<2>(all3);
-    case 4: return split_ptr<4>(all3);
-    case 8: return split_ptr<8>(all3);
+    case 2: return split_ptr<2>(all3, io);
+    case 4: return split_ptr<4>(all3, io);
+    case 8: return split_ptr<8>(all3, io);
     default: return nullptr;
     }
 }
@@ -1114,8 +1214,8 @@ int jwdev_main64s_occupancy(int R)
     int c = 0, st = 0;
     if (!jwdev_main64s_shape(R, &c, &st)) return 0;
     int best = -1;
-    for (int v = 0; v < 2; v++) {
-        const void* f = pick_split(R, v == 0);
+    for (int v = 0; v < 4; v++) {
+        const void* f = pick_split(R, (v & 1) == 0, v >= 2);
         const int lds = (int)jwdev_main64s_lds_bytes(R, c);
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
         int nb = 0;
@@ -1129,7 +1229,7 @@ int jwdev_main64s(const jw_run_desc* r, int R, int grid, int* d_err, void* strea
 {
     int c = 0, st = 0;
     if (!jwdev_main64s_shape(R, &c, &st)) { snprintf(g_err64, sizeof(g_err64), "no split kernel for R=%d", R); return -1; }
-    const void* f = pick_split(R, r->fwd && r->b1 && r->b2);
+    const void* f = pick_split(R, r->fwd && r->b1 && r->b2, r->fused_io != 0);
     const size_t lds = (size_t)jwdev_main64s_lds_bytes(R, c);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -2;

@@ -9,6 +9,7 @@
 
 #define JW_MAX_N_F64 2048   /* N > 1024: wave-per-system long kernel only */
 #define JW_MAX_N_DD 512
+#define JW_MAX_N_FUSED_BLOCKS 512   /* fused block I/O runs on the compute / store split kernel */
 
 int jw_plan_kind(const PfSolveConfiguration* cfg)
 {
@@ -67,6 +68,17 @@ PfSolveResult jw_configure(const PfSolveConfiguration* in, PfSolveConfiguration*
         }
         if (prec == JW_PREC_F64 && N > JW_MAX_N_F64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         if (prec != JW_PREC_F64 && N > JW_MAX_N_DD) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        /* block operators fused around the step: layout 4 (complex in, no accumulate)
+         * and layout 7 (complex out, optional accumulate), both present */
+        if (out->inputBlock || out->outputBlock) {
+            const int64_t ib = out->inputBlock, ob = out->outputBlock;
+            if (ib < 0 || ob < 0 || ib % 10 != 4 || ob % 10 != 7 || (ib / 10) % 10 != 0 ||
+                ((ob / 10) % 10 != 0 && (ob / 10) % 10 != 2) || ib / 100 > 4 || ob / 100 > 4)
+                return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+            if (prec != JW_PREC_F64 || N > JW_MAX_N_FUSED_BLOCKS) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        }
+    } else if (out->inputBlock || out->outputBlock) {
+        return PFSOLVE_ERROR_UNSUPPORTED_RADIX;   /* only a JW step takes fused blocks */
     } else if (kind == PF_KIND_BLOCK) {
         const int layout = out->block % 10;
         const int op = out->block / 100;

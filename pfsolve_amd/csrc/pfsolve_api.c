@@ -162,6 +162,75 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
     return PFSOLVE_SUCCESS;
 }
 
+/* JW step with fused block operators: packed complex systems in (block layout 4,
+ * PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_ReadWrite.h:2275-2371) and out
+ * (layout 7, optional accumulate), each row scaled by the block's operator
+ * (pfSolve_RegistersScaleC.h:27-201); the re/im halves of complex system q are the
+ * step's pair of real systems 2q, 2q+1.  Runs on the compute / store split kernel. */
+static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveLaunchParams* lp,
+                                         const jw_source_desc* src, jw_run_desc* run, int64_t nsys, hipStream_t s)
+{
+    struct pfsolve_plan* p = app->localFFTPlan;
+    PfSolveConfiguration* cfg = &app->configuration;
+    /* block codes, precision and N were checked when the plan was configured */
+    const int64_t ib = cfg->inputBlock, ob = cfg->outputBlock;
+    if (p->precision != JW_PREC_F64 || !p->split64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+    /* whole systems only: the keep-ranges of the step must be full */
+    if (run->in_lo != 0 || run->in_hi != p->N || run->out_lo != 0 || run->out_hi != p->N)
+        return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+    if (nsys % 2) return PFSOLVE_ERROR_EMPTY_size;
+    if (!cfg->inputBuffer || !cfg->inputBuffer[0] || cfg->inputBuffer == cfg->buffer)
+        return PFSOLVE_ERROR_EMPTY_inputBuffer;
+    if (!lp) lp = &(const PfSolveLaunchParams){0};
+    const int64_t m = cfg->jw_control_bitmask;
+    const int64_t ncx = nsys / 2;
+    run->fused_io = 1;
+    run->in_c = cfg->inputBuffer[0];
+    run->out_c = cfg->outputBuffer[0];
+    run->rhs = cfg->outputBuffer[0];
+    run->in_stride_c = (m & RUNTIME_INPUTBUFFERSTRIDE) ? (int64_t)lp->inputBufferStride : (int64_t)cfg->inputBufferStride[0];
+    run->out_stride_c = run->stride;
+    run->in_off_c = cfg->inputBlockOffset;
+    run->out_off_c = run->offSol;
+    run->in_op = (int32_t)(ib / 100);
+    run->out_op = (int32_t)(ob / 100);
+    run->out_acc = (ob / 10) % 10 == 2;
+    run->in_lshift = (int32_t)cfg->inputBlockLshift;
+    run->out_lshift = (int32_t)cfg->outputBlockLshift;
+    run->in_scale = cfg->inputBlockScale;
+    run->out_scale = cfg->outputBlockScale;
+    run->vec2 = 1;
+    if (((uintptr_t)run->in_c % 16) || ((uintptr_t)run->out_c % 16)) return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+    if (run->in_stride_c < 0 || run->out_stride_c < 0 || run->in_off_c < 0 || run->out_off_c < 0 ||
+        (ncx > 1 && (run->in_stride_c < p->N || run->out_stride_c < p->N)))
+        return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+    if (ncx > 0) {
+        const int64_t imax = (ncx - 1) * run->in_stride_c + run->in_off_c + p->N;   /* complex elements */
+        const int64_t omax = (ncx - 1) * run->out_stride_c + run->out_off_c + p->N;
+        if (cfg->inputBufferSize && (uint64_t)(imax * 16) > cfg->inputBufferSize[0]) return PFSOLVE_ERROR_EMPTY_inputBufferSize;
+        if (cfg->outputBufferSize && (uint64_t)(omax * 16) > cfg->outputBufferSize[0])
+            return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+    }
+    {
+        int64_t mmax, rmax;
+        jw_index_extent(src, run, &mmax, &rmax);
+        if (cfg->bufferSize && mmax >= 0 && (uint64_t)((mmax + 1) * 8) > cfg->bufferSize[0])
+            return PFSOLVE_ERROR_EMPTY_bufferSize;
+    }
+    run->table = p->table;
+    if (jwdev_prologue(src, cfg->buffer[0], p->table, 64, p->R64, p->precision, 2, s) != 0)
+        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    if (ncx <= 0) return PFSOLVE_SUCCESS;
+    int64_t need = (ncx + p->split64 - 1) / p->split64;
+    int64_t grid = (int64_t)p->cu_count * p->occupancy_split;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+    if (jwdev_main64s(run, p->R64, (int)grid, p->d_err, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+    return PFSOLVE_SUCCESS;
+}
+
 PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchParams* lp)
 {
     (void)inverse;
@@ -189,6 +258,7 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
         jw_run_desc run;
         PfSolveResult r = jw_resolve_launch(cfg, lp, nsys, &src, &run);
         if (r != PFSOLVE_SUCCESS) return r;
+        if (cfg->inputBlock || cfg->outputBlock) return append_fused_blocks(app, lp, &src, &run, nsys, s);
         /* index guard: every element the kernels touch must lie inside the
          * buffers whose sizes the caller declared (bufferSize/outputBufferSize) */
         if (!jw_offsets_valid(&src, &run)) return PFSOLVE_ERROR_EMPTY_outputBufferSize;
@@ -465,7 +535,10 @@ static const field_off k_cfg_fields[] = {
     OFF(PfSolveConfiguration, keepShaderCode), OFF(PfSolveConfiguration, disableCaching),
     OFF(PfSolveConfiguration, numConsecutiveJWIterations), OFF(PfSolveConfiguration, useMultipleInputBuffers),
     OFF(PfSolveConfiguration, offsetT), OFF(PfSolveConfiguration, autotune),
-    OFF(PfSolveConfiguration, kernelVariant),
+    OFF(PfSolveConfiguration, kernelVariant), OFF(PfSolveConfiguration, inputBlock),
+    OFF(PfSolveConfiguration, outputBlock), OFF(PfSolveConfiguration, inputBlockLshift),
+    OFF(PfSolveConfiguration, outputBlockLshift), OFF(PfSolveConfiguration, inputBlockScale),
+    OFF(PfSolveConfiguration, outputBlockScale), OFF(PfSolveConfiguration, inputBlockOffset),
 };
 static const field_off k_lp_fields[] = {
     OFF(PfSolveLaunchParams, buffer), OFF(PfSolveLaunchParams, outputBuffer), OFF(PfSolveLaunchParams, kernel),

@@ -120,6 +120,25 @@ def test_validation_error_codes():
     assert validate(cfg(LDA=5, KL=2, KU=2))[0] == 0
 
 
+def test_fused_block_codes_validation():
+    """block operators fused around a JW step: layout 4 in (no accumulate), layout 7
+    out (accumulate 0 or 2), operators 0-4, FP64, N <= 512, only on a JW plan"""
+    ok = dict(jw_type=14, inputBlock=204, outputBlock=327)
+    assert validate(cfg(**ok))[0] == 0
+    for ib in (4, 104, 204, 304, 404):
+        for ob in (7, 27, 107, 227, 427):
+            assert validate(cfg(jw_type=14, inputBlock=ib, outputBlock=ob))[0] == 0, (ib, ob)
+    assert validate(cfg(N=512, **ok))[0] == 0
+    bad_codes = [(204, 0), (0, 327), (207, 327), (224, 327), (204, 317), (204, 324), (504, 7), (4, 507), (-4, 7)]
+    for ib, ob in bad_codes:
+        assert validate(cfg(jw_type=14, inputBlock=ib, outputBlock=ob))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX, \
+            (ib, ob)
+    assert validate(cfg(N=513, **ok))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(quadDoubleDoublePrecision=1, **ok))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(block=104, inputBlock=4, outputBlock=7))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(LDA=5, KL=2, KU=2, inputBlock=4, outputBlock=7))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+
+
 @pytest.mark.parametrize("N", [1, 8, 64, 65, 128, 512, 513, 1000, 1024])
 def test_geometry_covers_every_row(N):
     rc, g = validate(cfg(N=N, jw_type=14))
