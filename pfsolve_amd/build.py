@@ -22,7 +22,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 C_SOURCES = ["jw_plan.c", "pfsolve_api.c", "jw_batch.c", "jw_serve.c"]
-HIP_SOURCES = ["jw_device.hip", "jw_main64.hip", "jw_batch_device.hip"]
+HIP_SOURCES = ["jw_device.hip", "jw_main64.hip"]
 
 C_FLAGS = [
     "-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter",
@@ -59,15 +59,11 @@ def build(verbose: bool = True) -> str:
     objs = []
     for src in C_SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(BUILD, src + ".o")
         objs.append(obj)
         jobs.append(["gcc", *C_FLAGS, "-c", "-o", obj, path])
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(BUILD, src + ".o")
         objs.append(obj)
         jobs.append([HIPCC, *HIP_FLAGS, "-c", "-o", obj, path])

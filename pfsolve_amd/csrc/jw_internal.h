@@ -143,7 +143,9 @@ int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream)
 int jwdev_main64s_shape(int R, int* ncomp, int* nstore);
 int64_t jwdev_main64s_lds_bytes(int R, int ncomp);
 int jwdev_main64s_occupancy(int R);
-int jwdev_main64s(const jw_run_desc* r, int R, int grid, int* d_err, void* stream);
+/* builds the table of `src` in each workgroup's LDS (no prologue launch needed) */
+int jwdev_main64s(const jw_run_desc* r, const jw_source_desc* src, const void* matrices, int R, int grid, int* d_err,
+                  void* stream);
 /* persistent uneven-batch kernel over a unit queue; *d_counter must be 0 at launch */
 int jwdev_batch64_occupancy(int R, int waves);
 int jwdev_zero_counters(int* d_counters, int n, void* stream);

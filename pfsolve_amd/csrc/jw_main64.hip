@@ -26,6 +26,7 @@
 #include <stdio.h>
 
 #include "jw_common.h"
+#include "jw_prologue.h"
 
 namespace {
 
@@ -38,6 +39,17 @@ __device__ __forceinline__ int swz(int c)
 }
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
+// store of a result chunk: written once and not read back by the step, so it is
+// marked nontemporal (streamed to memory without allocating in the caches)
+__device__ __forceinline__ void store_stream(double2* p, double2 v)
+{
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    d2v t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
+}
 
 // keeps the scheduler from hoisting one phase's table loads into the previous
 // phase, which would raise register pressure past the launch bound
@@ -155,6 +167,9 @@ __device__ __forceinline__ void dma_pair(double2* stg, int lane, const jw_pass64
     }
 }
 
+template <int R>
+__device__ __forceinline__ void read_edges(int lane, const jw_pass64& g, int64_t sys0, double (&x0)[R], double (&x1)[R]);
+
 // blocked read-back of the landed input (lane t: rows t R .. t R + R-1), odd range
 // ends read from global memory by their owning lane
 template <int R>
@@ -163,8 +178,6 @@ __device__ __forceinline__ void read_pair(const double2* stg, int lane, const jw
 {
     constexpr int H = R / 2;
     constexpr int SYSC = 64 * H;
-    const bool ok1 = sys0 + 1 < g.nsys;
-    const double* src0 = g.rhs + sys0 * g.stride + g.offSol;
 #pragma unroll
     for (int m = 0; m < H; m++) {
         const double2 v0 = stg[swz<H>(lane * H + m)];
@@ -174,6 +187,16 @@ __device__ __forceinline__ void read_pair(const double2* stg, int lane, const jw
         x1[2 * m] = v1.x;
         x1[2 * m + 1] = v1.y;
     }
+    read_edges<R>(lane, g, sys0, x0, x1);
+}
+
+// the odd ends of the input keep-range (rows whose 16-byte chunk is only half
+// inside), read from global memory by their owning lane
+template <int R>
+__device__ __forceinline__ void read_edges(int lane, const jw_pass64& g, int64_t sys0, double (&x0)[R], double (&x1)[R])
+{
+    const bool ok1 = sys0 + 1 < g.nsys;
+    const double* src0 = g.rhs + sys0 * g.stride + g.offSol;
     if (g.in_edge_lo || g.in_edge_hi) {
 #pragma unroll
         for (int e = 0; e < 2; e++) {
@@ -384,7 +407,7 @@ __device__ __forceinline__ void store_pair(const double2* stg, int lane, const j
 #pragma unroll
         for (int k = 0; k < H; k++) {
             const int c = 64 * k + lane;
-            if (c >= g.ocl && c < g.och) dst2[c] = v[s][k];
+            if (c >= g.ocl && c < g.och) store_stream(dst2 + c, v[s][k]);
         }
     }
     if (g.out_edge_lo || g.out_edge_hi) {
@@ -612,7 +635,7 @@ __device__ __forceinline__ void jw_single64(__amdgpu_buffer_rsrc_t tab, double2*
 #pragma unroll
     for (int k = 0; k < H; k++) {
         const int c = 64 * k + lane;
-        if (c >= g.ocl && c < g.och) dst2[c] = v[k];
+        if (c >= g.ocl && c < g.och) store_stream(dst2 + c, v[k]);
     }
 #pragma unroll
     for (int e = 0; e < 2; e++) {
@@ -654,10 +677,14 @@ __global__ __launch_bounds__(256) void jw_main64g_kernel(jw_run_desc d)
     }
 }
 
-// input of a pair fetched into registers, in the layout of the DMA (slot 64k + lane
-// of system s holds chunk swz(64k + lane)); chunks outside the keep-range are zero
+// Direct blocked loads (compute / store split kernel): lane t fetches its own R
+// rows, 16-byte chunks t H .. t H + H-1, of both systems, straight into the
+// registers the recurrences use.  One instruction's 64 accesses are spread over
+// a 64 x 16H-byte span that its H sibling instructions complete, so every line
+// is fetched from memory once and the other accesses hit the cache; no LDS
+// transposition is needed.
 template <int R>
-__device__ __forceinline__ void load_pair(double2 (&pf)[2][R / 2], int lane, const jw_pass64& g, int64_t sys0)
+__device__ __forceinline__ void load_pair_direct(double2 (&pf)[2][R / 2], int lane, const jw_pass64& g, int64_t sys0)
 {
     constexpr int H = R / 2;
     const bool ok1 = sys0 + 1 < g.nsys;
@@ -666,24 +693,28 @@ __device__ __forceinline__ void load_pair(double2 (&pf)[2][R / 2], int lane, con
     for (int s = 0; s < 2; s++) {
         const double2* src = src0 + s * (g.stride >> 1);
 #pragma unroll
-        for (int k = 0; k < H; k++) {
-            const int c = swz<H>(64 * k + lane);
+        for (int m = 0; m < H; m++) {
+            const int c = lane * H + m;
             double2 v = make_double2(0.0, 0.0);
             if (c >= g.icl && c < g.ich && (s == 0 || ok1)) v = src[c];
-            pf[s][k] = v;
+            pf[s][m] = v;
         }
     }
 }
 
+// the directly loaded chunks as rows, the odd keep-range ends from global memory
 template <int R>
-__device__ __forceinline__ void stage_pair(double2* stg, int lane, const double2 (&pf)[2][R / 2])
+__device__ __forceinline__ void unpack_pair(const double2 (&pf)[2][R / 2], int lane, const jw_pass64& g, int64_t sys0,
+                                            double (&x0)[R], double (&x1)[R])
 {
-    constexpr int H = R / 2;
-    constexpr int SYSC = 64 * H;
 #pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-        for (int k = 0; k < H; k++) stg[s * SYSC + 64 * k + lane] = pf[s][k];
+    for (int m = 0; m < R / 2; m++) {
+        x0[2 * m] = pf[0][m].x;
+        x0[2 * m + 1] = pf[0][m].y;
+        x1[2 * m] = pf[1][m].x;
+        x1[2 * m + 1] = pf[1][m].y;
+    }
+    read_edges<R>(lane, g, sys0, x0, x1);
 }
 
 // Fused block operators (compute / store split kernel, IOMODE 1): the pair of a
@@ -699,14 +730,13 @@ __device__ __forceinline__ void load_pair_packed(double2 (&pf)[2][R / 2], int la
     const double2* src = reinterpret_cast<const double2*>(d.in_c) + (q * d.in_stride_c + d.in_off_c);
     const bool zero_in = d.in_op == 1 && d.in_scale == 0.0;   // baked ScaleC of 0: zeros, input not read
 #pragma unroll
-    for (int k = 0; k < H; k++) {
-        const int c = swz<H>(64 * k + lane);
-        const int r0 = 2 * c, r1 = 2 * c + 1;
+    for (int m = 0; m < H; m++) {
+        const int r0 = lane * R + 2 * m, r1 = r0 + 1;   // the lane's own rows, as in load_pair_direct
         double2 e0 = make_double2(0.0, 0.0), e1 = make_double2(0.0, 0.0);
         if (!zero_in && r0 < d.N) e0 = src[r0];
         if (!zero_in && r1 < d.N) e1 = src[r1];
-        pf[0][k] = make_double2(e0.x, e1.x);
-        pf[1][k] = make_double2(e0.y, e1.y);
+        pf[0][m] = make_double2(e0.x, e1.x);
+        pf[1][m] = make_double2(e0.y, e1.y);
     }
 }
 
@@ -766,19 +796,21 @@ __device__ __forceinline__ void store_pair_packed(const double2* stg, int lane, 
             a += prev.x;
             b += prev.y;
         }
-        dst[r] = make_double2(a, b);
+        store_stream(dst + r, make_double2(a, b));
     }
 }
 
 // Compute / store split.  The first `ncomp` wavefronts of a workgroup compute,
-// the others store.  A compute wavefront owns an input and an output staging;
-// per pass it moves the input chunks it fetched into registers one pass
-// earlier through its input staging into the blocked layout, issues the loads
-// of its next pair at once, computes, waits until a storer has emptied its
-// output staging, writes the results there and raises its `full` flag.  Storer wavefronts poll the flags of their
-// compute wavefronts, store full stagings to global memory and lower the flag.
-// Compute wavefronts issue no global stores, so their vmcnt waits never wait
-// for write acknowledgements; stores stay in flight while compute goes on.
+// the others store.  Every workgroup first builds the coefficient table in its
+// LDS.  A compute wavefront owns two output stagings, used in turn; per pass it
+// takes the rows of its pair that it loaded straight into registers one pass
+// earlier, issues the loads of its next pair at once, computes, waits until a
+// storer has emptied this pass's staging (it held the pair of two passes ago),
+// writes the results there and raises the staging's `full` flag.  Storer
+// wavefronts poll the flags of their compute wavefronts, stream full stagings
+// to global memory (nontemporal stores) and lower the flag.  Compute
+// wavefronts issue no global stores, so their vmcnt waits never wait for write
+// acknowledgements; stores stay in flight while compute goes on.
 // Every wait on a flag is bounded (kSpinLimit polls without progress); a
 // wavefront that runs out stops and raises *err, so the grid always drains.
 constexpr int kSpinLimit = 1 << 22;
@@ -813,52 +845,54 @@ constexpr int split_threads()
 }
 
 template <int R, bool ALL3, int IOMODE>
-__global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_desc d, int ncomp, int* err)
+__global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_desc d, int ncomp, int* err,
+                                                                        jw_source_desc src, const void* mat)
 {
     constexpr int NTAB = JW_TAB64_ELEMS(R);
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_full[16], s_done[16];
-    __shared__ long long s_pair[16];
+    // two output stagings per compute wavefront (slot 2 w + b), each with its flag
+    // and pair index
+    __shared__ int s_full[32], s_done[16];
+    __shared__ long long s_pair[32];
     const double* tab = reinterpret_cast<const double*>(smem);
-    {
-        const uint4* src = reinterpret_cast<const uint4*>(d.table);
-        uint4* dst = reinterpret_cast<uint4*>(smem);
-        constexpr int n16 = NTAB * 8 / 16;
-        for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
-    }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = blockDim.x >> 6;
-    double2* in_base = reinterpret_cast<double2*>(smem + NTAB * 8);
-    double2* out_base = in_base + (size_t)ncomp * 64 * R;
+    double2* out_base = reinterpret_cast<double2*>(smem + NTAB * 8);
+    const int64_t npass = (d.nsys + 1) >> 1;
+    const int64_t step = (int64_t)gridDim.x * ncomp;
+    int64_t q = (int64_t)blockIdx.x * ncomp + wave;
+    // a compute wavefront's first pair is loaded into registers before the table
+    // build, so its latency overlaps the build
+    double2 pf[2][R / 2];
+    if (wave < ncomp && q < npass) {
+        if (IOMODE) {
+            load_pair_packed<R>(pf, lane, d, q);
+        } else {
+            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
+                                            (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
+                                            d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+            load_pair_direct<R>(pf, lane, g, 2 * q);
+        }
+    }
+    // every workgroup builds the coefficient table in its own LDS from the
+    // caller's matrices (no separate prologue launch); the staging area, unused
+    // until the barrier below, is the build's scratch
+    prologue_body<double>(src, mat, reinterpret_cast<double*>(smem), 64, R, 2, smem + NTAB * 8);
     if (wave < ncomp) {
         if (lane == 0) {
-            lds_set(&s_full[wave], 0);
+            lds_set(&s_full[2 * wave], 0);
+            lds_set(&s_full[2 * wave + 1], 0);
             lds_set(&s_done[wave], 0);
         }
     }
     __syncthreads();
-    const int64_t npass = (d.nsys + 1) >> 1;
     int* vfull = s_full;
     int* vdone = s_done;
 
     if (wave < ncomp) {
-        double2* sin = in_base + (size_t)wave * 64 * R;
-        double2* sout = out_base + (size_t)wave * 64 * R;
-        const int64_t step = (int64_t)gridDim.x * ncomp;
-        int64_t q = (int64_t)blockIdx.x * ncomp + wave;
         bool ok = true;
-        double2 pf[2][R / 2];
-        if (q < npass) {
-            if (IOMODE) {
-                load_pair_packed<R>(pf, lane, d, q);
-            } else {
-                const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
-                                                (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl,
-                                                d.ku, d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
-                load_pair<R>(pf, lane, g, 2 * q);
-            }
-        }
+        int b = 0;   // output staging of this pass
         // fused block input: whole systems, zero-filled past N by the packed loads, so
         // the keep-range is [0, N rounded up to even) and read_pair has no edge rows
         const int in_lo = IOMODE ? 0 : (int)d.in_lo;
@@ -867,26 +901,30 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
             const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol, in_lo,
                                             in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku, d.fwd, d.b1, d.b2,
                                             d.apply_scale, d.scale_hi, lane);
-            // the prefetched chunks (loaded a whole pass ago) through LDS into the blocked layout
-            stage_pair<R>(sin, lane, pf);
+            // the rows prefetched a whole pass ago, then the next pair's loads at once
             double x0[R], x1[R];
-            read_pair<R>(sin, lane, g, 2 * q, x0, x1);
+            unpack_pair<R>(pf, lane, g, 2 * q, x0, x1);
             if (IOMODE) scale_input<R>(x0, x1, lane, d);
             const int64_t qn = q + step;
             if (qn < npass) {
                 if (IOMODE) load_pair_packed<R>(pf, lane, d, qn);
-                else load_pair<R>(pf, lane, g, 2 * qn);
+                else load_pair_direct<R>(pf, lane, g, 2 * qn);
             }
             compute_pair<R, ALL3>(tab, lane, g, x0, x1);
-            if (!wait_flag(vfull + wave, 0)) { ok = false; break; }
-            write_blocked<R>(sout, lane, x0, x1);
-            if (lane == 0) __hip_atomic_store(&s_pair[wave], (long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // this pass's output staging (two alternate, so a storer may still be
+            // emptying the other one)
+            const int slot = 2 * wave + b;
+            if (!wait_flag(vfull + slot, 0)) { ok = false; break; }
+            write_blocked<R>(out_base + (size_t)slot * 64 * R, lane, x0, x1);
+            if (lane == 0) __hip_atomic_store(&s_pair[slot], (long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): staging and pair index written
             compiler_fence();
-            if (lane == 0) lds_set(vfull + wave, 1);
+            if (lane == 0) lds_set(vfull + slot, 1);
+            b ^= 1;
             q = qn;
         }
-        if (ok) ok = wait_flag(vfull + wave, 0);
+        if (ok) ok = wait_flag(vfull + 2 * wave, 0);
+        if (ok) ok = wait_flag(vfull + 2 * wave + 1, 0);
         __builtin_amdgcn_s_waitcnt(0);
         compiler_fence();
         if (lane == 0) {
@@ -903,22 +941,29 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
     for (;;) {
         bool all_done = true, progress = false;
         for (int w = j; w < ncomp; w += nstore) {
-            if (lds_flag(vfull + w)) {
-                const long long qq = __hip_atomic_load(&s_pair[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            bool busy = false;
+            for (int bb = 0; bb < 2; bb++) {
+                const int slot = 2 * w + bb;
+                if (!lds_flag(vfull + slot)) continue;
+                const long long qq = __hip_atomic_load(&s_pair[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 const int64_t q = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)qq) |
                                   ((int64_t)__builtin_amdgcn_readfirstlane((int)(qq >> 32)) << 32);
                 const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
                                                 (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl,
                                                 d.ku, d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
-                if (IOMODE) store_pair_packed<R>(out_base + (size_t)w * 64 * R, lane, d, q);
-                else store_pair<R>(out_base + (size_t)w * 64 * R, lane, g, 2 * q);
+                if (IOMODE) store_pair_packed<R>(out_base + (size_t)slot * 64 * R, lane, d, q);
+                else store_pair<R>(out_base + (size_t)slot * 64 * R, lane, g, 2 * q);
                 __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the staging has been read
                 compiler_fence();
-                if (lane == 0) lds_set(vfull + w, 0);
+                if (lane == 0) lds_set(vfull + slot, 0);
+                busy = true;
+            }
+            if (busy) {
                 progress = true;
                 all_done = false;
             } else if (!lds_flag(vdone + w)) {
-                all_done = false;   // the flag is read before `done`: a raised done with an empty staging is final
+                // the flags are read before `done`: a raised done with empty stagings is final
+                all_done = false;
             }
         }
         if (all_done) break;
@@ -1081,17 +1126,16 @@ const void* pick_split(int R, bool all3, bool io = false)
     }
 }
 
-// compute / storer wavefronts of the split kernel.  A storer wavefront moves one
-// pair's output (64 R 16-byte stores) in about the time a compute wavefront
-// needs for R/2 of them, so the workgroup is as many compute wavefronts as LDS
-// holds (table + input and output staging each, at most 7 / 10) and storers
-// for the rest of the launch bound (12 wavefronts for R = 8, 16 below).
+// compute / storer wavefronts of the split kernel: as many compute wavefronts
+// as LDS holds (table + two output stagings each, at most 7 for R = 8 and 10
+// below) and storers for the rest of the launch bound (12 wavefronts for R = 8,
+// 16 below).
 bool split_shape(int R, int* ncomp, int* nstore)
 {
     const int maxw = R <= 4 ? 16 : (R == 8 ? 12 : 5);
     const int cmax = R <= 4 ? 10 : (R == 8 ? 7 : 2);
     const int64_t table = jwdev_table64_elems(R) * 8;
-    const int64_t per = 2LL * 2 * 64 * R * 8;   // input + output staging of 2 systems
+    const int64_t per = 2LL * 2 * 64 * R * 8;   // two output stagings of 2 systems
     const int64_t avail = kLdsBytesMax - 256 - table;
     int c = (int)(avail / per);
     if (c > cmax) c = cmax;
@@ -1206,7 +1250,10 @@ int jwdev_main64s_shape(int R, int* ncomp, int* nstore)
 
 int64_t jwdev_main64s_lds_bytes(int R, int ncomp)
 {
-    return jwdev_table64_elems(R) * 8 + 2LL * 2 * 64 * R * 8 * ncomp;
+    // two output stagings per compute wavefront, at least the table build's scratch
+    const int64_t staging = 2LL * 2 * 64 * R * 8 * ncomp;
+    const int64_t scratch = (int64_t)prologue_lds_bytes(64 * R, 8);
+    return jwdev_table64_elems(R) * 8 + (staging > scratch ? staging : scratch);
 }
 
 int jwdev_main64s_occupancy(int R)
@@ -1225,7 +1272,8 @@ int jwdev_main64s_occupancy(int R)
     return best;
 }
 
-int jwdev_main64s(const jw_run_desc* r, int R, int grid, int* d_err, void* stream)
+int jwdev_main64s(const jw_run_desc* r, const jw_source_desc* src, const void* matrices, int R, int grid, int* d_err,
+                  void* stream)
 {
     int c = 0, st = 0;
     if (!jwdev_main64s_shape(R, &c, &st)) { snprintf(g_err64, sizeof(g_err64), "no split kernel for R=%d", R); return -1; }
@@ -1233,7 +1281,7 @@ int jwdev_main64s(const jw_run_desc* r, int R, int grid, int* d_err, void* strea
     const size_t lds = (size_t)jwdev_main64s_lds_bytes(R, c);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -2;
-    void* args[] = {const_cast<jw_run_desc*>(r), &c, &d_err};
+    void* args[] = {const_cast<jw_run_desc*>(r), &c, &d_err, const_cast<jw_source_desc*>(src), &matrices};
     e = hipLaunchKernel(f, dim3(grid), dim3(64 * (c + st)), args, lds, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : -3;
 }

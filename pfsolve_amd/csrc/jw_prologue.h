@@ -1,0 +1,378 @@
+// jw_prologue.h -- the coefficient-table build of the JW step (device code,
+// included by jw_device.hip and jw_main64.hip): one workgroup turns one matrix
+// set in the reference storage into the table the main kernels apply.
+//
+// Reference semantics restated (paths relative to the reference root):
+//   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_ReadWrite.h:1561-1733
+//   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_TridiagonalSolve.h:1578-1929
+#ifndef PFSOLVE_JW_PROLOGUE_H
+#define PFSOLVE_JW_PROLOGUE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jw_common.h"
+
+// ------------------------------------------------------------------------
+// table layout (elements of T):
+//   coefficient pair p (0..4) of row-in-lane i, lane t:  ((p*R + i)*L + t)*2 + {0,1}
+//     p0 = (vm2, vm1)  p1 = (v0, vp1)  p2 = (vp2, l)  p3 = (g, h)  p4 = (D, E)
+//   scan multipliers of level k, lane t: SC + (k*L + t)*4 + {0: F, 1: B1, 2: B2}
+// ------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int64_t tab_coef(int p, int i, int t, int c, int R, int L)
+{
+    return ((int64_t)(p * R + i) * L + t) * 2 + c;
+}
+__host__ __device__ __forceinline__ int64_t tab_scan(int k, int t, int j, int R, int L)
+{
+    return (int64_t)JW_NCOEF * R * L + ((int64_t)k * L + t) * 4 + j;
+}
+__host__ __device__ __forceinline__ int ilog2_rt(int v)
+{
+    int r = 0;
+    while ((1 << r) < v) r++;
+    return r;
+}
+
+// ------------------------------------------------------------------------
+// prologue: one workgroup (256 threads) per matrix set
+//
+// Every per-row quantity is staged in LDS (arrays of R*L elements):
+//   v[-2..2], l, g, h, D, E   (rows >= N are zero)
+// then written in one of two table formats:
+//   format 1: the pairs above + Hillis-Steele multipliers over L lanes
+//             (jw_main_kernel);
+//   format 2 (L = 64): scalings folded in, a_r = D_r g_r,
+//       v'_d[r] = a_r v_d[r],  l'_r = a_r l_r / a_{r-1},  h'_r = D_r h_r / D_{r+1},
+//     so that the three recurrences are W_r = u'_r - l'_r W_{r-1},
+//     Z_r = W_r - h'_r Z_{r+1}, y_r = Z_r - E_r y_{r+1} (W = a w, Z = D z);
+//     8 coefficients per row, and per recurrence and lane 8 multipliers of the
+//     DPP scan: products over the 2^k lanes a row_shr/row_shl:2^k step
+//     combines (k = 0..3), then the cross-row multipliers (forward: row
+//     prefix and the row_bcast31 step; backward: one per later row start)
+//     (jw_main64_kernel).  Table size JW_TAB64_ELEMS(R).
+// Matrix loads are spread over all threads (every load of a thread's rows issued
+// at once, the band kept in registers until the fold); the LU scan runs on
+// wavefront 0.
+// ------------------------------------------------------------------------
+constexpr int kPrologueThreads = 512;
+constexpr int kProRows = 4;   // rows per thread: R * L <= 2048
+
+// LDS of prologue_body: 5 arrays of R*L elements (ld -> l -> l', ud -> h -> h', D, E, g)
+static inline size_t prologue_lds_bytes(int RL, size_t elt) { return 5 * (size_t)RL * elt; }
+
+// the band coefficients v_{-2..2} of row r of the matvec (zero outside the
+// matrix), P = storage position of row r in the permuted diagonals
+template <typename T>
+__device__ __forceinline__ void row_band(const jw_source_desc& d, const void* mat, int64_t r, int64_t N, int64_t P,
+                                         T (&v)[5])
+{
+    using A = Ar<T>;
+    const int mf = d.mem_f64;
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[k] = A::zero();
+    switch (d.band_kind) {
+    case JW_BAND_DIAG:
+        v[2] = mem_load<T>(mat, d.mdA + P, mf);
+        break;
+    case JW_BAND_UPPER:
+        v[2] = mem_load<T>(mat, d.mdA + P, mf);
+        if (r + 1 < N) v[3] = mem_load<T>(mat, d.udA + P, mf);
+        break;
+    case JW_BAND_LOWER:
+        v[2] = mem_load<T>(mat, d.mdA + P, mf);
+        if (r >= 1) v[1] = mem_load<T>(mat, d.ldA + P, mf);
+        break;
+    case JW_BAND_GB:
+#pragma unroll
+        for (int off = -2; off <= 2; off++) {
+            const int64_t col = r + off;
+            if (off < -d.KL || off > d.KU || col < 0 || col >= N) continue;
+            v[2 + off] = mem_load<T>(mat, d.offV + (int64_t)(d.KU - off) * N + col, mf);
+        }
+        break;
+    default:
+        v[2] = A::one();
+        break;
+    }
+    if (d.mdS >= 0) {
+        const T sc = mem_load<T>(mat, d.mdS + P, mf);
+#pragma unroll
+        for (int k = 0; k < 5; k++) v[k] = A::mul(v[k], sc);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void prologue_body(const jw_source_desc& d, const void* mat, T* table, int L, int R, int format,
+                              unsigned char* smem_raw)
+{
+    using A = Ar<T>;
+    const int RL = R * L;
+    T* sl = reinterpret_cast<T*>(smem_raw);    // raw ld, then l, then (format 2) l'
+    T* sh = sl + RL;                           // raw ud, then h, then (format 2) h'
+    T* sD = sh + RL;
+    T* sE = sD + RL;
+    T* sg = sE + RL;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int64_t N = d.N;
+    const int mf = d.mem_f64;
+    const bool has_b1 = d.solve_kind == JW_SOLVE_UPPER || d.solve_kind == JW_SOLVE_TRI;
+    const bool has_f = d.solve_kind == JW_SOLVE_LOWER || d.solve_kind == JW_SOLVE_TRI;
+
+    // phase 1: raw solve coefficients per row (zeros outside the matrix) into LDS,
+    // band coefficients into registers (format 1: straight into the table); row
+    // r = tid + j * nthr of thread tid
+    T vb[kProRows][5];
+#pragma unroll
+    for (int j = 0; j < kProRows; j++) {
+        const int r = tid + j * nthr;
+        T ldv = A::zero(), udv = A::zero(), Dv = A::zero(), Ev = A::zero();
+#pragma unroll
+        for (int k = 0; k < 5; k++) vb[j][k] = A::zero();
+        if (r < N) {
+            const int64_t P = d.permuted ? reorder_pos(r, N) : (int64_t)r;
+            row_band<T>(d, mat, r, N, P, vb[j]);
+            if (has_f && r >= 1) ldv = mem_load<T>(mat, d.ldS + P, mf);
+            if (has_b1 && r + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);
+            if (d.has_m) {
+                Dv = mem_load<T>(mat, d.mdM + P, mf);
+                if (r + 1 < N) Ev = mem_load<T>(mat, d.udM + P, mf);
+            } else {
+                Dv = A::one();
+            }
+        }
+        if (r < RL) {
+            sl[r] = ldv;
+            sh[r] = udv;
+            sD[r] = Dv;
+            sE[r] = Ev;
+            if (format == 1) {
+                const int t = r / R, i = r % R;
+                table[tab_coef(0, i, t, 0, R, L)] = vb[j][0];
+                table[tab_coef(0, i, t, 1, R, L)] = vb[j][1];
+                table[tab_coef(1, i, t, 0, R, L)] = vb[j][2];
+                table[tab_coef(1, i, t, 1, R, L)] = vb[j][3];
+                table[tab_coef(2, i, t, 0, R, L)] = vb[j][4];
+            }
+        }
+    }
+    __syncthreads();
+
+    // phase 2: factors of the unit-diagonal solve, lane t owns rows t*R .. t*R+R-1
+    if (d.solve_kind == JW_SOLVE_TRI) {
+        if (tid < 64) {
+            const int t = tid;
+            const bool active = t < L;
+            const int log2L = ilog2_rt(L);
+            // pivots p_r = 1 - m_r / p_{r-1}, m_r = ld_r ud_{r-1}: the Moebius map
+            // p -> (p - m)/p is the 2x2 matrix [[1,-m],[1,0]] on (num, den).
+            T a = A::one(), b = A::zero(), c = A::zero(), e = A::one();
+            if (active) {
+                for (int i = 0; i < R; i++) {
+                    const int r = t * R + i;
+                    T m = (r >= 1 && r < N) ? A::mul(sl[r], sh[r - 1]) : A::zero();
+                    T na = A::msub(a, m, c), nb = A::msub(b, m, e);
+                    c = a;
+                    e = b;
+                    a = na;
+                    b = nb;
+                    double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
+                    if (mx > 0.0) {
+                        int ex;
+                        frexp(mx, &ex);
+                        a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
+                    }
+                }
+            }
+            // inclusive scan over lanes: X_t <- X_t * X_{t-s}
+            for (int k = 0; k < log2L; k++) {
+                const int s = 1 << k;
+                const int src = (t >= s) ? t - s : t;
+                T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
+                if (active && t >= s) {
+                    T na = A::madd(A::mul(a, pa), b, pc);
+                    T nb = A::madd(A::mul(a, pb), b, pe);
+                    T nc = A::madd(A::mul(c, pa), e, pc);
+                    T ne = A::madd(A::mul(c, pb), e, pe);
+                    a = na; b = nb; c = nc; e = ne;
+                    double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
+                    if (mx > 0.0) {
+                        int ex;
+                        frexp(mx, &ex);
+                        a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
+                    }
+                }
+            }
+            // exclusive: composition of the maps of lanes 0..t-1 applied to (1,1)
+            const int src = (t >= 1) ? t - 1 : t;
+            T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
+            // g_{r-1} = 1 / p_{r-1} of the row before this lane's first
+            T gprev = A::one();
+            if (t >= 1) gprev = A::div(A::add(pc, pe), A::add(pa, pb));
+            // ud of that row is read before any lane overwrites its ud with h
+            T udprev = (active && t >= 1) ? sh[t * R - 1] : A::zero();
+            if (active) {
+                for (int i = 0; i < R; i++) {
+                    const int r = t * R + i;
+                    const T ldr = sl[r], udr = sh[r];
+                    T l = A::zero(), g = A::zero(), h = A::zero();
+                    if (r < N) {
+                        // l_r = ld_r / p_{r-1},  p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r,  h_r = ud_r g_r
+                        l = (r >= 1) ? A::mul(ldr, gprev) : A::zero();
+                        const T p = A::sub(A::one(), A::mul(l, udprev));
+                        g = A::div(A::one(), p);
+                        h = A::mul(udr, g);
+                        gprev = g;
+                    }
+                    udprev = udr;
+                    sl[r] = l;
+                    sg[r] = g;
+                    sh[r] = h;
+                }
+            }
+        }
+    } else {
+        for (int r = tid; r < RL; r += nthr) {
+            const bool in = r < N;
+            if (!(in && d.solve_kind == JW_SOLVE_LOWER)) sl[r] = A::zero();
+            sg[r] = in ? A::one() : A::zero();
+            if (!(in && d.solve_kind == JW_SOLVE_UPPER)) sh[r] = A::zero();
+        }
+    }
+    __syncthreads();
+
+    if (format == 2) {
+        // fold the scalings (row-parallel); l' and h' replace l and h in LDS (a row
+        // reads only its own l and h)
+#pragma unroll
+        for (int j = 0; j < kProRows; j++) {
+            const int r = tid + j * nthr;
+            if (r >= RL) continue;
+            const int t = r / R, i = r % R;
+            const T ar = A::mul(sD[r], sg[r]);
+            T cf[8];
+#pragma unroll
+            for (int k = 0; k < 5; k++) cf[k] = A::mul(ar, vb[j][k]);
+            T lp = A::zero(), hp = A::zero();
+            if (r >= 1 && r < N) lp = A::div(A::mul(ar, sl[r]), A::mul(sD[r - 1], sg[r - 1]));
+            if (r + 1 < N) hp = A::div(A::mul(sD[r], sh[r]), sD[r + 1]);
+            cf[5] = lp;
+            cf[6] = hp;
+            cf[7] = sE[r];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                table[tab_coef64(p, i, t, R)] = cf[2 * p];
+                table[tab_coef64(p, i, t, R) + 1] = cf[2 * p + 1];
+            }
+            sl[r] = lp;
+            sh[r] = hp;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const int t = tid;
+            T bf = A::one(), bb1 = A::one(), bb2 = A::one();
+            for (int i = 0; i < R; i++) {
+                const int r = t * R + i;
+                bf = A::mul(bf, A::neg(sl[r]));
+                bb1 = A::mul(bb1, A::neg(sh[r]));
+                bb2 = A::mul(bb2, A::neg(sE[r]));
+            }
+            // in-row Hillis-Steele: before step k the value covers the 2^k lanes a
+            // shift of 2^k combines; after 4 steps, the row prefix / suffix
+            const int tr = t & 15;
+            const int64_t SC = (int64_t)8 * R * 64;
+            T cur[3] = {bf, bb1, bb2};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int s = 1 << k;
+#pragma unroll
+                for (int j = 0; j < 3; j++) table[SC + (int64_t)(j * 8 + k) * 64 + t] = cur[j];
+                T up = lane_fetch(cur[0], tr >= s ? t - s : t);
+                T dn1 = lane_fetch(cur[1], tr + s <= 15 ? t + s : t);
+                T dn2 = lane_fetch(cur[2], tr + s <= 15 ? t + s : t);
+                if (tr >= s) cur[0] = A::mul(cur[0], up);
+                if (tr + s <= 15) {
+                    cur[1] = A::mul(cur[1], dn1);
+                    cur[2] = A::mul(cur[2], dn2);
+                }
+            }
+            const int rw = t >> 4;
+            // forward cross-row combine in two broadcasts: rows 1,3 take the end of
+            // rows 0,2 (multiplier: row prefix), then rows 2,3 take lane 31, which
+            // row 3 reaches through all of row 2 (prefix of lane 47 times its own)
+            {
+                const T pre47 = lane_fetch(cur[0], 47);
+                T m2 = A::zero();
+                if (rw == 2) m2 = cur[0];
+                if (rw == 3) m2 = A::mul(pre47, cur[0]);
+                table[SC + (int64_t)4 * 64 + t] = cur[0];
+                table[SC + (int64_t)5 * 64 + t] = m2;
+                table[SC + (int64_t)6 * 64 + t] = A::zero();
+                table[SC + (int64_t)7 * 64 + t] = A::zero();
+            }
+            // backward cross-row combine from the in-row results S_1..S_3 at the
+            // row starts (lanes 16, 32, 48): a lane of row r adds
+            //   suf_t * F_{r+1},  F_q = S_q + P_q F_{q+1},  P_q = suffix product of lane 16q,
+            // i.e. c_k,t * S_k with c_k,t = suf_t * prod_{r<q<k} P_q for k > r, else 0
+            for (int j = 1; j < 3; j++) {
+                const T P1 = lane_fetch(cur[j], 16), P2 = lane_fetch(cur[j], 32);
+                T c1 = A::zero(), c2 = A::zero(), c3 = A::zero();
+                if (rw == 0) { c1 = cur[j]; c2 = A::mul(cur[j], P1); c3 = A::mul(c2, P2); }
+                if (rw == 1) { c2 = cur[j]; c3 = A::mul(cur[j], P2); }
+                if (rw == 2) c3 = cur[j];
+                table[SC + (int64_t)(j * 8 + 4) * 64 + t] = c1;
+                table[SC + (int64_t)(j * 8 + 5) * 64 + t] = c2;
+                table[SC + (int64_t)(j * 8 + 6) * 64 + t] = c3;
+                table[SC + (int64_t)(j * 8 + 7) * 64 + t] = A::zero();
+            }
+        }
+        return;
+    }
+
+    // format 1: pairs (vm2,vm1) (v0,vp1) (vp2,l) (g,h) (D,E) + Hillis-Steele multipliers
+    // (the band pairs were written in phase 1)
+    for (int r = tid; r < RL; r += nthr) {
+        const int t = r / R, i = r % R;
+        table[tab_coef(2, i, t, 1, R, L)] = sl[r];
+        table[tab_coef(3, i, t, 0, R, L)] = sg[r];
+        table[tab_coef(3, i, t, 1, R, L)] = sh[r];
+        table[tab_coef(4, i, t, 0, R, L)] = sD[r];
+        table[tab_coef(4, i, t, 1, R, L)] = sE[r];
+    }
+    if (tid < 64) {
+        const int t = tid;
+        const bool active = t < L;
+        const int log2L = ilog2_rt(L);
+        T bf = A::one(), bb1 = A::one(), bb2 = A::one();
+        if (active) {
+            for (int i = 0; i < R; i++) {
+                const int r = t * R + i;
+                bf = A::mul(bf, A::neg(sl[r]));
+                bb1 = A::mul(bb1, A::neg(sh[r]));
+                bb2 = A::mul(bb2, A::neg(sE[r]));
+            }
+        }
+        for (int k = 0; k < log2L; k++) {
+            const int s = 1 << k;
+            if (active) {
+                table[tab_scan(k, t, 0, R, L)] = bf;
+                table[tab_scan(k, t, 1, R, L)] = bb1;
+                table[tab_scan(k, t, 2, R, L)] = bb2;
+                table[tab_scan(k, t, 3, R, L)] = A::zero();
+            }
+            const int su = (t >= s) ? t - s : t;
+            const int sdn = (t + s < L) ? t + s : t;
+            T ubf = lane_fetch(bf, su);
+            T ub1 = lane_fetch(bb1, sdn);
+            T ub2 = lane_fetch(bb2, sdn);
+            if (active && t >= s) bf = A::mul(bf, ubf);
+            if (active && t + s < L) {
+                bb1 = A::mul(bb1, ub1);
+                bb2 = A::mul(bb2, ub2);
+            }
+        }
+    }
+}
+
+#endif  // PFSOLVE_JW_PROLOGUE_H

@@ -218,15 +218,14 @@ static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveL
             return PFSOLVE_ERROR_EMPTY_bufferSize;
     }
     run->table = p->table;
-    if (jwdev_prologue(src, cfg->buffer[0], p->table, 64, p->R64, p->precision, 2, s) != 0)
-        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     if (ncx <= 0) return PFSOLVE_SUCCESS;
     int64_t need = (ncx + p->split64 - 1) / p->split64;
     int64_t grid = (int64_t)p->cu_count * p->occupancy_split;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
     if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
-    if (jwdev_main64s(run, p->R64, (int)grid, p->d_err, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    if (jwdev_main64s(run, src, cfg->buffer[0], p->R64, (int)grid, p->d_err, s) != 0)
+        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
     return PFSOLVE_SUCCESS;
 }
@@ -276,10 +275,11 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
         const uintptr_t base = (uintptr_t)run.rhs;
         run.vec2 = p->precision == JW_PREC_F64 && (base % 16) == 0 && (run.stride % 2) == 0 && (run.offSol % 2) == 0;
         if (p->R64 && run.vec2) {
-            /* wave-per-system kernel: table format 2 */
-            if (jwdev_prologue(&src, cfg->buffer[0], p->table, 64, p->R64, p->precision, 2, s) != 0)
-                return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            /* wave-per-system kernel: table format 2, built by a prologue launch or,
+             * in the compute / store split kernel, by every workgroup in LDS */
             if (nsys <= 0) return PFSOLVE_SUCCESS;
+            if (!p->split64 && jwdev_prologue(&src, cfg->buffer[0], p->table, 64, p->R64, p->precision, 2, s) != 0)
+                return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             /* wavefront passes: pairs of systems, single systems for R = 32 */
             const int64_t npass = p->R64 == 32 ? nsys : (nsys + 1) / 2;
             const int per_wg = p->split64 ? p->split64 : p->waves64;
@@ -289,7 +289,8 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
             if (grid < 1) grid = 1;
             if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
             if (p->split64) {
-                if (jwdev_main64s(&run, p->R64, (int)grid, p->d_err, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+                if (jwdev_main64s(&run, &src, cfg->buffer[0], p->R64, (int)grid, p->d_err, s) != 0)
+                    return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             } else if (jwdev_main64(&run, p->R64, p->waves64, (int)grid, s) != 0) {
                 return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             }
