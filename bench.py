@@ -70,6 +70,28 @@ def algorithmic_bytes(N: int, nreal: int, rows_read: int, rows_written: int, elt
     return nreal * elt * (rows_read + rows_written) + 10 * N * elt
 
 
+def kernel_window_ms(torch, lib, plan, stream, k: int, launch) -> float:
+    """Average duration of the plan's main kernel over k back-to-back launches on
+    `stream`: the library records one HIP event just before the first kernel and
+    one just after the last (pfsolve_set_kernel_events), so the window holds the k
+    kernels and the launch gaps between them."""
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    # torch creates the underlying HIP events on first record; create them before
+    # handing the raw handles to the library
+    e0.record(stream)
+    e1.record(stream)
+    stream.synchronize()
+    with torch.cuda.stream(stream):
+        for i in range(k):
+            lib.pfsolve_set_kernel_events(C.byref(plan.app), C.c_void_p(e0.cuda_event) if i == 0 else None,
+                                          C.c_void_p(e1.cuda_event) if i == k - 1 else None)
+            launch()
+    stream.synchronize()
+    lib.pfsolve_set_kernel_events(C.byref(plan.app), None, None)
+    return e0.elapsed_time(e1) / k
+
+
 def cpu_baseline(buf, offs, N, budget_s=10.0):
     """Oracle FP64 restatement of the reference step on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -308,9 +330,7 @@ def bench_fused_blocks(torch, pf, steps: int, warmup: int, N: int = 512, ncx: in
         for _ in range(warmup):
             fn()
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-        kst = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-        ken = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-        for e in evs + kst + ken:
+        for e in evs:
             e.record(stream)
         stream.synchronize()
         with torch.cuda.stream(stream):
@@ -318,16 +338,9 @@ def bench_fused_blocks(torch, pf, steps: int, warmup: int, N: int = 512, ncx: in
                 evs[i].record(stream)
                 fn()
             evs[steps].record(stream)
-            if kernel_events:
-                for i in range(steps):
-                    lib.pfsolve_set_kernel_events(C.byref(fused.app), C.c_void_p(kst[i].cuda_event),
-                                                  C.c_void_p(ken[i].cuda_event))
-                    fn()
         stream.synchronize()
-        if kernel_events:
-            lib.pfsolve_set_kernel_events(C.byref(fused.app), None, None)
         ms = evs[0].elapsed_time(evs[steps]) / steps
-        kms = float(np.mean([kst[i].elapsed_time(ken[i]) for i in range(steps)])) if kernel_events else None
+        kms = kernel_window_ms(torch, lib, fused, stream, steps, fn) if kernel_events else None
         return ms, kms
 
     ms_f, kms_f = timed(run_fused, kernel_events=True)
@@ -470,11 +483,7 @@ def main() -> int:
 
     K = args.steps
     step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
-    kst = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    ken = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    # torch creates the underlying HIP event on first record; create them before handing
-    # the raw handles to the library
-    for e in kst + ken + step_ev:
+    for e in step_ev:
         e.record(stream)
     torch.cuda.synchronize()
     # timed region: K steps, one event per step boundary
@@ -487,22 +496,14 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     assert torch.isfinite(x).all().item(), "non-finite output"
-    # main-kernel durations: K more launches with events recorded around the kernel
-    # alone (outside the timed region: the two extra event packets per launch slow
-    # the step down)
-    with torch.cuda.stream(stream):
-        for i in range(K):
-            lib.pfsolve_set_kernel_events(C.byref(plan.app), C.c_void_p(kst[i].cuda_event),
-                                          C.c_void_p(ken[i].cuda_event))
-            plan.append(**launch)
-    torch.cuda.synchronize()
-    lib.pfsolve_set_kernel_events(C.byref(plan.app), None, None)
+    # main-kernel duration: K more launches (outside the timed region), an event
+    # recorded just before the first kernel and just after the last one, so the
+    # window holds the K kernels and the gaps between them and no other work
+    kavg = kernel_window_ms(torch, lib, plan, stream, K, lambda: plan.append(**launch))
 
     total_ms = step_ev[0].elapsed_time(step_ev[K])
     ms_step = total_ms / K
     steps_ms = np.array([step_ev[i].elapsed_time(step_ev[i + 1]) for i in range(K)])
-    kern_ms = np.array([kst[i].elapsed_time(ken[i]) for i in range(K)])
-    kavg = float(kern_ms.mean())
     rows_read = ZP[1] - ZP[0]
     alg = algorithmic_bytes(N, nreal, rows_read, N)
     achieved = alg / (kavg * 1e-3)

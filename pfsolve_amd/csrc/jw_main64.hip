@@ -802,11 +802,10 @@ __device__ __forceinline__ void store_pair_packed(const double2* stg, int lane, 
 
 // Compute / store split.  The first `ncomp` wavefronts of a workgroup compute,
 // the others store.  Every workgroup first builds the coefficient table in its
-// LDS.  A compute wavefront owns two output stagings, used in turn; per pass it
-// takes the rows of its pair that it loaded straight into registers one pass
-// earlier, issues the loads of its next pair at once, computes, waits until a
-// storer has emptied this pass's staging (it held the pair of two passes ago),
-// writes the results there and raises the staging's `full` flag.  Storer
+// LDS.  A compute wavefront owns an output staging; per pass it takes the rows
+// of its pair that it loaded straight into registers one pass earlier, issues
+// the loads of its next pair at once, computes, waits until a storer has
+// emptied its staging, writes the results there and raises its `full` flag.  Storer
 // wavefronts poll the flags of their compute wavefronts, stream full stagings
 // to global memory (nontemporal stores) and lower the flag.  Compute
 // wavefronts issue no global stores, so their vmcnt waits never wait for write
@@ -850,10 +849,8 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
 {
     constexpr int NTAB = JW_TAB64_ELEMS(R);
     extern __shared__ __align__(16) unsigned char smem[];
-    // two output stagings per compute wavefront (slot 2 w + b), each with its flag
-    // and pair index
-    __shared__ int s_full[32], s_done[16];
-    __shared__ long long s_pair[32];
+    __shared__ int s_full[16], s_done[16];
+    __shared__ long long s_pair[16];
     const double* tab = reinterpret_cast<const double*>(smem);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -881,8 +878,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
     prologue_body<double>(src, mat, reinterpret_cast<double*>(smem), 64, R, 2, smem + NTAB * 8);
     if (wave < ncomp) {
         if (lane == 0) {
-            lds_set(&s_full[2 * wave], 0);
-            lds_set(&s_full[2 * wave + 1], 0);
+            lds_set(&s_full[wave], 0);
             lds_set(&s_done[wave], 0);
         }
     }
@@ -892,7 +888,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
 
     if (wave < ncomp) {
         bool ok = true;
-        int b = 0;   // output staging of this pass
+        double2* stg = out_base + (size_t)wave * 64 * R;
         // fused block input: whole systems, zero-filled past N by the packed loads, so
         // the keep-range is [0, N rounded up to even) and read_pair has no edge rows
         const int in_lo = IOMODE ? 0 : (int)d.in_lo;
@@ -911,20 +907,15 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
                 else load_pair_direct<R>(pf, lane, g, 2 * qn);
             }
             compute_pair<R, ALL3>(tab, lane, g, x0, x1);
-            // this pass's output staging (two alternate, so a storer may still be
-            // emptying the other one)
-            const int slot = 2 * wave + b;
-            if (!wait_flag(vfull + slot, 0)) { ok = false; break; }
-            write_blocked<R>(out_base + (size_t)slot * 64 * R, lane, x0, x1);
-            if (lane == 0) __hip_atomic_store(&s_pair[slot], (long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!wait_flag(vfull + wave, 0)) { ok = false; break; }
+            write_blocked<R>(stg, lane, x0, x1);
+            if (lane == 0) __hip_atomic_store(&s_pair[wave], (long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): staging and pair index written
             compiler_fence();
-            if (lane == 0) lds_set(vfull + slot, 1);
-            b ^= 1;
+            if (lane == 0) lds_set(vfull + wave, 1);
             q = qn;
         }
-        if (ok) ok = wait_flag(vfull + 2 * wave, 0);
-        if (ok) ok = wait_flag(vfull + 2 * wave + 1, 0);
+        if (ok) ok = wait_flag(vfull + wave, 0);
         __builtin_amdgcn_s_waitcnt(0);
         compiler_fence();
         if (lane == 0) {
@@ -941,29 +932,22 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
     for (;;) {
         bool all_done = true, progress = false;
         for (int w = j; w < ncomp; w += nstore) {
-            bool busy = false;
-            for (int bb = 0; bb < 2; bb++) {
-                const int slot = 2 * w + bb;
-                if (!lds_flag(vfull + slot)) continue;
-                const long long qq = __hip_atomic_load(&s_pair[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (lds_flag(vfull + w)) {
+                const long long qq = __hip_atomic_load(&s_pair[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 const int64_t q = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)qq) |
                                   ((int64_t)__builtin_amdgcn_readfirstlane((int)(qq >> 32)) << 32);
                 const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
                                                 (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl,
                                                 d.ku, d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
-                if (IOMODE) store_pair_packed<R>(out_base + (size_t)slot * 64 * R, lane, d, q);
-                else store_pair<R>(out_base + (size_t)slot * 64 * R, lane, g, 2 * q);
+                if (IOMODE) store_pair_packed<R>(out_base + (size_t)w * 64 * R, lane, d, q);
+                else store_pair<R>(out_base + (size_t)w * 64 * R, lane, g, 2 * q);
                 __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the staging has been read
                 compiler_fence();
-                if (lane == 0) lds_set(vfull + slot, 0);
-                busy = true;
-            }
-            if (busy) {
+                if (lane == 0) lds_set(vfull + w, 0);
                 progress = true;
                 all_done = false;
             } else if (!lds_flag(vdone + w)) {
-                // the flags are read before `done`: a raised done with empty stagings is final
-                all_done = false;
+                all_done = false;   // the flag is read before `done`: a raised done with an empty staging is final
             }
         }
         if (all_done) break;
@@ -1126,16 +1110,17 @@ const void* pick_split(int R, bool all3, bool io = false)
     }
 }
 
-// compute / storer wavefronts of the split kernel: as many compute wavefronts
-// as LDS holds (table + two output stagings each, at most 7 for R = 8 and 10
-// below) and storers for the rest of the launch bound (12 wavefronts for R = 8,
-// 16 below).
+// compute / storer wavefronts of the split kernel: compute wavefronts up to
+// what LDS holds (table + output staging each), at most 8 for R = 8 and 12
+// below, so that every SIMD holds the same number (2 or 3: a SIMD with one more
+// than the others is the slowest), and storers for the rest of the launch bound
+// (12 wavefronts for R = 8, 16 below).
 bool split_shape(int R, int* ncomp, int* nstore)
 {
     const int maxw = R <= 4 ? 16 : (R == 8 ? 12 : 5);
-    const int cmax = R <= 4 ? 10 : (R == 8 ? 7 : 2);
+    const int cmax = R <= 4 ? 12 : (R == 8 ? 8 : 2);
     const int64_t table = jwdev_table64_elems(R) * 8;
-    const int64_t per = 2LL * 2 * 64 * R * 8;   // two output stagings of 2 systems
+    const int64_t per = 2LL * 64 * R * 8;   // output staging of 2 systems
     const int64_t avail = kLdsBytesMax - 256 - table;
     int c = (int)(avail / per);
     if (c > cmax) c = cmax;
@@ -1250,8 +1235,8 @@ int jwdev_main64s_shape(int R, int* ncomp, int* nstore)
 
 int64_t jwdev_main64s_lds_bytes(int R, int ncomp)
 {
-    // two output stagings per compute wavefront, at least the table build's scratch
-    const int64_t staging = 2LL * 2 * 64 * R * 8 * ncomp;
+    // output staging of the compute wavefronts, at least the table build's scratch
+    const int64_t staging = 2LL * 64 * R * 8 * ncomp;
     const int64_t scratch = (int64_t)prologue_lds_bytes(64 * R, 8);
     return jwdev_table64_elems(R) * 8 + (staging > scratch ? staging : scratch);
 }
