@@ -70,6 +70,31 @@ def algorithmic_bytes(N: int, nreal: int, rows_read: int, rows_written: int, elt
     return nreal * elt * (rows_read + rows_written) + 10 * N * elt
 
 
+def time_plan_steps(torch, lib, plan, stream, warmup: int, k: int, step):
+    """W untimed steps, then K timed steps on `stream` (one event per step boundary,
+    synchronised on both sides); then the main-kernel window (outside the timed
+    region).  Returns (ms per step, per-step ms, wall seconds, kernel ms)."""
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(k + 1)]
+    for e in ev:
+        e.record(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        for i in range(k):
+            ev[i].record(stream)
+            step()
+        ev[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev[0].elapsed_time(ev[k]) / k
+    per = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(k)])
+    kavg = kernel_window_ms(torch, lib, plan, stream, k, step)
+    return ms, per, wall, kavg
+
+
 def kernel_window_ms(torch, lib, plan, stream, k: int, launch) -> float:
     """Average duration of the plan's main kernel over k back-to-back launches on
     `stream`: the library records one HIP event just before the first kernel and
@@ -454,56 +479,35 @@ def main() -> int:
     buf, (offV, offT, offM) = gen_fused_matrices(N, 0, lib.pfsolve_reorder_i0)
     dmat = torch.from_numpy(buf).cuda()
     g = torch.Generator(device="cuda").manual_seed(0)
-    x0 = torch.rand(nreal * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
-    x = x0.clone()
+    # complex input, interleaved (re, im) rows, one system per N complex elements;
+    # output: the packed real systems (re, im halves of a complex system adjacent)
+    xin = torch.rand(2 * N_COMPLEX * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    y = torch.zeros(nreal * N, dtype=torch.float64, device="cuda")
     stream = torch.cuda.Stream()
     sptr = stream.cuda_stream
-    mask = pf.RUNTIME_SCALEC | pf.RUNTIME_INPUTZEROPAD
     bsize = (C.c_uint64 * 1)(dmat.numel() * 8)
-    osize = (C.c_uint64 * 1)(x.numel() * 8)
-    plan = pf.Plan(dmat.data_ptr(), x.data_ptr(), stream=sptr, size=[N, nreal, 1], jw_type=14, doublePrecision=1,
-                   scaleC=1.0, offsetV=offV, offsetT=offT, offsetM=offM, jw_control_bitmask=mask)
+    isize = (C.c_uint64 * 1)(xin.numel() * 8)
+    osize = (C.c_uint64 * 1)(y.numel() * 8)
+    # block type 4 (complex strided in, packed out) with the ScaleC operator (104),
+    # scaleC 0.75, fused into the step; runtime input keep-range [480, 512)
+    plan = pf.Plan(dmat.data_ptr(), y.data_ptr(), stream=sptr, size=[N, nreal, 1], jw_type=14, doublePrecision=1,
+                   scaleC=1.0, offsetV=offV, offsetT=offT, offsetM=offM, inputBlock=104, inputBlockScale=SCALE_C,
+                   inputBufferStride=[N], jw_control_bitmask=pf.RUNTIME_INPUTZEROPAD)
     plan.app.configuration.bufferSize = C.cast(bsize, pf._abi.u64p)
+    plan.app.configuration.inputBufferSize = C.cast(isize, pf._abi.u64p)
     plan.app.configuration.outputBufferSize = C.cast(osize, pf._abi.u64p)
     geo = plan.geometry()
-    fast = geo.fast_rows_per_lane > 0
-    if fast and geo.fast_store_waves:
-        kname = f"jw_main64s_kernel<{geo.fast_rows_per_lane}, true>"
-    elif fast:
-        kname = f"jw_main64_kernel<{geo.fast_rows_per_lane}, true>"
-    else:
-        kname = f"jw_main_kernel<double,{geo.rows_per_lane},{geo.lanes}>"
-    launch = dict(scaleC=SCALE_C, inputZeropad=list(ZP))
-
-    for _ in range(args.warmup):
-        plan.append(**launch)
-    torch.cuda.synchronize()
-    x.copy_(x0)
-    torch.cuda.synchronize()
+    if not (geo.fast_rows_per_lane > 0 and geo.fast_store_waves):
+        print("the compute / store split kernel is not available for this workload", file=sys.stderr)
+        return 1
+    kname = f"jw_main64s_kernel<{geo.fast_rows_per_lane}, true, 2>"
+    launch = dict(inputZeropad=list(ZP), inputBuffer=xin.data_ptr())
 
     K = args.steps
-    step_ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
-    for e in step_ev:
-        e.record(stream)
-    torch.cuda.synchronize()
-    # timed region: K steps, one event per step boundary
-    t0 = time.perf_counter()
-    with torch.cuda.stream(stream):
-        for i in range(K):
-            step_ev[i].record(stream)
-            plan.append(**launch)
-        step_ev[K].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    assert torch.isfinite(x).all().item(), "non-finite output"
-    # main-kernel duration: K more launches (outside the timed region), an event
-    # recorded just before the first kernel and just after the last one, so the
-    # window holds the K kernels and the gaps between them and no other work
-    kavg = kernel_window_ms(torch, lib, plan, stream, K, lambda: plan.append(**launch))
-
-    total_ms = step_ev[0].elapsed_time(step_ev[K])
-    ms_step = total_ms / K
-    steps_ms = np.array([step_ev[i].elapsed_time(step_ev[i + 1]) for i in range(K)])
+    ms_step, steps_ms, wall, kavg = time_plan_steps(torch, lib, plan, stream, args.warmup, K,
+                                                    lambda: plan.append(**launch))
+    assert torch.isfinite(y).all().item(), "non-finite output"
+    plan.close()
     rows_read = ZP[1] - ZP[0]
     alg = algorithmic_bytes(N, nreal, rows_read, N)
     achieved = alg / (kavg * 1e-3)
@@ -514,7 +518,24 @@ def main() -> int:
             traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
-    plan.close()
+
+    # the same step on real systems already in the packed layout, solved in place,
+    # with scaleC 0.75 as the step's row-0 scale and the same keep-range
+    del xin
+    plan2 = pf.Plan(dmat.data_ptr(), y.data_ptr(), stream=sptr, size=[N, nreal, 1], jw_type=14, doublePrecision=1,
+                    scaleC=1.0, offsetV=offV, offsetT=offT, offsetM=offM,
+                    jw_control_bitmask=pf.RUNTIME_SCALEC | pf.RUNTIME_INPUTZEROPAD)
+    launch2 = dict(scaleC=SCALE_C, inputZeropad=list(ZP))
+    ms2, steps2, _, kavg2 = time_plan_steps(torch, lib, plan2, stream, args.warmup, K, lambda: plan2.append(**launch2))
+    plan2.close()
+    in_place = {
+        "workload": "fused JW step on packed real systems in place, N=512, 131072 real systems, runtime scaleC "
+                    "0.75 on row 0, runtime inputZeropad=[480,512), FP64",
+        "value": N_COMPLEX / (ms2 * 1e-3), "unit": "systems/s", "ms_per_step": ms2,
+        "p50_ms": float(np.percentile(steps2, 50)), "p99_ms": float(np.percentile(steps2, 99)),
+        "kernel": f"jw_main64s_kernel<{geo.fast_rows_per_lane}, true, 0>", "kernel_ms": kavg2,
+        "hbm_frac": alg / (kavg2 * 1e-3) / HBM_PEAK,
+    }
 
     result = {
         "metric": METRIC,
@@ -530,13 +551,14 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic (numpy/torch RNG, seeded; distributions of the stated workload)",
         "config": {
-            "workload": "fused JW step y=M^-1 T^-1 (V x), N=512, 65536 complex systems (131072 real, XXII), "
-                        "KL=KU=2, scaleC=0.75 (runtime, row 0), inputZeropad=[480,512) (runtime keep-range), FP64",
-            "N": N, "complex_systems": N_COMPLEX, "real_systems": nreal, "jw_type": 14,
-            "lanes": 64 if fast else geo.lanes,
-            "rows_per_lane": geo.fast_rows_per_lane if fast else geo.rows_per_lane,
-            "waves_per_workgroup": geo.fast_waves if fast else geo.waves,
-            "store_waves_per_workgroup": geo.fast_store_waves if fast else 0,
+            "workload": "JW step y=M^-1 T^-1 (V x), N=512, 65536 complex systems: complex strided input read "
+                        "through block type 4 with the ScaleC operator (scaleC=0.75) fused into the step, "
+                        "inputZeropad=[480,512) (runtime), output the packed XXII real systems; KL=KU=2, FP64",
+            "N": N, "complex_systems": N_COMPLEX, "real_systems": nreal, "jw_type": 14, "inputBlock": 104,
+            "lanes": 64,
+            "rows_per_lane": geo.fast_rows_per_lane,
+            "waves_per_workgroup": geo.fast_waves,
+            "store_waves_per_workgroup": geo.fast_store_waves,
         },
         "p50_ms": float(np.percentile(steps_ms, 50)),
         "p99_ms": float(np.percentile(steps_ms, 99)),
@@ -552,6 +574,7 @@ def main() -> int:
             "algorithmic_bytes": alg,
             "kernel_ms": kavg,
         },
+        "in_place_real": in_place,
     }
     if not args.no_batch:
         result["batch"] = bench_batch(torch, pf, max(10, K), max(3, args.warmup))

@@ -73,7 +73,7 @@ typedef struct jw_run_desc {
     int32_t vec2;             /* FP64: 16-byte aligned pairs may use 128-bit access */
     int32_t mem_f64;          /* DD compute with FP64 storage */
     /* fused block operators (pair = re/im of one packed complex system) */
-    int32_t fused_io;         /* 1: read in_c, write out_c; rhs unused */
+    int32_t fused_io;         /* 1: read in_c, write out_c; 2: read in_c, write rhs (real systems) */
     int32_t in_op, out_op, out_acc;
     int32_t in_lshift, out_lshift;
     double in_scale, out_scale;

@@ -717,12 +717,14 @@ __device__ __forceinline__ void unpack_pair(const double2 (&pf)[2][R / 2], int l
     read_edges<R>(lane, g, sys0, x0, x1);
 }
 
-// Fused block operators (compute / store split kernel, IOMODE 1): the pair of a
-// pass is the re/im of complex system q of a packed (XIXI) array.
-// Input (block layout 4): each row scaled by the input operator, rows >= N zero;
-// output (block layout 7): each row scaled by the output operator, then added to
-// the packed output when accumulating.  The arithmetic per element is the block
-// kernel's (jw_block_kernel), so the fused step equals the three-plan chain.
+// Fused block operators (compute / store split kernel, IOMODE 1 and 2): the pair
+// of a pass is the re/im of complex system q of an interleaved (re, im) array.
+// Input (block layout 4): rows of the input keep-range, each scaled by the input
+// operator, other rows zero; output (IOMODE 1, block layout 7): rows of the
+// output keep-range, each scaled by the output operator, then added to the
+// complex output when accumulating; IOMODE 2 writes the two real systems (the
+// layout-4 packed form) like the plain step.  The arithmetic per element is the
+// block kernel's (jw_block_kernel), so the fused step equals the chain of plans.
 template <int R>
 __device__ __forceinline__ void load_pair_packed(double2 (&pf)[2][R / 2], int lane, const jw_run_desc& d, int64_t q)
 {
@@ -733,8 +735,8 @@ __device__ __forceinline__ void load_pair_packed(double2 (&pf)[2][R / 2], int la
     for (int m = 0; m < H; m++) {
         const int r0 = lane * R + 2 * m, r1 = r0 + 1;   // the lane's own rows, as in load_pair_direct
         double2 e0 = make_double2(0.0, 0.0), e1 = make_double2(0.0, 0.0);
-        if (!zero_in && r0 < d.N) e0 = src[r0];
-        if (!zero_in && r1 < d.N) e1 = src[r1];
+        if (!zero_in && r0 >= d.in_lo && r0 < d.in_hi) e0 = src[r0];
+        if (!zero_in && r1 >= d.in_lo && r1 < d.in_hi) e1 = src[r1];
         pf[0][m] = make_double2(e0.x, e1.x);
         pf[1][m] = make_double2(e0.y, e1.y);
     }
@@ -781,7 +783,7 @@ __device__ __forceinline__ void store_pair_packed(const double2* stg, int lane, 
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const int r = 64 * k + lane;
-        if (r >= d.N) continue;
+        if (r < d.out_lo || r >= d.out_hi) continue;
         double a = re[k], b = im[k];
         if (zero_out) {
             a = 0.0;
@@ -863,7 +865,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
     // build, so its latency overlaps the build
     double2 pf[2][R / 2];
     if (wave < ncomp && q < npass) {
-        if (IOMODE) {
+        if (IOMODE != 0) {
             load_pair_packed<R>(pf, lane, d, q);
         } else {
             const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
@@ -891,8 +893,8 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
         double2* stg = out_base + (size_t)wave * 64 * R;
         // fused block input: whole systems, zero-filled past N by the packed loads, so
         // the keep-range is [0, N rounded up to even) and read_pair has no edge rows
-        const int in_lo = IOMODE ? 0 : (int)d.in_lo;
-        const int in_hi = IOMODE ? (int)(d.N + (d.N & 1)) : (int)d.in_hi;
+        const int in_lo = IOMODE != 0 ? 0 : (int)d.in_lo;
+        const int in_hi = IOMODE != 0 ? (int)(d.N + (d.N & 1)) : (int)d.in_hi;
         while (q < npass) {
             const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol, in_lo,
                                             in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku, d.fwd, d.b1, d.b2,
@@ -900,10 +902,10 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
             // the rows prefetched a whole pass ago, then the next pair's loads at once
             double x0[R], x1[R];
             unpack_pair<R>(pf, lane, g, 2 * q, x0, x1);
-            if (IOMODE) scale_input<R>(x0, x1, lane, d);
+            if (IOMODE != 0) scale_input<R>(x0, x1, lane, d);
             const int64_t qn = q + step;
             if (qn < npass) {
-                if (IOMODE) load_pair_packed<R>(pf, lane, d, qn);
+                if (IOMODE != 0) load_pair_packed<R>(pf, lane, d, qn);
                 else load_pair_direct<R>(pf, lane, g, 2 * qn);
             }
             compute_pair<R, ALL3>(tab, lane, g, x0, x1);
@@ -939,7 +941,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
                 const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
                                                 (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl,
                                                 d.ku, d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
-                if (IOMODE) store_pair_packed<R>(out_base + (size_t)w * 64 * R, lane, d, q);
+                if (IOMODE == 1) store_pair_packed<R>(out_base + (size_t)w * 64 * R, lane, d, q);
                 else store_pair<R>(out_base + (size_t)w * 64 * R, lane, g, 2 * q);
                 __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the staging has been read
                 compiler_fence();
@@ -1091,16 +1093,21 @@ const void* pick_batch64(int R)
 }
 
 template <int R>
-const void* split_ptr(bool all3, bool io)
+const void* split_ptr(bool all3, int io)
 {
-    if (io)
+    if (io == 1)
         return all3 ? reinterpret_cast<const void*>(&jw_main64s_kernel<R, true, 1>)
                     : reinterpret_cast<const void*>(&jw_main64s_kernel<R, false, 1>);
+    if (io == 2)
+        return all3 ? reinterpret_cast<const void*>(&jw_main64s_kernel<R, true, 2>)
+                    : reinterpret_cast<const void*>(&jw_main64s_kernel<R, false, 2>);
     return all3 ? reinterpret_cast<const void*>(&jw_main64s_kernel<R, true, 0>)
                 : reinterpret_cast<const void*>(&jw_main64s_kernel<R, false, 0>);
 }
 
-const void* pick_split(int R, bool all3, bool io = false)
+// io: 0 plain step, 1 complex in and out (block layouts 4 and 7), 2 complex in
+// (layout 4), real systems out
+const void* pick_split(int R, bool all3, int io = 0)
 {
     switch (R) {
     case 2: return split_ptr<2>(all3, io);
@@ -1246,8 +1253,8 @@ int jwdev_main64s_occupancy(int R)
     int c = 0, st = 0;
     if (!jwdev_main64s_shape(R, &c, &st)) return 0;
     int best = -1;
-    for (int v = 0; v < 4; v++) {
-        const void* f = pick_split(R, (v & 1) == 0, v >= 2);
+    for (int v = 0; v < 6; v++) {
+        const void* f = pick_split(R, (v & 1) == 0, v >> 1);
         const int lds = (int)jwdev_main64s_lds_bytes(R, c);
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
         int nb = 0;
@@ -1262,7 +1269,7 @@ int jwdev_main64s(const jw_run_desc* r, const jw_source_desc* src, const void* m
 {
     int c = 0, st = 0;
     if (!jwdev_main64s_shape(R, &c, &st)) { snprintf(g_err64, sizeof(g_err64), "no split kernel for R=%d", R); return -1; }
-    const void* f = pick_split(R, r->fwd && r->b1 && r->b2, r->fused_io != 0);
+    const void* f = pick_split(R, r->fwd && r->b1 && r->b2, (int)r->fused_io);
     const size_t lds = (size_t)jwdev_main64s_lds_bytes(R, c);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -2;

@@ -68,12 +68,13 @@ PfSolveResult jw_configure(const PfSolveConfiguration* in, PfSolveConfiguration*
         }
         if (prec == JW_PREC_F64 && N > JW_MAX_N_F64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         if (prec != JW_PREC_F64 && N > JW_MAX_N_DD) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
-        /* block operators fused around the step: layout 4 (complex in, no accumulate)
-         * and layout 7 (complex out, optional accumulate), both present */
+        /* block operators fused around the step: layout 4 on the input (complex in, no
+         * accumulate), and on the output layout 7 (complex out, optional accumulate)
+         * or none (the step's real systems) */
         if (out->inputBlock || out->outputBlock) {
             const int64_t ib = out->inputBlock, ob = out->outputBlock;
-            if (ib < 0 || ob < 0 || ib % 10 != 4 || ob % 10 != 7 || (ib / 10) % 10 != 0 ||
-                ((ob / 10) % 10 != 0 && (ob / 10) % 10 != 2) || ib / 100 > 4 || ob / 100 > 4)
+            if (ib <= 0 || ob < 0 || ib % 10 != 4 || (ib / 10) % 10 != 0 || ib / 100 > 4 ||
+                (ob && (ob % 10 != 7 || ((ob / 10) % 10 != 0 && (ob / 10) % 10 != 2) || ob / 100 > 4)))
                 return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
             if (prec != JW_PREC_F64 || N > JW_MAX_N_FUSED_BLOCKS) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         }

@@ -162,11 +162,12 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
     return PFSOLVE_SUCCESS;
 }
 
-/* JW step with fused block operators: packed complex systems in (block layout 4,
- * PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_ReadWrite.h:2275-2371) and out
- * (layout 7, optional accumulate), each row scaled by the block's operator
- * (pfSolve_RegistersScaleC.h:27-201); the re/im halves of complex system q are the
- * step's pair of real systems 2q, 2q+1.  Runs on the compute / store split kernel. */
+/* JW step with fused block operators: complex systems in (block layout 4,
+ * PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_ReadWrite.h:2275-2371) and either complex
+ * out (layout 7, optional accumulate) or the step's real systems out, each row scaled by the
+ * block's operator (pfSolve_RegistersScaleC.h:27-201); the re/im halves of complex system q are
+ * the step's pair of real systems 2q, 2q+1.  The input keep-range selects the rows read, the
+ * output keep-range the rows written.  Runs on the compute / store split kernel. */
 static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveLaunchParams* lp,
                                          const jw_source_desc* src, jw_run_desc* run, int64_t nsys, hipStream_t s)
 {
@@ -175,22 +176,19 @@ static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveL
     /* block codes, precision and N were checked when the plan was configured */
     const int64_t ib = cfg->inputBlock, ob = cfg->outputBlock;
     if (p->precision != JW_PREC_F64 || !p->split64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
-    /* whole systems only: the keep-ranges of the step must be full */
-    if (run->in_lo != 0 || run->in_hi != p->N || run->out_lo != 0 || run->out_hi != p->N)
-        return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
     if (nsys % 2) return PFSOLVE_ERROR_EMPTY_size;
     if (!cfg->inputBuffer || !cfg->inputBuffer[0] || cfg->inputBuffer == cfg->buffer)
         return PFSOLVE_ERROR_EMPTY_inputBuffer;
     if (!lp) lp = &(const PfSolveLaunchParams){0};
     const int64_t m = cfg->jw_control_bitmask;
     const int64_t ncx = nsys / 2;
-    run->fused_io = 1;
+    run->fused_io = ob ? 1 : 2;
     run->in_c = cfg->inputBuffer[0];
     run->out_c = cfg->outputBuffer[0];
     run->rhs = cfg->outputBuffer[0];
     run->in_stride_c = (m & RUNTIME_INPUTBUFFERSTRIDE) ? (int64_t)lp->inputBufferStride : (int64_t)cfg->inputBufferStride[0];
-    run->out_stride_c = run->stride;
     run->in_off_c = cfg->inputBlockOffset;
+    run->out_stride_c = run->stride;
     run->out_off_c = run->offSol;
     run->in_op = (int32_t)(ib / 100);
     run->out_op = (int32_t)(ob / 100);
@@ -200,16 +198,21 @@ static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveL
     run->in_scale = cfg->inputBlockScale;
     run->out_scale = cfg->outputBlockScale;
     run->vec2 = 1;
-    if (((uintptr_t)run->in_c % 16) || ((uintptr_t)run->out_c % 16)) return PFSOLVE_ERROR_EMPTY_outputBufferSize;
-    if (run->in_stride_c < 0 || run->out_stride_c < 0 || run->in_off_c < 0 || run->out_off_c < 0 ||
-        (ncx > 1 && (run->in_stride_c < p->N || run->out_stride_c < p->N)))
+    /* complex input: 16-byte elements; real output (mode 2): 16-byte aligned pairs of rows */
+    if ((uintptr_t)run->in_c % 16) return PFSOLVE_ERROR_EMPTY_inputBufferSize;
+    if (run->fused_io == 1 ? ((uintptr_t)run->out_c % 16) != 0
+                           : (((uintptr_t)run->rhs % 16) || (run->stride % 2) || (run->offSol % 2)))
         return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+    if (run->in_stride_c < 0 || run->in_off_c < 0 || (ncx > 1 && run->in_stride_c < p->N))
+        return PFSOLVE_ERROR_EMPTY_inputBufferSize;
+    if (run->stride < 0 || run->offSol < 0 || (nsys > 1 && run->stride < p->N)) return PFSOLVE_ERROR_EMPTY_outputBufferSize;
     if (ncx > 0) {
         const int64_t imax = (ncx - 1) * run->in_stride_c + run->in_off_c + p->N;   /* complex elements */
-        const int64_t omax = (ncx - 1) * run->out_stride_c + run->out_off_c + p->N;
         if (cfg->inputBufferSize && (uint64_t)(imax * 16) > cfg->inputBufferSize[0]) return PFSOLVE_ERROR_EMPTY_inputBufferSize;
-        if (cfg->outputBufferSize && (uint64_t)(omax * 16) > cfg->outputBufferSize[0])
-            return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+        /* bytes up to the last output element: complex elements (mode 1) or doubles (mode 2) */
+        const int64_t oend = run->fused_io == 1 ? ((ncx - 1) * run->stride + run->offSol + p->N) * 16
+                                                : ((nsys - 1) * run->stride + run->offSol + p->N) * 8;
+        if (cfg->outputBufferSize && (uint64_t)oend > cfg->outputBufferSize[0]) return PFSOLVE_ERROR_EMPTY_outputBufferSize;
     }
     {
         int64_t mmax, rmax;

@@ -122,14 +122,16 @@ def test_validation_error_codes():
 
 def test_fused_block_codes_validation():
     """block operators fused around a JW step: layout 4 in (no accumulate), layout 7
-    out (accumulate 0 or 2), operators 0-4, FP64, N <= 512, only on a JW plan"""
+    out (accumulate 0 or 2) or no output block, operators 0-4, FP64, N <= 512, only
+    on a JW plan"""
     ok = dict(jw_type=14, inputBlock=204, outputBlock=327)
     assert validate(cfg(**ok))[0] == 0
     for ib in (4, 104, 204, 304, 404):
-        for ob in (7, 27, 107, 227, 427):
+        for ob in (0, 7, 27, 107, 227, 427):
             assert validate(cfg(jw_type=14, inputBlock=ib, outputBlock=ob))[0] == 0, (ib, ob)
     assert validate(cfg(N=512, **ok))[0] == 0
-    bad_codes = [(204, 0), (0, 327), (207, 327), (224, 327), (204, 317), (204, 324), (504, 7), (4, 507), (-4, 7)]
+    bad_codes = [(0, 327), (207, 327), (224, 327), (204, 317), (204, 324), (504, 7), (4, 507), (-4, 7), (4, -7),
+                 (204, 4)]
     for ib, ob in bad_codes:
         assert validate(cfg(jw_type=14, inputBlock=ib, outputBlock=ob))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX, \
             (ib, ob)
