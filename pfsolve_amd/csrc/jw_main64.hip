@@ -39,6 +39,7 @@ __device__ __forceinline__ int swz(int c)
 }
 
 __device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+__host__ __device__ __forceinline__ int64_t max_i64(int64_t a, int64_t b) { return a > b ? a : b; }
 
 // store of a result chunk: written once and not read back by the step, so it is
 // marked nontemporal (streamed to memory without allocating in the caches)
@@ -744,24 +745,32 @@ __device__ __forceinline__ void load_pair_packed(double2 (&pf)[2][R / 2], int la
 
 // the input operator on a lane's rows, once they are in registers (rows >= N are zero)
 template <int R>
-__device__ __forceinline__ void scale_input(double (&x0)[R], double (&x1)[R], int lane, const jw_run_desc& d)
+__device__ __forceinline__ void scale_input(double (&x0)[R], double (&x1)[R], int lane, const jw_run_desc& d,
+                                            const double* fin)
 {
     if (d.in_op < 1 || d.in_op > 4) return;
-    // recomputed every pass: hoisting the R factors out of the pass loop would hold
-    // them in registers for the whole kernel
-    asm volatile("" : "+v"(lane));
 #pragma unroll
     for (int i = 0; i < R; i++) {
-        // rows >= N stay exact zeros (a factor there may be NaN for some scales)
-        const int r = lane * R + i;
-        const double f = r < d.N ? jw_block_scale(d.in_op, r, d.in_lshift, d.in_scale) : 1.0;
+        const double f = fin[lane * R + i];
         x0[i] *= f;
         x1[i] *= f;
     }
 }
 
+// per-row factors of the block operators, once per workgroup into LDS (rows >= N
+// get 1, so their zeros stay exact zeros even where a factor would be NaN)
+__device__ __forceinline__ void block_factors(double* fin, double* fout, int rows, const jw_run_desc& d)
+{
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        const bool in = r < d.N;
+        fin[r] = in && d.in_op >= 1 && d.in_op <= 4 ? jw_block_scale(d.in_op, r, d.in_lshift, d.in_scale) : 1.0;
+        fout[r] = in && d.out_op >= 1 && d.out_op <= 4 ? jw_block_scale(d.out_op, r, d.out_lshift, d.out_scale) : 1.0;
+    }
+}
+
 template <int R>
-__device__ __forceinline__ void store_pair_packed(const double2* stg, int lane, const jw_run_desc& d, int64_t q)
+__device__ __forceinline__ void store_pair_packed(const double2* stg, int lane, const jw_run_desc& d, int64_t q,
+                                                  const double* fout)
 {
     constexpr int H = R / 2;
     constexpr int SYSC = 64 * H;
@@ -769,17 +778,26 @@ __device__ __forceinline__ void store_pair_packed(const double2* stg, int lane, 
     const double* s0 = reinterpret_cast<const double*>(stg);
     const double* s1 = reinterpret_cast<const double*>(stg + SYSC);
     const bool zero_out = d.out_op == 1 && d.out_scale == 0.0;
-    asm volatile("" : "+v"(lane));   // factors recomputed per pair, not held across the loop
-    double re[R], im[R];
+    const bool scaled = d.out_op >= 1 && d.out_op <= 4;
+    // the previous output first (all loads in flight together), then the staging
+    double2 prev[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int r = 64 * k + lane;
+        prev[k] = make_double2(0.0, 0.0);
+        if (d.out_acc && r >= d.out_lo && r < d.out_hi) prev[k] = dst[r];
+    }
+    double re[R], im[R], f[R];
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const int r = 64 * k + lane;   // complex row r: chunk r / 2, half r % 2, of both systems
         const int e = 2 * swz<H>(r >> 1) + (r & 1);
         re[k] = s0[e];
         im[k] = s1[e];
+        f[k] = scaled ? fout[r] : 1.0;
     }
 #pragma unroll
-    for (int k = 0; k < R; k++) asm volatile("" : "+v"(re[k]), "+v"(im[k]));
+    for (int k = 0; k < R; k++) asm volatile("" : "+v"(re[k]), "+v"(im[k]), "+v"(f[k]));
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const int r = 64 * k + lane;
@@ -788,15 +806,13 @@ __device__ __forceinline__ void store_pair_packed(const double2* stg, int lane, 
         if (zero_out) {
             a = 0.0;
             b = 0.0;
-        } else if (d.out_op >= 1 && d.out_op <= 4) {
-            const double f = jw_block_scale(d.out_op, r, d.out_lshift, d.out_scale);
-            a *= f;
-            b *= f;
+        } else if (scaled) {
+            a *= f[k];
+            b *= f[k];
         }
         if (d.out_acc) {
-            const double2 prev = dst[r];
-            a += prev.x;
-            b += prev.y;
+            a += prev[k].x;
+            b += prev[k].y;
         }
         store_stream(dst + r, make_double2(a, b));
     }
@@ -878,6 +894,11 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
     // caller's matrices (no separate prologue launch); the staging area, unused
     // until the barrier below, is the build's scratch
     prologue_body<double>(src, mat, reinterpret_cast<double*>(smem), 64, R, 2, smem + NTAB * 8);
+    // block operator factors past the staging and the build's scratch
+    double* fin = reinterpret_cast<double*>(
+        smem + NTAB * 8 + max_i64(2LL * 64 * R * 8 * ncomp, (int64_t)prologue_lds_bytes(64 * R, 8)));
+    double* fout = fin + 64 * R;
+    if (IOMODE != 0) block_factors(fin, fout, 64 * R, d);
     if (wave < ncomp) {
         if (lane == 0) {
             lds_set(&s_full[wave], 0);
@@ -902,12 +923,12 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
             // the rows prefetched a whole pass ago, then the next pair's loads at once
             double x0[R], x1[R];
             unpack_pair<R>(pf, lane, g, 2 * q, x0, x1);
-            if (IOMODE != 0) scale_input<R>(x0, x1, lane, d);
             const int64_t qn = q + step;
             if (qn < npass) {
                 if (IOMODE != 0) load_pair_packed<R>(pf, lane, d, qn);
                 else load_pair_direct<R>(pf, lane, g, 2 * qn);
             }
+            if (IOMODE != 0) scale_input<R>(x0, x1, lane, d, fin);
             compute_pair<R, ALL3>(tab, lane, g, x0, x1);
             if (!wait_flag(vfull + wave, 0)) { ok = false; break; }
             write_blocked<R>(stg, lane, x0, x1);
@@ -941,7 +962,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
                 const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
                                                 (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl,
                                                 d.ku, d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
-                if (IOMODE == 1) store_pair_packed<R>(out_base + (size_t)w * 64 * R, lane, d, q);
+                if (IOMODE == 1) store_pair_packed<R>(out_base + (size_t)w * 64 * R, lane, d, q, fout);
                 else store_pair<R>(out_base + (size_t)w * 64 * R, lane, g, 2 * q);
                 __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the staging has been read
                 compiler_fence();
@@ -1128,7 +1149,7 @@ bool split_shape(int R, int* ncomp, int* nstore)
     const int cmax = R <= 4 ? 12 : (R == 8 ? 8 : 2);
     const int64_t table = jwdev_table64_elems(R) * 8;
     const int64_t per = 2LL * 64 * R * 8;   // output staging of 2 systems
-    const int64_t avail = kLdsBytesMax - 256 - table;
+    const int64_t avail = kLdsBytesMax - 256 - table - 2LL * 64 * R * 8;   // less the block factors
     int c = (int)(avail / per);
     if (c > cmax) c = cmax;
     if (c < 1 || c >= maxw) return false;
@@ -1242,10 +1263,11 @@ int jwdev_main64s_shape(int R, int* ncomp, int* nstore)
 
 int64_t jwdev_main64s_lds_bytes(int R, int ncomp)
 {
-    // output staging of the compute wavefronts, at least the table build's scratch
+    // output staging of the compute wavefronts, at least the table build's scratch,
+    // and the block operators' row factors (in and out) of the fused form
     const int64_t staging = 2LL * 64 * R * 8 * ncomp;
     const int64_t scratch = (int64_t)prologue_lds_bytes(64 * R, 8);
-    return jwdev_table64_elems(R) * 8 + (staging > scratch ? staging : scratch);
+    return jwdev_table64_elems(R) * 8 + max_i64(staging, scratch) + 2LL * 64 * R * 8;
 }
 
 int jwdev_main64s_occupancy(int R)

@@ -59,7 +59,7 @@ constexpr int kPrologueThreads = 512;
 constexpr int kProRows = 4;   // rows per thread: R * L <= 2048
 
 // LDS of prologue_body: 5 arrays of R*L elements (ld -> l -> l', ud -> h -> h', D, E, g)
-static inline size_t prologue_lds_bytes(int RL, size_t elt) { return 5 * (size_t)RL * elt; }
+__host__ __device__ inline size_t prologue_lds_bytes(int RL, size_t elt) { return 5 * (size_t)RL * elt; }
 
 // the band coefficients v_{-2..2} of row r of the matvec (zero outside the
 // matrix), P = storage position of row r in the permuted diagonals
