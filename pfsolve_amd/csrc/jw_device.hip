@@ -42,9 +42,11 @@ __global__ __launch_bounds__(kPrologueThreads) void jw_prologue_kernel(jw_source
     prologue_body<T>(d, mat, table, L, R, format, pro_smem);
 }
 
-// batched: one workgroup per matrix set; workgroup 0 also clears the work-queue counters
-template <typename T>
-__global__ __launch_bounds__(kPrologueThreads) void jw_prologue_batch_kernel(const jw_prologue_job* jobs, int L_unused,
+// batched: one workgroup per matrix set; workgroup 0 also clears the work-queue counters.
+// NT = 256 when every set has at most 1024 rows: four such workgroups share a CU
+// (LDS and registers), so 1024 sets need one round instead of two.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void jw_prologue_batch_kernel(const jw_prologue_job* jobs, int L_unused,
                                                                              int* counters, int ncounters)
 {
     extern __shared__ __align__(16) unsigned char pro_smem[];
@@ -517,18 +519,24 @@ int jwdev_prologue_batch(const jw_prologue_job* d_jobs, int njobs, int max_RL, i
         return -1;
     }
     if (njobs < 1) return 0;
-    if (ncounters > kPrologueThreads) { snprintf(g_err, sizeof(g_err), "too many counters"); return -1; }
+    const bool narrow = max_RL <= kProRows * 256;
+    const int nt = narrow ? 256 : kPrologueThreads;
+    if (ncounters > nt) { snprintf(g_err, sizeof(g_err), "too many counters"); return -1; }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const void* fn = precision == JW_PREC_F64 ? reinterpret_cast<const void*>(&jw_prologue_batch_kernel<double>)
-                                              : reinterpret_cast<const void*>(&jw_prologue_batch_kernel<dd>);
+    const void* fn;
+    if (precision == JW_PREC_F64)
+        fn = narrow ? reinterpret_cast<const void*>(&jw_prologue_batch_kernel<double, 256>)
+                    : reinterpret_cast<const void*>(&jw_prologue_batch_kernel<double, kPrologueThreads>);
+    else
+        fn = narrow ? reinterpret_cast<const void*>(&jw_prologue_batch_kernel<dd, 256>)
+                    : reinterpret_cast<const void*>(&jw_prologue_batch_kernel<dd, kPrologueThreads>);
     hipError_t ea = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (ea != hipSuccess) { set_err("hipFuncSetAttribute(prologue batch)", ea); return -2; }
-    if (precision == JW_PREC_F64)
-        hipLaunchKernelGGL(jw_prologue_batch_kernel<double>, dim3(njobs), dim3(kPrologueThreads), lds, s, d_jobs, 0,
-                           d_counters, ncounters);
-    else
-        hipLaunchKernelGGL(jw_prologue_batch_kernel<dd>, dim3(njobs), dim3(kPrologueThreads), lds, s, d_jobs, 0,
-                           d_counters, ncounters);
+    const jw_prologue_job* jobs = d_jobs;
+    int zero = 0;
+    void* args[] = {const_cast<jw_prologue_job**>(&jobs), &zero, &d_counters, &ncounters};
+    hipError_t el = hipLaunchKernel(fn, dim3(njobs), dim3(nt), args, lds, s);
+    if (el != hipSuccess) { set_err("prologue batch launch", el); return -1; }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { set_err("prologue batch launch", e); return -1; }
     return 0;
