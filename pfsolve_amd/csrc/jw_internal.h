@@ -1,12 +1,14 @@
 /*
  * jw_internal.h -- descriptors shared by the C host code (jw_plan.c,
- * pfsolve_api.c) and the gfx950 device code (jw_device.hip).  Plain C.
+ * pfsolve_api.c, jw_batch.c, jw_serve.c) and the gfx950 device code
+ * (jw_device.hip, jw_main64.hip).  Plain C.
  *
  * Data model of the JW step on the device
  * ---------------------------------------
  * Every supported operator is y = B2^{-1} B1^{-1} F^{-1} (V x), row 0 scaled,
  * where V is banded (|offset| <= 2) and F (unit lower bidiagonal), B1, B2
- * (upper bidiagonal) are the factors of the solve matrices.  A prologue kernel
+ * (upper bidiagonal) are the factors of the solve matrices.  A table build
+ * (a prologue kernel, or each workgroup of the compute / store split kernel)
  * turns the caller's matrices (in the reference storage, any mode) into one
  * coefficient table per matrix set:
  *   per row r:  vm2 vm1 v0 vp1 vp2 | l | g h | D E

@@ -1,14 +1,14 @@
-// jw_main64.hip -- wave-per-system FP64 JW kernel for gfx950 (table format 2).
+// jw_main64.hip -- wave-per-system FP64 JW kernels for gfx950 (table format 2).
 //
 // One wavefront owns a pair of systems (the re/im halves of a complex system
 // in XXII layout, or two consecutive real systems); lane t owns rows
 // t*R .. t*R+R-1 of both.  Per pair:
-//   1. input: the 16-byte chunks (row pairs) inside the input keep-range are
-//      DMA'd global -> LDS (global_load_lds_dwordx4), 64 lanes per
-//      instruction on one contiguous KiB, each lane fetching the chunk whose
-//      XOR-swizzled slot it lands in, so the blocked read-back (lane t reads its
-//      R rows) is bank-conflict free; an odd range end is read by its owning
-//      lane alone;
+//   1. input: the 16-byte chunks (row pairs) inside the input keep-range; the
+//      compute / store split kernel loads a lane's own chunks straight into
+//      registers one pass ahead, the pair and batch kernels DMA them global ->
+//      LDS (global_load_lds_dwordx4, 64 lanes per instruction on one contiguous
+//      KiB, XOR-swizzled slots so the blocked read-back is bank-conflict free);
+//      an odd range end is read by its owning lane alone;
 //   2. banded matvec u' = a.(V x) with neighbour rows by DPP wave shifts;
 //   3. three first-order recurrences (forward W, backward Z, backward y),
 //      each: lane-local sweep, affine scan across the wavefront, lane-local
@@ -16,11 +16,12 @@
 //      rows DPP row_bcast15/row_bcast31 (forward) or readlane of the row
 //      starts (backward);
 //   4. row-0 scaleC; output written blocked to LDS, read back contiguous and
-//      stored with 16-byte stores (rows outside the output keep-range untouched;
-//      an odd range end is written by its owning lane alone).
+//      stored with 16-byte nontemporal stores (rows outside the output
+//      keep-range untouched; an odd range end is written by its owning lane).
 // Coefficients (8 per row, scalings folded) and scan multipliers come from the
-// per-matrix-set table written by jw_prologue_kernel (format 2), staged once
-// per workgroup in LDS.
+// per-matrix-set table (format 2): built by each workgroup of the split kernel
+// in its LDS, written by jw_prologue_kernel / jw_prologue_batch_kernel for the
+// other kernels and staged once per workgroup (or per work unit) in LDS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
