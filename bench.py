@@ -95,6 +95,28 @@ def time_plan_steps(torch, lib, plan, stream, warmup: int, k: int, step):
     return ms, per, wall, kavg
 
 
+def time_graph_replays(torch, stream, warmup: int, k: int, step) -> float:
+    """The step captured once in a hipGraph on `stream` (torch's graph API over the
+    same plan; the plan launches on `stream`), then W untimed and K timed replays
+    between two events: ms per replay."""
+    graph = torch.cuda.CUDAGraph()
+    stream.synchronize()
+    with torch.cuda.graph(graph, stream=stream, capture_error_mode="relaxed"):
+        step()
+    for _ in range(warmup):
+        graph.replay()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+        for _ in range(k):
+            graph.replay()
+        e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_time(e1) / k
+
+
 def kernel_window_ms(torch, lib, plan, stream, k: int, launch) -> float:
     """Average duration of the plan's main kernel over k back-to-back launches on
     `stream`: the library records one HIP event just before the first kernel and
@@ -454,6 +476,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--warm-seconds", type=float, default=0.5,
+                    help="untimed steps for this long before the first measurement: the first tens of ms of "
+                         "load on a fresh box run measurably slower")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-size batch measurement")
     ap.add_argument("--no-dd", action="store_true", help="skip the double-double measurement")
     ap.add_argument("--no-fused-blocks", action="store_true", help="skip the fused block I/O measurement")
@@ -504,9 +529,15 @@ def main() -> int:
     launch = dict(inputZeropad=list(ZP), inputBuffer=xin.data_ptr())
 
     K = args.steps
+    t_warm = time.perf_counter()
+    while time.perf_counter() - t_warm < args.warm_seconds:
+        for _ in range(20):
+            plan.append(**launch)
+        stream.synchronize()
     ms_step, steps_ms, wall, kavg = time_plan_steps(torch, lib, plan, stream, args.warmup, K,
                                                     lambda: plan.append(**launch))
     assert torch.isfinite(y).all().item(), "non-finite output"
+    graph_ms = time_graph_replays(torch, stream, args.warmup, K, lambda: plan.append(**launch))
     plan.close()
     rows_read = ZP[1] - ZP[0]
     alg = algorithmic_bytes(N, nreal, rows_read, N)
@@ -562,6 +593,8 @@ def main() -> int:
         },
         "p50_ms": float(np.percentile(steps_ms, 50)),
         "p99_ms": float(np.percentile(steps_ms, 99)),
+        "graph_ms_per_step": graph_ms,
+        "device_warm_s": args.warm_seconds,
         "wall_s": wall,
         "roofline": {
             "bound": "hbm",
