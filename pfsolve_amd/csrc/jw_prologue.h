@@ -159,78 +159,102 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     }
     __syncthreads();
 
-    // phase 2: factors of the unit-diagonal solve, lane t owns rows t*R .. t*R+R-1
+    // phase 2: factors of the unit-diagonal solve.  Pivots p_r = 1 - m_r / p_{r-1},
+    // m_r = ld_r ud_{r-1}: the Moebius map p -> (p - m)/p is the 2x2 matrix
+    // [[1,-m],[1,0]] on (num, den).  Thread t < NP composes the maps of its RT
+    // consecutive rows; an inclusive scan over those threads (Hillis-Steele in
+    // each wavefront, then the wavefront totals through LDS) gives every thread
+    // the map of all rows before its own, applied to (1,1) the pivot of the row
+    // before its first; each thread then runs the exact recurrence over its rows.
+    // NP is fixed, so the rounding does not depend on the workgroup size (every
+    // kernel that builds a table gives the same table, bit for bit).
     if (d.solve_kind == JW_SOLVE_TRI) {
-        if (tid < 64) {
-            const int t = tid;
-            const bool active = t < L;
-            const int log2L = ilog2_rt(L);
-            // pivots p_r = 1 - m_r / p_{r-1}, m_r = ld_r ud_{r-1}: the Moebius map
-            // p -> (p - m)/p is the 2x2 matrix [[1,-m],[1,0]] on (num, den).
-            T a = A::one(), b = A::zero(), c = A::zero(), e = A::one();
-            if (active) {
-                for (int i = 0; i < R; i++) {
-                    const int r = t * R + i;
-                    T m = (r >= 1 && r < N) ? A::mul(sl[r], sh[r - 1]) : A::zero();
-                    T na = A::msub(a, m, c), nb = A::msub(b, m, e);
-                    c = a;
-                    e = b;
-                    a = na;
-                    b = nb;
-                    double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
-                    if (mx > 0.0) {
-                        int ex;
-                        frexp(mx, &ex);
-                        a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
-                    }
-                }
+        constexpr int NP = 256;      // participating threads (workgroups have >= 256)
+        __shared__ T s_tot[NP / 64][4];
+        const int RT = (RL + NP - 1) / NP;
+        const int r0 = tid < NP ? tid * RT : RL;   // threads >= NP own no rows
+        const int lane = tid & 63, wv = tid >> 6;
+        // the entries are rescaled by a power of two after every product (exact)
+        auto normalize = [](T& a, T& b, T& c, T& e) {
+            double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
+            if (mx > 0.0) {
+                int ex;
+                frexp(mx, &ex);
+                a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
             }
-            // inclusive scan over lanes: X_t <- X_t * X_{t-s}
-            for (int k = 0; k < log2L; k++) {
-                const int s = 1 << k;
-                const int src = (t >= s) ? t - s : t;
-                T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
-                if (active && t >= s) {
-                    T na = A::madd(A::mul(a, pa), b, pc);
-                    T nb = A::madd(A::mul(a, pb), b, pe);
-                    T nc = A::madd(A::mul(c, pa), e, pc);
-                    T ne = A::madd(A::mul(c, pb), e, pe);
-                    a = na; b = nb; c = nc; e = ne;
-                    double mx = fmax(fmax(fabs(A::hi(a)), fabs(A::hi(b))), fmax(fabs(A::hi(c)), fabs(A::hi(e))));
-                    if (mx > 0.0) {
-                        int ex;
-                        frexp(mx, &ex);
-                        a = A::ldexp_(a, -ex); b = A::ldexp_(b, -ex); c = A::ldexp_(c, -ex); e = A::ldexp_(e, -ex);
-                    }
-                }
-            }
-            // exclusive: composition of the maps of lanes 0..t-1 applied to (1,1)
-            const int src = (t >= 1) ? t - 1 : t;
+        };
+        // X <- X * P (P: the maps of earlier rows, applied first)
+        auto compose = [&](T& a, T& b, T& c, T& e, T pa, T pb, T pc, T pe) {
+            T na = A::madd(A::mul(a, pa), b, pc);
+            T nb = A::madd(A::mul(a, pb), b, pe);
+            T nc = A::madd(A::mul(c, pa), e, pc);
+            T ne = A::madd(A::mul(c, pb), e, pe);
+            a = na; b = nb; c = nc; e = ne;
+            normalize(a, b, c, e);
+        };
+        T a = A::one(), b = A::zero(), c = A::zero(), e = A::one();
+        for (int i = 0; i < RT; i++) {
+            const int r = r0 + i;
+            if (r >= RL) break;
+            T m = (r >= 1 && r < N) ? A::mul(sl[r], sh[r - 1]) : A::zero();
+            T na = A::msub(a, m, c), nb = A::msub(b, m, e);
+            c = a;
+            e = b;
+            a = na;
+            b = nb;
+            normalize(a, b, c, e);
+        }
+        // ud of the row before this thread's first, read before any thread
+        // overwrites its ud with h (after the barrier below)
+        T udprev = (r0 >= 1 && r0 < RL) ? sh[r0 - 1] : A::zero();
+        // inclusive scan inside the wavefront
+        for (int k = 0; k < 6; k++) {
+            const int sft = 1 << k;
+            const int src = lane >= sft ? lane - sft : lane;
             T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
-            // g_{r-1} = 1 / p_{r-1} of the row before this lane's first
-            T gprev = A::one();
-            if (t >= 1) gprev = A::div(A::add(pc, pe), A::add(pa, pb));
-            // ud of that row is read before any lane overwrites its ud with h
-            T udprev = (active && t >= 1) ? sh[t * R - 1] : A::zero();
-            if (active) {
-                for (int i = 0; i < R; i++) {
-                    const int r = t * R + i;
-                    const T ldr = sl[r], udr = sh[r];
-                    T l = A::zero(), g = A::zero(), h = A::zero();
-                    if (r < N) {
-                        // l_r = ld_r / p_{r-1},  p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r,  h_r = ud_r g_r
-                        l = (r >= 1) ? A::mul(ldr, gprev) : A::zero();
-                        const T p = A::sub(A::one(), A::mul(l, udprev));
-                        g = A::div(A::one(), p);
-                        h = A::mul(udr, g);
-                        gprev = g;
-                    }
-                    udprev = udr;
-                    sl[r] = l;
-                    sg[r] = g;
-                    sh[r] = h;
-                }
+            if (lane >= sft) compose(a, b, c, e, pa, pb, pc, pe);
+        }
+        if (lane == 63 && tid < NP) {
+            s_tot[wv][0] = a;
+            s_tot[wv][1] = b;
+            s_tot[wv][2] = c;
+            s_tot[wv][3] = e;
+        }
+        // this thread's exclusive map within the wavefront (lane - 1's inclusive)
+        const int src = lane >= 1 ? lane - 1 : lane;
+        T xa = lane_fetch(a, src), xb = lane_fetch(b, src), xc = lane_fetch(c, src), xe = lane_fetch(e, src);
+        if (lane == 0) {
+            xa = A::one(); xb = A::zero(); xc = A::zero(); xe = A::one();
+        }
+        __syncthreads();
+        // the maps of all earlier wavefronts, then the in-wavefront part on top
+        T wa = A::one(), wb = A::zero(), wc = A::zero(), we = A::one();
+        for (int w = 0; w < wv && w < NP / 64; w++) {
+            T ta = s_tot[w][0], tb = s_tot[w][1], tc = s_tot[w][2], te = s_tot[w][3];
+            compose(ta, tb, tc, te, wa, wb, wc, we);
+            wa = ta; wb = tb; wc = tc; we = te;
+        }
+        compose(xa, xb, xc, xe, wa, wb, wc, we);
+        // g_{r-1} = 1 / p_{r-1} of the row before this thread's first
+        T gprev = A::one();
+        if (r0 >= 1) gprev = A::div(A::add(xc, xe), A::add(xa, xb));
+        for (int i = 0; i < RT; i++) {
+            const int r = r0 + i;
+            if (r >= RL) break;
+            const T ldr = sl[r], udr = sh[r];
+            T l = A::zero(), g = A::zero(), h = A::zero();
+            if (r < N) {
+                // l_r = ld_r / p_{r-1},  p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r,  h_r = ud_r g_r
+                l = (r >= 1) ? A::mul(ldr, gprev) : A::zero();
+                const T p = A::sub(A::one(), A::mul(l, udprev));
+                g = A::div(A::one(), p);
+                h = A::mul(udr, g);
+                gprev = g;
             }
+            udprev = udr;
+            sl[r] = l;
+            sg[r] = g;
+            sh[r] = h;
         }
     } else {
         for (int r = tid; r < RL; r += nthr) {
