@@ -217,11 +217,17 @@ typedef struct {
     int64_t kernelVariant;       /* 0: automatic, 1: general kernel, 2: wave-per-system FP64 kernel */
     /* Fused block operators around a JW step (block codes as in `block`: layout +
      * 20 for accumulate + 100/200/300/400 for ScaleC/ScaleD/SphLaplA/SphLaplB).
-     * inputBlock (layout 4) reads packed complex systems from inputBuffer and
-     * feeds their re/im halves to the step; outputBlock (layout 7) writes the
-     * result as packed complex systems to outputBuffer.  Both or neither; 0 = none.
-     * Strides count complex elements (input/outputBufferStride); input system 0
-     * starts at inputBlockOffset, output system 0 at offsetSolution. */
+     * inputBlock (layout 4) reads complex (interleaved re, im) systems from
+     * inputBuffer and feeds their re/im halves to the step as a pair of real
+     * systems; outputBlock (layout 7, optional accumulate) writes the result as
+     * complex systems to outputBuffer, or, when 0, the step writes its real
+     * systems there as usual.  0 / 0 = no fusion; an output block needs an input
+     * block.  Input strides count complex elements (inputBufferStride), input
+     * system 0 starts at inputBlockOffset; outputBufferStride and offsetSolution
+     * count complex elements with an output block, doubles without.  FP64,
+     * N <= 512; inputZeropad selects the rows read, outputZeropad those written.
+     * No counterpart in the reference, where these are separate block plans
+     * (PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel2/pfSolve_block.h). */
     int64_t inputBlock;
     int64_t outputBlock;
     int64_t inputBlockLshift;
