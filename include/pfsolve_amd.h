@@ -101,6 +101,10 @@ typedef struct pfsolve_batch_tuning {
                                                            per lane (half the lanes, one scan level
                                                            fewer), when 2R <= 16 */
     /* unit_pairs counts systems, not pairs, for the R = 32 class (one system per pass) */
+    int32_t kernel[PFSOLVE_BATCH_CLASSES];      /* 0: default, 1: every wavefront loads, computes and
+                                                   stores its pairs, 2: compute / store split
+                                                   (R <= 16; `waves` counts the computing
+                                                   wavefronts) */
     int32_t order;            /* 0: largest N first, 1: sets as given */
     int32_t pad;
 } pfsolve_batch_tuning;
@@ -136,8 +140,12 @@ PfSolveResult pfsolve_batch_capture_graph(pfsolve_batch* b, hipStream_t stream);
 /* solve_only = 1: the graph holds pfsolve_batch_solve (tables prepared beforehand) */
 PfSolveResult pfsolve_batch_capture_graph_mode(pfsolve_batch* b, hipStream_t stream, int solve_only);
 PfSolveResult pfsolve_batch_launch_graph(pfsolve_batch* b, hipStream_t stream);
-/* Autotuner: times every candidate (waves, unit size, grid, rows per lane,
- * queue order) on scratch right-hand sides of the batch's shapes (the
+/* Waits for `stream` and reports a raised device error flag (a bounded wait
+ * inside the compute / store split kernel ran out) as
+ * PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE, clearing it. */
+PfSolveResult pfsolve_batch_sync(pfsolve_batch* b, hipStream_t stream);
+/* Autotuner: times every candidate (kernel, waves, unit size, grid, rows per
+ * lane, queue order) on scratch right-hand sides of the batch's shapes (the
  * caller's systems are not touched), keeps the fastest per class, applies it
  * to the batch and returns it in *best (may be NULL).  `repeats` timed runs
  * per candidate (>= 1).  Synchronises the stream. */

@@ -184,6 +184,10 @@ class Batch:
     def solve(self, stream: int = 0) -> None:
         check(lib().pfsolve_batch_solve(self.handle, stream), "pfsolve_batch_solve")
 
+    def sync(self, stream: int = 0) -> None:
+        """waits for `stream`; raises if a kernel's bounded wait ran out"""
+        check(lib().pfsolve_batch_sync(self.handle, stream), "pfsolve_batch_sync")
+
     def run_plain(self, stream: int = 0) -> None:
         check(lib().pfsolve_batch_run_plain(self.handle, stream), "pfsolve_batch_run_plain")
 

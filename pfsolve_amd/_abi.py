@@ -266,6 +266,7 @@ class PfSolveBatchTuning(C.Structure):
         ("unit_pairs", C.c_int32 * BATCH_CLASSES),
         ("grid_per_cu", C.c_int32 * BATCH_CLASSES),
         ("rows_per_lane_shift", C.c_int32 * BATCH_CLASSES),
+        ("kernel", C.c_int32 * BATCH_CLASSES),
         ("order", C.c_int32),
         ("pad", C.c_int32),
     ]
@@ -349,7 +350,8 @@ def declare_ext(lib: C.CDLL) -> None:
     lib.pfsolve_batch_create.argtypes = [P(vp), P(PfSolveBatchSet), i64, P(PfSolveBatchTuning)]
     lib.pfsolve_batch_create.restype = C.c_int
     for name in ("pfsolve_batch_run", "pfsolve_batch_run_plain", "pfsolve_batch_capture_graph",
-                 "pfsolve_batch_launch_graph", "pfsolve_batch_prepare", "pfsolve_batch_solve"):
+                 "pfsolve_batch_launch_graph", "pfsolve_batch_prepare", "pfsolve_batch_solve",
+                 "pfsolve_batch_sync"):
         f = getattr(lib, name)
         f.argtypes = [vp, vp]
         f.restype = C.c_int

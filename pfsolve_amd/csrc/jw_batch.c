@@ -34,10 +34,10 @@ struct pfsolve_batch {
     jw_prologue_job* d_jobs;
     jw_batch_item* d_items;
     jw_batch_unit* d_units;           /* units of all launch groups, back to back */
-    int* d_counters;                  /* one queue counter per launch group */
+    int* d_counters;                  /* one queue counter per launch group, then the error flag */
     /* launch groups: sets run with R_eff = 2 << g rows per lane */
     int64_t unit_first[NCLS], unit_count[NCLS];
-    int group_waves[NCLS], group_grid[NCLS];
+    int group_waves[NCLS], group_grid[NCLS], group_kernel[NCLS];
     int64_t total_units, total_systems;
     pfsolve_batch_tuning tuning;
     int max_RL;
@@ -49,7 +49,7 @@ struct pfsolve_batch {
 /* ---- tuning table (process-wide) ---- */
 typedef struct tune_entry {
     int cls, bucket;
-    int waves, unit_pairs, grid_per_cu, shift, order;
+    int waves, unit_pairs, grid_per_cu, shift, order, kernel;
 } tune_entry;
 static tune_entry* g_tune;
 static int g_ntune;
@@ -70,6 +70,7 @@ static void tune_put(int cls, int bucket, const pfsolve_batch_tuning* t)
             g_tune[i].grid_per_cu = t->grid_per_cu[cls];
             g_tune[i].shift = t->rows_per_lane_shift[cls];
             g_tune[i].order = t->order;
+            g_tune[i].kernel = t->kernel[cls];
             return;
         }
     }
@@ -77,7 +78,7 @@ static void tune_put(int cls, int bucket, const pfsolve_batch_tuning* t)
     if (!n) return;
     g_tune = n;
     tune_entry e = {cls, bucket, t->waves[cls], t->unit_pairs[cls], t->grid_per_cu[cls], t->rows_per_lane_shift[cls],
-                    t->order};
+                    t->order, t->kernel[cls]};
     g_tune[g_ntune++] = e;
 }
 
@@ -105,10 +106,12 @@ PfSolveResult pfsolve_tuning_save(const char* path)
     if (!path) return PFSOLVE_ERROR_EMPTY_FILE;
     FILE* f = fopen(path, "w");
     if (!f) return PFSOLVE_ERROR_EMPTY_FILE;
-    fprintf(f, "# pfsolve_amd batch tuning: class rows_per_lane pairs_log2 waves unit_pairs grid_per_cu shift order\n");
+    fprintf(f, "# pfsolve_amd batch tuning: class rows_per_lane pairs_log2 waves unit_pairs grid_per_cu shift order "
+               "kernel\n");
     for (int i = 0; i < g_ntune; i++)
-        fprintf(f, "%d %d %d %d %d %d %d %d\n", g_tune[i].cls, 2 << g_tune[i].cls, g_tune[i].bucket, g_tune[i].waves,
-                g_tune[i].unit_pairs, g_tune[i].grid_per_cu, g_tune[i].shift, g_tune[i].order);
+        fprintf(f, "%d %d %d %d %d %d %d %d %d\n", g_tune[i].cls, 2 << g_tune[i].cls, g_tune[i].bucket,
+                g_tune[i].waves, g_tune[i].unit_pairs, g_tune[i].grid_per_cu, g_tune[i].shift, g_tune[i].order,
+                g_tune[i].kernel);
     fclose(f);
     return PFSOLVE_SUCCESS;
 }
@@ -122,9 +125,11 @@ PfSolveResult pfsolve_tuning_load(const char* path)
     int bad = 0;
     while (fgets(line, sizeof(line), f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
-        int c, r, b, w, u, g, s, o;
-        if (sscanf(line, "%d %d %d %d %d %d %d %d", &c, &r, &b, &w, &u, &g, &s, &o) != 8 || c < 0 || c >= NCLS ||
-            r != (2 << c) || w < 0 || u < 0 || g < 0 || s < 0 || s > 1 || o < 0 || o > 1) {
+        /* the kernel column is optional (0: default) */
+        int c, r, b, w, u, g, s, o, k = 0;
+        const int nf = sscanf(line, "%d %d %d %d %d %d %d %d %d", &c, &r, &b, &w, &u, &g, &s, &o, &k);
+        if ((nf != 8 && nf != 9) || c < 0 || c >= NCLS || r != (2 << c) || w < 0 || u < 0 || g < 0 || s < 0 ||
+            s > 1 || o < 0 || o > 1 || k < 0 || k > 2) {
             bad = 1;
             continue;
         }
@@ -135,6 +140,7 @@ PfSolveResult pfsolve_tuning_load(const char* path)
         t.grid_per_cu[c] = g;
         t.rows_per_lane_shift[c] = s;
         t.order = o;
+        t.kernel[c] = k;
         tune_put(c, b, &t);
     }
     fclose(f);
@@ -204,6 +210,14 @@ static int group_waves_default(int g)
     return jwdev_main64_waves(2 << g);
 }
 
+/* kernel of launch group g: 1 = load, compute and store in every wavefront,
+ * 2 = compute / store split (R <= 16).  Default: the split where it is
+ * measured faster (R <= 8: 5.35 against 4.94 TB/s at N = 512); at R = 16 the
+ * table leaves LDS for 4-5 computing wavefronts and both reach the same rate. */
+static int group_kernel_default(int g) { return g <= 2 ? 2 : 1; }
+
+static int group_kernel_ok(int g, int k) { return k == 1 || (k == 2 && g <= 3); }
+
 typedef struct n_index {
     int64_t N, idx;
 } n_index;
@@ -272,23 +286,34 @@ static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
         b->group_grid[g] = 0;
         if (!b->unit_count[g]) continue;
         /* the tuning of the native class that feeds this group wins; default otherwise */
-        int w = 0, gpc = 0;
+        int w = 0, gpc = 0, k = 0;
         for (int c = 0; c < NCLS; c++) {
             const int ge = c + (c < NSHIFT && b->tuning.rows_per_lane_shift[c] ? 1 : 0);
             if (ge != g) continue;
             if (b->tuning.waves[c] > w) w = b->tuning.waves[c];
             if (b->tuning.grid_per_cu[c] > gpc) gpc = b->tuning.grid_per_cu[c];
+            if (b->tuning.kernel[c] > k) k = b->tuning.kernel[c];
         }
-        const int wmax = group_waves_default(g);
-        if (w <= 0 || w > wmax) w = wmax;
-        if (w < 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
-        const int occ = jwdev_batch64_occupancy(2 << g, w);
+        if (!group_kernel_ok(g, k)) k = group_kernel_default(g);
+        int occ = 0;
+        if (k == 2) {
+            int nc = 0, ns = 0;
+            if (!jwdev_batch64s_shape(2 << g, w, &nc, &ns)) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+            w = nc;
+            occ = jwdev_batch64s_occupancy(2 << g, w);
+        } else {
+            const int wmax = group_waves_default(g);
+            if (w <= 0 || w > wmax) w = wmax;
+            if (w < 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+            occ = jwdev_batch64_occupancy(2 << g, w);
+        }
         if (occ < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
         if (gpc <= 0 || gpc > occ) gpc = occ;
         int64_t grid = (int64_t)b->cu_count * gpc;
         if (grid > b->unit_count[g]) grid = b->unit_count[g];
         b->group_waves[g] = w;
         b->group_grid[g] = (int)(grid > 0 ? grid : 1);
+        b->group_kernel[g] = k;
     }
     return PFSOLVE_SUCCESS;
 }
@@ -403,9 +428,13 @@ PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set*
     if (hipMalloc(&b->tables, (size_t)off) != hipSuccess ||
         hipMalloc((void**)&b->d_jobs, sizeof(jw_prologue_job) * (size_t)nsets) != hipSuccess ||
         hipMalloc((void**)&b->d_items, sizeof(jw_batch_item) * (size_t)nsets) != hipSuccess ||
-        hipMalloc((void**)&b->d_counters, sizeof(int) * NCLS) != hipSuccess) {
+        hipMalloc((void**)&b->d_counters, sizeof(int) * (NCLS + 1)) != hipSuccess) {
         pfsolve_batch_destroy(b);
         return PFSOLVE_ERROR_FAILED_TO_ALLOCATE;
+    }
+    if (hipMemset(b->d_counters, 0, sizeof(int) * (NCLS + 1)) != hipSuccess) {
+        pfsolve_batch_destroy(b);
+        return PFSOLVE_ERROR_FAILED_TO_COPY;
     }
     if (tuning) {
         b->tuning = *tuning;
@@ -419,6 +448,7 @@ PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set*
             b->tuning.unit_pairs[c] = e->unit_pairs;
             b->tuning.grid_per_cu[c] = e->grid_per_cu;
             b->tuning.rows_per_lane_shift[c] = e->shift;
+            b->tuning.kernel[c] = e->kernel;
             b->tuning.order = e->order;
         }
     }
@@ -435,7 +465,7 @@ PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tun
     if (!b || !t) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     for (int c = 0; c < NCLS; c++)
         if (t->waves[c] < 0 || t->unit_pairs[c] < 0 || t->grid_per_cu[c] < 0 || t->rows_per_lane_shift[c] < 0 ||
-            t->rows_per_lane_shift[c] > 1)
+            t->rows_per_lane_shift[c] > 1 || t->kernel[c] < 0 || t->kernel[c] > 2)
             return PFSOLVE_ERROR_EMPTY_size;
     if (t->order < 0 || t->order > 1) return PFSOLVE_ERROR_EMPTY_size;
     b->tuning = *t;
@@ -451,9 +481,13 @@ static PfSolveResult launch_step(pfsolve_batch* b, const jw_batch_item* d_items,
         return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     for (int g = 0; g < NCLS; g++) {
         if (!b->unit_count[g] || (only_group >= 0 && g != only_group)) continue;
-        if (jwdev_batch64(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g], b->d_counters + g, 2 << g,
-                          b->group_waves[g], b->group_grid[g], stream) != 0)
-            return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+        const int rc = b->group_kernel[g] == 2
+                           ? jwdev_batch64s(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
+                                            b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g],
+                                            b->d_counters + NCLS, stream)
+                           : jwdev_batch64(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
+                                           b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g], stream);
+        if (rc != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     }
     return PFSOLVE_SUCCESS;
 }
@@ -555,6 +589,20 @@ PfSolveResult pfsolve_batch_launch_graph(pfsolve_batch* b, hipStream_t stream)
     return hipGraphLaunch(b->graph_exec, stream) == hipSuccess ? PFSOLVE_SUCCESS : PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
 }
 
+PfSolveResult pfsolve_batch_sync(pfsolve_batch* b, hipStream_t stream)
+{
+    if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    if (hipStreamSynchronize(stream) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+    int e = 0;
+    if (hipMemcpy(&e, b->d_counters + NCLS, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+    if (e) {
+        (void)hipMemset(b->d_counters + NCLS, 0, sizeof(int));
+        return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+    }
+    return PFSOLVE_SUCCESS;
+}
+
 PfSolveResult pfsolve_batch_get_info(const pfsolve_batch* b, pfsolve_batch_info* info)
 {
     if (!b || !info) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
@@ -599,6 +647,8 @@ static int time_group(pfsolve_batch* b, const jw_batch_item* d_items, hipStream_
         hipEventElapsedTime(&ms, e0, e1);
         if (ms < best) best = ms;
     }
+    /* a candidate whose bounded waits ran out is an error, not a timing */
+    if (rc == 0 && pfsolve_batch_sync(b, stream) != PFSOLVE_SUCCESS) rc = -1;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     *ms_out = best;
@@ -649,31 +699,50 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
             const int shifts = c < NSHIFT ? 2 : 1;
             for (int sh = 0; sh < shifts && res == PFSOLVE_SUCCESS; sh++) {
                 const int g = c + sh;
-                const int wmax = group_waves_default(g);
-                for (int w = 1; w <= wmax && res == PFSOLVE_SUCCESS; w++) {
-                    /* a few wave counts: the largest, its half and quarter */
-                    if (w != wmax && w != (wmax + 1) / 2 && w != (wmax + 3) / 4) continue;
-                    const int occ = jwdev_batch64_occupancy(2 << g, w);
-                    for (int gpc = 1; gpc <= occ && gpc <= 2; gpc++) {
-                        for (size_t ui = 0; ui < sizeof(units_try) / sizeof(units_try[0]); ui++) {
-                            pfsolve_batch_tuning t = cur;
-                            t.rows_per_lane_shift[c] = sh;
-                            t.waves[c] = w;
-                            t.grid_per_cu[c] = gpc;
-                            t.unit_pairs[c] = units_try[ui];
-                            /* the whole solve is timed: a class moved to the next group shares
-                             * that launch with the next class's sets */
-                            b->tuning = t;
-                            res = upload_queue(b);
-                            if (res != PFSOLVE_SUCCESS) break;
-                            float ms = 0.f;
-                            if (time_group(b, d_items, stream, -1, repeats, &ms) != 0) {
-                                res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
-                                break;
+                for (int k = 1; k <= 2 && res == PFSOLVE_SUCCESS; k++) {
+                    if (!group_kernel_ok(g, k)) continue;
+                    /* a few wave counts: for the one-wavefront kernel the largest, its half and
+                     * quarter; for the split kernel (computing wavefronts) the default, its half
+                     * and the most LDS holds */
+                    int wl[3] = {0, 0, 0};
+                    if (k == 1) {
+                        const int wmax = group_waves_default(g);
+                        wl[0] = wmax;
+                        wl[1] = (wmax + 1) / 2;
+                        wl[2] = (wmax + 3) / 4;
+                    } else {
+                        int nc = 0, ns = 0;
+                        if (jwdev_batch64s_shape(2 << g, 0, &nc, &ns)) wl[0] = nc;
+                        if (wl[0] > 1 && jwdev_batch64s_shape(2 << g, wl[0] / 2, &nc, &ns)) wl[1] = nc;
+                        if (jwdev_batch64s_shape(2 << g, 64, &nc, &ns) && nc != wl[0]) wl[2] = nc;
+                    }
+                    for (int wi = 0; wi < 3 && res == PFSOLVE_SUCCESS; wi++) {
+                        const int w = wl[wi];
+                        if (w < 1 || (wi > 0 && w == wl[0]) || (wi == 2 && w == wl[1])) continue;
+                        const int occ = k == 2 ? jwdev_batch64s_occupancy(2 << g, w)
+                                               : jwdev_batch64_occupancy(2 << g, w);
+                        for (int gpc = 1; gpc <= occ && gpc <= 2; gpc++) {
+                            for (size_t ui = 0; ui < sizeof(units_try) / sizeof(units_try[0]); ui++) {
+                                pfsolve_batch_tuning t = cur;
+                                t.rows_per_lane_shift[c] = sh;
+                                t.kernel[c] = k;
+                                t.waves[c] = w;
+                                t.grid_per_cu[c] = gpc;
+                                t.unit_pairs[c] = units_try[ui];
+                                /* the whole solve is timed: a class moved to the next group shares
+                                 * that launch with the next class's sets */
+                                b->tuning = t;
+                                res = upload_queue(b);
+                                if (res != PFSOLVE_SUCCESS) break;
+                                float ms = 0.f;
+                                if (time_group(b, d_items, stream, -1, repeats, &ms) != 0) {
+                                    res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+                                    break;
+                                }
+                                if (ms < best_ms) { best_ms = ms; best = t; }
                             }
-                            if (ms < best_ms) { best_ms = ms; best = t; }
+                            if (res != PFSOLVE_SUCCESS) break;
                         }
-                        if (res != PFSOLVE_SUCCESS) break;
                     }
                 }
             }

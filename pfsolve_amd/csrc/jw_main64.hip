@@ -259,8 +259,10 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
             m1b = ib;
             x0[i] = ua;
             x1[i] = ub;
+            if (R >= 16 && (i & 3) == 3) phase_barrier();
         }
     }
+    if (R >= 16) phase_barrier();
 
     // ---- 3a. forward recurrence W_r = u'_r - l'_r W_{r-1} ----
     if (ALL3 || g.fwd) {
@@ -306,9 +308,11 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
         }
     }
 
+    if (R >= 16) phase_barrier();
     // ---- 3b/3c. backward recurrences Z_r = W_r - h'_r Z_{r+1}, y_r = Z_r - E_r y_{r+1} ----
 #pragma unroll
     for (int stage = 0; stage < 2; stage++) {
+        if (R >= 16 && stage == 1) phase_barrier();
             if (!ALL3 && stage == 0 && !g.b1) continue;
         if (!ALL3 && stage == 1 && !g.b2) continue;
         double hh[R];
@@ -987,6 +991,22 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
     }
 }
 
+// A set's table (NTAB doubles) from global memory into LDS at `smem`:
+// global_load_lds moves 1 KiB per instruction, every wavefront issues all of
+// its blocks before one wait; the caller's barrier then publishes the table.
+template <int NTAB>
+__device__ __forceinline__ void copy_table_lds(unsigned char* smem, const void* table, int wave, int nwaves, int lane)
+{
+    static_assert(NTAB * 8 % 1024 == 0, "the table is whole 1 KiB blocks");
+    constexpr int nblk = NTAB * 8 / 1024;
+    const double2* src = reinterpret_cast<const double2*>(table);
+    double2* dst = reinterpret_cast<double2*>(smem);
+    for (int k = wave; k < nblk; k += nwaves)
+        __builtin_amdgcn_global_load_lds(src + 64 * k + lane, (__attribute__((address_space(3))) void*)(dst + 64 * k),
+                                         16, 0, 0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's blocks have landed
+}
+
 // Uneven-size batch: a persistent grid drains a queue of work units (a matrix
 // set and a range of its system pairs), in the order the host sorted them
 // (largest N first).  A workgroup takes the next unit with one atomic add on
@@ -1016,10 +1036,7 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_batch64_kernel(const jw_b
         const jw_batch_unit un = units[u];
         const jw_batch_item& it = items[un.item];
         if (un.item != cur) {
-            const uint4* src = reinterpret_cast<const uint4*>(it.table);
-            uint4* dst = reinterpret_cast<uint4*>(smem);
-            constexpr int n16 = NTAB * 8 / 16;
-            for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
+            copy_table_lds<NTAB>(smem, it.table, wave, nwaves, lane);
             zero_staging<R>(stg, lane);   // the new set's keep-range may be narrower
             __syncthreads();
             cur = un.item;
@@ -1070,6 +1087,129 @@ __global__ __launch_bounds__(256) void jw_batch64g_kernel(const jw_batch_item* i
     }
 }
 
+// The batch queue with the compute / store split (R <= 16): the first `ncomp`
+// wavefronts of a workgroup compute, the others store, as in jw_main64s_kernel.
+// Per work unit the workgroup stages the set's table in LDS (when the set
+// changes), compute wavefronts run the unit's pairs with the next pair's loads
+// in registers one pass ahead, and storers stream the results.  Every
+// wavefront meets at two barriers per unit: the table and the stagings change
+// only while nobody uses them.  A wait that runs out of polls raises *err and
+// the workgroup's abort flag; every wavefront then leaves at the next unit.
+template <int R>
+constexpr int bsplit_threads()
+{
+    return R <= 4 ? 1024 : (R == 8 ? 768 : 512);
+}
+
+template <int R>
+__global__ __launch_bounds__(bsplit_threads<R>()) void jw_batch64s_kernel(const jw_batch_item* items,
+                                                                          const jw_batch_unit* units, int nunits,
+                                                                          int* counter, int ncomp, int* err)
+{
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
+    constexpr int STG = 64 * R;   // staging chunks (16 bytes) of a computing wavefront: two systems
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_full[16], s_done[16];
+    __shared__ long long s_pair[16];
+    __shared__ int s_unit, s_abort;
+    const double* tab = reinterpret_cast<const double*>(smem);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* out_base = reinterpret_cast<double2*>(smem + NTAB * 8);
+    if (threadIdx.x == 0) s_abort = 0;
+    if (wave < ncomp && lane == 0) lds_set(&s_full[wave], 0);
+    int cur = -1;
+    for (;;) {
+        __syncthreads();   // the previous unit is finished: stagings empty, the table unused
+        // the abort flag is only raised inside a unit, so every wavefront sees the same answer
+        if (threadIdx.x == 0) s_unit = s_abort ? nunits : atomicAdd(counter, 1);
+        if (wave < ncomp && lane == 0) lds_set(&s_done[wave], 0);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (u >= nunits) break;
+        const jw_batch_unit un = units[u];
+        const jw_batch_item& it = items[un.item];
+        const jw_pass64 g = make_pass64(reinterpret_cast<double*>(it.rhs), it.nsys, it.stride, it.offSol, 0, it.N, 0,
+                                        it.N, it.kl, it.ku, true, true, true, it.apply_scale != 0, it.scale, lane);
+        int64_t q = un.pair0 + wave;
+        // a compute wavefront's first pair of the unit is loaded before the table copy
+        double2 pf[2][R / 2];
+        if (wave < ncomp && q < un.pair1) load_pair_direct<R>(pf, lane, g, 2 * q);
+        if (un.item != cur) {
+            copy_table_lds<NTAB>(smem, it.table, wave, nwaves, lane);
+            __syncthreads();
+            cur = un.item;
+        }
+        if (wave < ncomp) {
+            bool ok = true;
+            double2* stg = out_base + (size_t)wave * STG;
+            while (q < un.pair1) {
+                const int64_t qn = q + ncomp;
+                double x0[R], x1[R];
+                unpack_pair<R>(pf, lane, g, 2 * q, x0, x1);
+                if (qn < un.pair1) load_pair_direct<R>(pf, lane, g, 2 * qn);
+                compute_pair<R, true>(tab, lane, g, x0, x1);
+                if (!wait_flag(&s_full[wave], 0)) { ok = false; break; }
+                write_blocked<R>(stg, lane, x0, x1);
+                if (lane == 0)
+                    __hip_atomic_store(&s_pair[wave], (long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): staging and pair index written
+                compiler_fence();
+                if (lane == 0) lds_set(&s_full[wave], 1);
+                q = qn;
+            }
+            if (ok) ok = wait_flag(&s_full[wave], 0);
+            __builtin_amdgcn_s_waitcnt(0);
+            compiler_fence();
+            if (lane == 0) {
+                lds_set(&s_done[wave], 1);
+                if (!ok) {
+                    *err = 1;
+                    lds_set(&s_abort, 1);
+                }
+            }
+            continue;
+        }
+        // storer wavefront j serves compute wavefronts j, j + nstore, ...
+        const int nstore = nwaves - ncomp;
+        const int j = wave - ncomp;
+        int idle = 0;
+        for (;;) {
+            bool all_done = true, progress = false;
+            for (int w = j; w < ncomp; w += nstore) {
+                if (lds_flag(&s_full[w])) {
+                    const long long qq =
+                        __hip_atomic_load(&s_pair[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const int64_t qs = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)qq) |
+                                       ((int64_t)__builtin_amdgcn_readfirstlane((int)(qq >> 32)) << 32);
+                    store_pair<R>(out_base + (size_t)w * STG, lane, g, 2 * qs);
+                    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the staging has been read
+                    compiler_fence();
+                    if (lane == 0) lds_set(&s_full[w], 0);
+                    progress = true;
+                    all_done = false;
+                } else if (!lds_flag(&s_done[w])) {
+                    all_done = false;   // `full` is read before `done`: done with an empty staging is final
+                }
+            }
+            if (all_done) break;
+            if (progress) {
+                idle = 0;
+            } else {
+                if (++idle >= kSpinLimit) {
+                    if (lane == 0) {
+                        *err = 1;
+                        lds_set(&s_abort, 1);
+                    }
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+}
+
 template <int R>
 const void* kernel_ptr(bool all3)
 {
@@ -1113,6 +1253,19 @@ const void* pick_batch64(int R)
     default: return nullptr;
     }
 }
+
+const void* pick_batch64s(int R)
+{
+    switch (R) {
+    case 2: return reinterpret_cast<const void*>(&jw_batch64s_kernel<2>);
+    case 4: return reinterpret_cast<const void*>(&jw_batch64s_kernel<4>);
+    case 8: return reinterpret_cast<const void*>(&jw_batch64s_kernel<8>);
+    case 16: return reinterpret_cast<const void*>(&jw_batch64s_kernel<16>);
+    default: return nullptr;
+    }
+}
+
+int bsplit_threads_rt(int R) { return R <= 4 ? 1024 : (R == 8 ? 768 : 512); }
 
 template <int R>
 const void* split_ptr(bool all3, int io)
@@ -1253,6 +1406,62 @@ int jwdev_zero_counters(int* d_counters, int n, void* stream)
     hipLaunchKernelGGL(jw_zero_counters_kernel, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
                        d_counters, n);
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+/* batch queue with the compute / store split (R <= 16).  Compute wavefronts:
+ * `ncomp` when 1 <= ncomp and the table plus their stagings fit in LDS with a
+ * storer left over, else the default (12 for R <= 4, 8 for R = 8, 4 for
+ * R = 16: the same number on every SIMD); storers fill the launch bound. */
+int jwdev_batch64s_shape(int R, int ncomp, int* ncomp_out, int* nstore_out)
+{
+    if (!pick_batch64s(R)) return 0;
+    const int nw = bsplit_threads_rt(R) / 64;
+    const int64_t per = 2LL * 64 * R * 8;
+    const int64_t avail = kLdsBytesMax - 512 - jwdev_table64_elems(R) * 8;
+    int cmax = (int)(avail / per);
+    if (cmax > nw - 1) cmax = nw - 1;
+    const int cdef = R <= 4 ? 12 : (R == 8 ? 8 : 4);
+    int c = ncomp >= 1 ? ncomp : cdef;
+    if (c > cmax) c = cmax < cdef ? cmax : cdef;
+    if (c < 1) return 0;
+    *ncomp_out = c;
+    *nstore_out = nw - c;
+    return 1;
+}
+
+int64_t jwdev_batch64s_lds_bytes(int R, int ncomp)
+{
+    return jwdev_table64_elems(R) * 8 + 2LL * 64 * R * 8 * ncomp;
+}
+
+int jwdev_batch64s_occupancy(int R, int ncomp)
+{
+    int c = 0, st = 0;
+    if (!jwdev_batch64s_shape(R, ncomp, &c, &st)) return 0;
+    const void* f = pick_batch64s(R);
+    const int lds = (int)jwdev_batch64s_lds_bytes(R, c);
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * (c + st), lds) != hipSuccess) return 0;
+    return nb;
+}
+
+int jwdev_batch64s(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
+                   int ncomp, int grid, int* d_err, void* stream)
+{
+    int c = 0, st = 0;
+    if (!jwdev_batch64s_shape(R, ncomp, &c, &st) || grid < 1) {
+        snprintf(g_err64, sizeof(g_err64), "no split batch kernel for R=%d", R);
+        return -1;
+    }
+    const void* f = pick_batch64s(R);
+    const size_t lds = (size_t)jwdev_batch64s_lds_bytes(R, c);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return -2;
+    void* args[] = {const_cast<jw_batch_item**>(&d_items), const_cast<jw_batch_unit**>(&d_units), &nunits,
+                    &d_counter, &c, &d_err};
+    e = hipLaunchKernel(f, dim3(grid), dim3(64 * (c + st)), args, lds, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? 0 : -3;
 }
 
 /* compute / store split kernel (R <= 8): shape, occupancy, launch */

@@ -119,6 +119,7 @@ def run_batch(data, tuning=None, mode="run"):
     data.reset()
     with pf.Batch(data.sets, tuning) as b:
         getattr(b, mode)(stream)
+        b.sync(stream)
         return data.result()
 
 
@@ -155,6 +156,13 @@ def tuning(**kw):
     dict(order=1, unit_pairs=3),
     dict(rows_per_lane_shift=[1, 1, 1, 0, 0]),
     dict(rows_per_lane_shift=[1, 0, 1, 0, 0], unit_pairs=1, waves=1),
+    dict(kernel=1),
+    dict(kernel=2, waves=1),
+    dict(kernel=2, waves=2, unit_pairs=3),
+    dict(kernel=[2, 2, 2, 2, 1], unit_pairs=1),
+    dict(kernel=2, waves=64, order=1),
+    dict(kernel=[1, 2, 1, 2, 1], rows_per_lane_shift=[1, 1, 0, 0, 0]),
+    dict(kernel=[2, 2, 2, 2, 1], waves=5, unit_pairs=2),
 ])
 def test_batch_tunings(kw):
     """every tuning gives the oracle's results; without a rows-per-lane change the
@@ -202,6 +210,7 @@ def test_batch_tuner_keeps_caller_data_and_results():
     d.check(got)
     for c in range(_abi.BATCH_CLASSES):
         assert best.rows_per_lane_shift[c] in (0, 1)
+        assert best.kernel[c] in (0, 1, 2)
         assert best.waves[c] >= 0
 
 
@@ -227,6 +236,7 @@ def test_batch_tuning_table_is_consulted(tmp_path):
         assert info.tuning.waves[c] == best.waves[c]
         assert info.tuning.grid_per_cu[c] == best.grid_per_cu[c]
         assert info.tuning.rows_per_lane_shift[c] == best.rows_per_lane_shift[c]
+        assert info.tuning.kernel[c] == best.kernel[c]
     L.pfsolve_tuning_clear()
 
 
