@@ -207,54 +207,65 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         // ud of the row before this thread's first, read before any thread
         // overwrites its ud with h (after the barrier below)
         T udprev = (r0 >= 1 && r0 < RL) ? sh[r0 - 1] : A::zero();
+        // the wavefronts past the first NP threads own no rows: they skip the scan
+        // (wave-uniform) and only keep the barriers
+        const bool part = wv < NP / 64;
         // inclusive scan inside the wavefront
-        for (int k = 0; k < 6; k++) {
-            const int sft = 1 << k;
-            const int src = lane >= sft ? lane - sft : lane;
-            T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
-            if (lane >= sft) compose(a, b, c, e, pa, pb, pc, pe);
-        }
-        if (lane == 63 && tid < NP) {
-            s_tot[wv][0] = a;
-            s_tot[wv][1] = b;
-            s_tot[wv][2] = c;
-            s_tot[wv][3] = e;
-        }
-        // this thread's exclusive map within the wavefront (lane - 1's inclusive)
-        const int src = lane >= 1 ? lane - 1 : lane;
-        T xa = lane_fetch(a, src), xb = lane_fetch(b, src), xc = lane_fetch(c, src), xe = lane_fetch(e, src);
-        if (lane == 0) {
-            xa = A::one(); xb = A::zero(); xc = A::zero(); xe = A::one();
+        T xa = A::one(), xb = A::zero(), xc = A::zero(), xe = A::one();
+        if (part) {
+            for (int k = 0; k < 6; k++) {
+                const int sft = 1 << k;
+                const int src = lane >= sft ? lane - sft : lane;
+                T pa = lane_fetch(a, src), pb = lane_fetch(b, src), pc = lane_fetch(c, src), pe = lane_fetch(e, src);
+                if (lane >= sft) compose(a, b, c, e, pa, pb, pc, pe);
+            }
+            if (lane == 63) {
+                s_tot[wv][0] = a;
+                s_tot[wv][1] = b;
+                s_tot[wv][2] = c;
+                s_tot[wv][3] = e;
+            }
+            // this thread's exclusive map within the wavefront (lane - 1's inclusive)
+            const int src = lane >= 1 ? lane - 1 : lane;
+            xa = lane_fetch(a, src);
+            xb = lane_fetch(b, src);
+            xc = lane_fetch(c, src);
+            xe = lane_fetch(e, src);
+            if (lane == 0) {
+                xa = A::one(); xb = A::zero(); xc = A::zero(); xe = A::one();
+            }
         }
         __syncthreads();
-        // the maps of all earlier wavefronts, then the in-wavefront part on top
-        T wa = A::one(), wb = A::zero(), wc = A::zero(), we = A::one();
-        for (int w = 0; w < wv && w < NP / 64; w++) {
-            T ta = s_tot[w][0], tb = s_tot[w][1], tc = s_tot[w][2], te = s_tot[w][3];
-            compose(ta, tb, tc, te, wa, wb, wc, we);
-            wa = ta; wb = tb; wc = tc; we = te;
-        }
-        compose(xa, xb, xc, xe, wa, wb, wc, we);
-        // g_{r-1} = 1 / p_{r-1} of the row before this thread's first
-        T gprev = A::one();
-        if (r0 >= 1) gprev = A::div(A::add(xc, xe), A::add(xa, xb));
-        for (int i = 0; i < RT; i++) {
-            const int r = r0 + i;
-            if (r >= RL) break;
-            const T ldr = sl[r], udr = sh[r];
-            T l = A::zero(), g = A::zero(), h = A::zero();
-            if (r < N) {
-                // l_r = ld_r / p_{r-1},  p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r,  h_r = ud_r g_r
-                l = (r >= 1) ? A::mul(ldr, gprev) : A::zero();
-                const T p = A::sub(A::one(), A::mul(l, udprev));
-                g = A::div(A::one(), p);
-                h = A::mul(udr, g);
-                gprev = g;
+        if (part) {
+            // the maps of all earlier wavefronts, then the in-wavefront part on top
+            T wa = A::one(), wb = A::zero(), wc = A::zero(), we = A::one();
+            for (int w = 0; w < wv; w++) {
+                T ta = s_tot[w][0], tb = s_tot[w][1], tc = s_tot[w][2], te = s_tot[w][3];
+                compose(ta, tb, tc, te, wa, wb, wc, we);
+                wa = ta; wb = tb; wc = tc; we = te;
             }
-            udprev = udr;
-            sl[r] = l;
-            sg[r] = g;
-            sh[r] = h;
+            compose(xa, xb, xc, xe, wa, wb, wc, we);
+            // g_{r-1} = 1 / p_{r-1} of the row before this thread's first
+            T gprev = A::one();
+            if (r0 >= 1) gprev = A::div(A::add(xc, xe), A::add(xa, xb));
+            for (int i = 0; i < RT; i++) {
+                const int r = r0 + i;
+                if (r >= RL) break;
+                const T ldr = sl[r], udr = sh[r];
+                T l = A::zero(), g = A::zero(), h = A::zero();
+                if (r < N) {
+                    // l_r = ld_r / p_{r-1},  p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r,  h_r = ud_r g_r
+                    l = (r >= 1) ? A::mul(ldr, gprev) : A::zero();
+                    const T p = A::sub(A::one(), A::mul(l, udprev));
+                    g = A::div(A::one(), p);
+                    h = A::mul(udr, g);
+                    gprev = g;
+                }
+                udprev = udr;
+                sl[r] = l;
+                sg[r] = g;
+                sh[r] = h;
+            }
         }
     } else {
         for (int r = tid; r < RL; r += nthr) {
@@ -293,58 +304,53 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
             sh[r] = hp;
         }
         __syncthreads();
-        if (tid < 64) {
-            const int t = tid;
-            T bf = A::one(), bb1 = A::one(), bb2 = A::one();
-            for (int i = 0; i < R; i++) {
-                const int r = t * R + i;
-                bf = A::mul(bf, A::neg(sl[r]));
-                bb1 = A::mul(bb1, A::neg(sh[r]));
-                bb2 = A::mul(bb2, A::neg(sE[r]));
-            }
+        // scan multipliers: wavefront j (wave-uniform) takes recurrence j (0: forward
+        // through l', 1: backward through h', 2: backward through E)
+        if (tid < 192) {
+            const int j = tid >> 6;
+            const int t = tid & 63;
+            const T* src = j == 0 ? sl : (j == 1 ? sh : sE);
+            T v = A::one();
+            for (int i = 0; i < R; i++) v = A::mul(v, A::neg(src[t * R + i]));
             // in-row Hillis-Steele: before step k the value covers the 2^k lanes a
-            // shift of 2^k combines; after 4 steps, the row prefix / suffix
+            // shift of 2^k combines; after 4 steps, the row prefix (j = 0) or suffix
             const int tr = t & 15;
             const int64_t SC = (int64_t)8 * R * 64;
-            T cur[3] = {bf, bb1, bb2};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int s = 1 << k;
-#pragma unroll
-                for (int j = 0; j < 3; j++) table[SC + (int64_t)(j * 8 + k) * 64 + t] = cur[j];
-                T up = lane_fetch(cur[0], tr >= s ? t - s : t);
-                T dn1 = lane_fetch(cur[1], tr + s <= 15 ? t + s : t);
-                T dn2 = lane_fetch(cur[2], tr + s <= 15 ? t + s : t);
-                if (tr >= s) cur[0] = A::mul(cur[0], up);
-                if (tr + s <= 15) {
-                    cur[1] = A::mul(cur[1], dn1);
-                    cur[2] = A::mul(cur[2], dn2);
+                table[SC + (int64_t)(j * 8 + k) * 64 + t] = v;
+                if (j == 0) {
+                    const T up = lane_fetch(v, tr >= s ? t - s : t);
+                    if (tr >= s) v = A::mul(v, up);
+                } else {
+                    const T dn = lane_fetch(v, tr + s <= 15 ? t + s : t);
+                    if (tr + s <= 15) v = A::mul(v, dn);
                 }
             }
             const int rw = t >> 4;
-            // forward cross-row combine in two broadcasts: rows 1,3 take the end of
-            // rows 0,2 (multiplier: row prefix), then rows 2,3 take lane 31, which
-            // row 3 reaches through all of row 2 (prefix of lane 47 times its own)
-            {
-                const T pre47 = lane_fetch(cur[0], 47);
+            if (j == 0) {
+                // forward cross-row combine in two broadcasts: rows 1,3 take the end of
+                // rows 0,2 (multiplier: row prefix), then rows 2,3 take lane 31, which
+                // row 3 reaches through all of row 2 (prefix of lane 47 times its own)
+                const T pre47 = lane_fetch(v, 47);
                 T m2 = A::zero();
-                if (rw == 2) m2 = cur[0];
-                if (rw == 3) m2 = A::mul(pre47, cur[0]);
-                table[SC + (int64_t)4 * 64 + t] = cur[0];
+                if (rw == 2) m2 = v;
+                if (rw == 3) m2 = A::mul(pre47, v);
+                table[SC + (int64_t)4 * 64 + t] = v;
                 table[SC + (int64_t)5 * 64 + t] = m2;
                 table[SC + (int64_t)6 * 64 + t] = A::zero();
                 table[SC + (int64_t)7 * 64 + t] = A::zero();
-            }
-            // backward cross-row combine from the in-row results S_1..S_3 at the
-            // row starts (lanes 16, 32, 48): a lane of row r adds
-            //   suf_t * F_{r+1},  F_q = S_q + P_q F_{q+1},  P_q = suffix product of lane 16q,
-            // i.e. c_k,t * S_k with c_k,t = suf_t * prod_{r<q<k} P_q for k > r, else 0
-            for (int j = 1; j < 3; j++) {
-                const T P1 = lane_fetch(cur[j], 16), P2 = lane_fetch(cur[j], 32);
+            } else {
+                // backward cross-row combine from the in-row results S_1..S_3 at the
+                // row starts (lanes 16, 32, 48): a lane of row r adds
+                //   suf_t * F_{r+1},  F_q = S_q + P_q F_{q+1},  P_q = suffix product of lane 16q,
+                // i.e. c_k,t * S_k with c_k,t = suf_t * prod_{r<q<k} P_q for k > r, else 0
+                const T P1 = lane_fetch(v, 16), P2 = lane_fetch(v, 32);
                 T c1 = A::zero(), c2 = A::zero(), c3 = A::zero();
-                if (rw == 0) { c1 = cur[j]; c2 = A::mul(cur[j], P1); c3 = A::mul(c2, P2); }
-                if (rw == 1) { c2 = cur[j]; c3 = A::mul(cur[j], P2); }
-                if (rw == 2) c3 = cur[j];
+                if (rw == 0) { c1 = v; c2 = A::mul(v, P1); c3 = A::mul(c2, P2); }
+                if (rw == 1) { c2 = v; c3 = A::mul(v, P2); }
+                if (rw == 2) c3 = v;
                 table[SC + (int64_t)(j * 8 + 4) * 64 + t] = c1;
                 table[SC + (int64_t)(j * 8 + 5) * 64 + t] = c2;
                 table[SC + (int64_t)(j * 8 + 6) * 64 + t] = c3;
