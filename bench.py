@@ -265,6 +265,36 @@ def bench_batch(torch, pf, steps: int, warmup: int) -> dict:
     return out
 
 
+def batch_traffic_probe(torch, pf) -> dict:
+    """For PMC counter runs: the 1024-degree batch with its default tuning, tables
+    prepared once and three solves, then torch kernels of known traffic (a 1 GiB
+    read and a 1 GiB write) that calibrate FETCH_SIZE / WRITE_SIZE."""
+    lib = pf.lib()
+    sets, mats, nrhs = harmonic_batch_sets(lib, 0, 0)
+    dmat = torch.from_numpy(mats).cuda()
+    x = torch.rand(nrhs, dtype=torch.float64, device="cuda") * 2 - 1
+    for st in sets:
+        st["matrices"] = dmat.data_ptr()
+        st["rhs"] += x.data_ptr()
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    with pf.Batch(sets) as b:
+        b.prepare(sp)
+        for _ in range(3):
+            b.solve(sp)
+        b.sync(sp)
+        info = b.info()
+    n = (1 << 30) // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    a.sum()          # reads 1 GiB
+    a.fill_(0.5)     # writes 1 GiB
+    torch.cuda.synchronize()
+    return {"probe": "batch traffic", "algorithmic_bytes_per_solve": int(info.algorithmic_bytes),
+            "table_bytes": int(info.table_bytes), "rhs_bytes_read_and_written": int(16 * nrhs),
+            "calibration": {"read_bytes": 8 * n, "write_bytes": 8 * n}}
+
+
 def to_dd(q):
     """exact rational -> (hi, lo) double-double, round to nearest"""
     from fractions import Fraction
@@ -504,6 +534,9 @@ def main() -> int:
                     help="length of the serving stream (60 = the full stream; 0 skips it)")
     ap.add_argument("--no-stream-reference", action="store_true",
                     help="skip timing torch fill_ / sum / copy_ / in-place mul_ over 1 GiB (practical ceilings)")
+    ap.add_argument("--batch-traffic-probe", action="store_true",
+                    help="only run the 1024-degree batch solve three times and 1 GiB torch read / write "
+                         "kernels (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE), print one JSON line, exit")
     ap.add_argument("--calibrate", action="store_true",
                     help="time the streaming reference over the headline kernel's algorithmic byte count instead "
                          "of 1 GiB (the PMC counter calibration runs)")
@@ -520,6 +553,9 @@ def main() -> int:
         print("no GPU available", file=sys.stderr)
         return 1
     torch.cuda.init()
+    if args.batch_traffic_probe:
+        print(json.dumps(batch_traffic_probe(torch, pf)))
+        return 0
     lib = pf.lib()
     N, nreal = N_ROWS, 2 * N_COMPLEX
     buf, (offV, offT, offM) = gen_fused_matrices(N, 0, lib.pfsolve_reorder_i0)
