@@ -59,7 +59,12 @@ constexpr int kPrologueThreads = 512;
 constexpr int kProRows = 4;   // rows per thread: R * L <= 2048
 
 // LDS of prologue_body: 5 arrays of R*L elements (ld -> l -> l', ud -> h -> h', D, E, g)
-__host__ __device__ inline size_t prologue_lds_bytes(int RL, size_t elt) { return 5 * (size_t)RL * elt; }
+// five arrays of RL values; the last (g) is at least 16 values long, since the
+// pivot scan keeps its wavefront totals there before g is written
+__host__ __device__ inline size_t prologue_lds_bytes(int RL, size_t elt)
+{
+    return (4 * (size_t)RL + (RL > 16 ? (size_t)RL : 16)) * elt;
+}
 
 // the band coefficients v_{-2..2} of row r of the matvec (zero outside the
 // matrix), P = storage position of row r in the permuted diagonals
@@ -170,7 +175,10 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     // kernel that builds a table gives the same table, bit for bit).
     if (d.solve_kind == JW_SOLVE_TRI) {
         constexpr int NP = 256;      // participating threads (workgroups have >= 256)
-        __shared__ T s_tot[NP / 64][4];
+        // the four wavefront totals live in g's array, which nothing reads or writes
+        // until the per-row pass below (a barrier keeps that pass after every read):
+        // no static LDS, so four 256-thread builds of 1024 rows fit one CU
+        T* s_tot = sg;
         const int RT = (RL + NP - 1) / NP;
         const int r0 = tid < NP ? tid * RT : RL;   // threads >= NP own no rows
         const int lane = tid & 63, wv = tid >> 6;
@@ -220,10 +228,10 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 if (lane >= sft) compose(a, b, c, e, pa, pb, pc, pe);
             }
             if (lane == 63) {
-                s_tot[wv][0] = a;
-                s_tot[wv][1] = b;
-                s_tot[wv][2] = c;
-                s_tot[wv][3] = e;
+                s_tot[4 * wv + 0] = a;
+                s_tot[4 * wv + 1] = b;
+                s_tot[4 * wv + 2] = c;
+                s_tot[4 * wv + 3] = e;
             }
             // this thread's exclusive map within the wavefront (lane - 1's inclusive)
             const int src = lane >= 1 ? lane - 1 : lane;
@@ -236,18 +244,21 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
             }
         }
         __syncthreads();
+        T gprev = A::one();
         if (part) {
             // the maps of all earlier wavefronts, then the in-wavefront part on top
             T wa = A::one(), wb = A::zero(), wc = A::zero(), we = A::one();
             for (int w = 0; w < wv; w++) {
-                T ta = s_tot[w][0], tb = s_tot[w][1], tc = s_tot[w][2], te = s_tot[w][3];
+                T ta = s_tot[4 * w + 0], tb = s_tot[4 * w + 1], tc = s_tot[4 * w + 2], te = s_tot[4 * w + 3];
                 compose(ta, tb, tc, te, wa, wb, wc, we);
                 wa = ta; wb = tb; wc = tc; we = te;
             }
             compose(xa, xb, xc, xe, wa, wb, wc, we);
             // g_{r-1} = 1 / p_{r-1} of the row before this thread's first
-            T gprev = A::one();
             if (r0 >= 1) gprev = A::div(A::add(xc, xe), A::add(xa, xb));
+        }
+        __syncthreads();   // the totals are read: g's array is free
+        if (part) {
             for (int i = 0; i < RT; i++) {
                 const int r = r0 + i;
                 if (r >= RL) break;
