@@ -468,21 +468,32 @@ __device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int l
     compiler_fence();
 }
 
+// A set's table (NTAB doubles) from global memory into LDS at `smem`:
+// global_load_lds moves 1 KiB per instruction, every wavefront issues all of
+// its blocks before one wait; the caller's barrier then publishes the table.
+template <int NTAB>
+__device__ __forceinline__ void copy_table_lds(unsigned char* smem, const void* table, int wave, int nwaves, int lane)
+{
+    static_assert(NTAB * 8 % 1024 == 0, "the table is whole 1 KiB blocks");
+    constexpr int nblk = NTAB * 8 / 1024;
+    const double2* src = reinterpret_cast<const double2*>(table);
+    double2* dst = reinterpret_cast<double2*>(smem);
+    for (int k = wave; k < nblk; k += nwaves)
+        __builtin_amdgcn_global_load_lds(src + 64 * k + lane, (__attribute__((address_space(3))) void*)(dst + 64 * k),
+                                         16, 0, 0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's blocks have landed
+}
+
 template <int R, bool ALL3>
 __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc d)
 {
     constexpr int NTAB = JW_TAB64_ELEMS(R);
     extern __shared__ __align__(16) unsigned char smem[];
     const double* tab = reinterpret_cast<const double*>(smem);
-    {
-        const uint4* src = reinterpret_cast<const uint4*>(d.table);
-        uint4* dst = reinterpret_cast<uint4*>(smem);
-        constexpr int n16 = NTAB * 8 / 16;
-        for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
-    }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = blockDim.x >> 6;
+    copy_table_lds<NTAB>(smem, d.table, wave, nwaves, lane);
     double2* stg = reinterpret_cast<double2*>(smem + NTAB * 8) + (size_t)wave * 64 * R;
     zero_staging<R>(stg, lane);
     __syncthreads();
@@ -989,22 +1000,6 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
             __builtin_amdgcn_s_sleep(2);
         }
     }
-}
-
-// A set's table (NTAB doubles) from global memory into LDS at `smem`:
-// global_load_lds moves 1 KiB per instruction, every wavefront issues all of
-// its blocks before one wait; the caller's barrier then publishes the table.
-template <int NTAB>
-__device__ __forceinline__ void copy_table_lds(unsigned char* smem, const void* table, int wave, int nwaves, int lane)
-{
-    static_assert(NTAB * 8 % 1024 == 0, "the table is whole 1 KiB blocks");
-    constexpr int nblk = NTAB * 8 / 1024;
-    const double2* src = reinterpret_cast<const double2*>(table);
-    double2* dst = reinterpret_cast<double2*>(smem);
-    for (int k = wave; k < nblk; k += nwaves)
-        __builtin_amdgcn_global_load_lds(src + 64 * k + lane, (__attribute__((address_space(3))) void*)(dst + 64 * k),
-                                         16, 0, 0);
-    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's blocks have landed
 }
 
 // Uneven-size batch: a persistent grid drains a queue of work units (a matrix
