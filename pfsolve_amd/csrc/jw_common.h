@@ -202,6 +202,24 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
 }
 
 
+// `bytes` (a multiple of 16) from global memory at `src` (16-byte aligned) into LDS
+// at `lds`: global_load_lds moves 64 x 16 bytes per instruction, every wavefront
+// issues all of its blocks before one wait (a copy through registers would wait
+// for each load in turn); the caller's barrier then publishes the data.  `wave`
+// must be wavefront-uniform.
+__device__ __forceinline__ void copy_to_lds(void* lds, const void* src, int bytes, int wave, int nwaves, int lane)
+{
+    const int n16 = bytes / 16;
+    const double2* s = reinterpret_cast<const double2*>(src);
+    double2* d = reinterpret_cast<double2*>(lds);
+    for (int k = wave; 64 * k < n16; k += nwaves) {
+        const int c = 64 * k + lane;
+        if (c < n16)
+            __builtin_amdgcn_global_load_lds(s + c, (__attribute__((address_space(3))) void*)(d + 64 * k), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's blocks have landed
+}
+
 // coefficient table of format 2 (64 lanes): pair p of row-in-lane i, lane t
 __host__ __device__ __forceinline__ int tab_coef64(int p, int i, int t, int R)
 {

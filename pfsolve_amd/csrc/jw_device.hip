@@ -80,13 +80,10 @@ __global__ __launch_bounds__(512) void jw_main_kernel(jw_run_desc d)
     constexpr int NTAB = JW_NCOEF * R * L + 4 * LOG2L * L;
     constexpr int G = 64 / L;
 
-    // stage the coefficient table in LDS (16-byte moves)
-    {
-        const uint4* src = reinterpret_cast<const uint4*>(d.table);
-        uint4* dst = reinterpret_cast<uint4*>(jw_smem);
-        constexpr int n16 = (NTAB * (int)sizeof(T)) / 16;
-        for (int k = threadIdx.x; k < n16; k += blockDim.x) dst[k] = src[k];
-    }
+    // stage the coefficient table in LDS
+    static_assert(NTAB * (int)sizeof(T) % 16 == 0, "the table is whole 16-byte chunks");
+    copy_to_lds(jw_smem, d.table, NTAB * (int)sizeof(T), __builtin_amdgcn_readfirstlane(threadIdx.x >> 6),
+                blockDim.x >> 6, threadIdx.x & 63);
     __syncthreads();
 
     const int lane = threadIdx.x & 63;

@@ -468,20 +468,12 @@ __device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int l
     compiler_fence();
 }
 
-// A set's table (NTAB doubles) from global memory into LDS at `smem`:
-// global_load_lds moves 1 KiB per instruction, every wavefront issues all of
-// its blocks before one wait; the caller's barrier then publishes the table.
+// A set's table (NTAB doubles, whole 1 KiB blocks) from global memory into LDS at `smem`
 template <int NTAB>
 __device__ __forceinline__ void copy_table_lds(unsigned char* smem, const void* table, int wave, int nwaves, int lane)
 {
     static_assert(NTAB * 8 % 1024 == 0, "the table is whole 1 KiB blocks");
-    constexpr int nblk = NTAB * 8 / 1024;
-    const double2* src = reinterpret_cast<const double2*>(table);
-    double2* dst = reinterpret_cast<double2*>(smem);
-    for (int k = wave; k < nblk; k += nwaves)
-        __builtin_amdgcn_global_load_lds(src + 64 * k + lane, (__attribute__((address_space(3))) void*)(dst + 64 * k),
-                                         16, 0, 0);
-    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's blocks have landed
+    copy_to_lds(smem, table, NTAB * 8, wave, nwaves, lane);
 }
 
 template <int R, bool ALL3>
