@@ -60,6 +60,13 @@ int pfsolve_kernel_supported(int lanes, int rows_per_lane, int precision);
 /* last device-side error text of the calling thread */
 const char* pfsolve_last_error(void);
 
+/* Bounded waits of the compute / store split kernel: polls before a wait gives up
+ * and raises the device error flag that PfSolveSync reports (0: the default, 2^22;
+ * negative: none, so every launch takes the time-out path at once and still
+ * exits normally).  For tests of failure handling; no counterpart in the
+ * reference. */
+PfSolveResult pfsolve_set_wait_polls(PfSolveApplication* app, int32_t polls);
+
 /* Timing hook: when set, every PfSolveAppend records `start` right before and
  * `stop` right after the main JW kernel on the plan's stream (NULL clears). */
 PfSolveResult pfsolve_set_kernel_events(PfSolveApplication* app, hipEvent_t start, hipEvent_t stop);

@@ -346,6 +346,8 @@ def declare(lib: C.CDLL) -> None:
 def declare_ext(lib: C.CDLL) -> None:
     lib.pfsolve_set_kernel_events.argtypes = [C.POINTER(PfSolveApplication), C.c_void_p, C.c_void_p]
     lib.pfsolve_set_kernel_events.restype = C.c_int
+    lib.pfsolve_set_wait_polls.argtypes = [C.POINTER(PfSolveApplication), C.c_int32]
+    lib.pfsolve_set_wait_polls.restype = C.c_int
     P = C.POINTER
     lib.pfsolve_batch_create.argtypes = [P(vp), P(PfSolveBatchSet), i64, P(PfSolveBatchTuning)]
     lib.pfsolve_batch_create.restype = C.c_int

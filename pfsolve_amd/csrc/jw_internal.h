@@ -82,6 +82,10 @@ typedef struct jw_run_desc {
     const void* in_c;         /* packed complex input */
     void* out_c;              /* packed complex output */
     int64_t in_stride_c, in_off_c, out_stride_c, out_off_c;   /* complex elements */
+    /* polls before a bounded wait of the compute / store split kernel gives up:
+     * 0 the default (2^22), negative none (the time-out path at once) */
+    int32_t wait_polls;
+    int32_t pad_wait;
 } jw_run_desc;
 
 /* One matrix set for the batched prologue (one workgroup each). */

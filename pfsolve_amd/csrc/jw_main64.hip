@@ -853,10 +853,10 @@ __device__ __forceinline__ void lds_set(int* f, int v)
     __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// waits until *f == v; false when the limit ran out
-__device__ __forceinline__ bool wait_flag(int* f, int v)
+// waits until *f == v; false when `polls` polls ran out
+__device__ __forceinline__ bool wait_flag(int* f, int v, int polls = kSpinLimit)
 {
-    for (int it = 0; it < kSpinLimit; it++) {
+    for (int it = 0; it < polls; it++) {
         if (lds_flag(f) == v) return true;
         __builtin_amdgcn_s_sleep(2);
     }
@@ -884,6 +884,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
     double2* out_base = reinterpret_cast<double2*>(smem + NTAB * 8);
     const int64_t npass = (d.nsys + 1) >> 1;
     const int64_t step = (int64_t)gridDim.x * ncomp;
+    const int polls = d.wait_polls > 0 ? d.wait_polls : (d.wait_polls < 0 ? 0 : kSpinLimit);
     int64_t q = (int64_t)blockIdx.x * ncomp + wave;
     // a compute wavefront's first pair is loaded into registers before the table
     // build, so its latency overlaps the build
@@ -938,7 +939,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
             }
             if (IOMODE != 0) scale_input<R>(x0, x1, lane, d, fin);
             compute_pair<R, ALL3>(tab, lane, g, x0, x1);
-            if (!wait_flag(vfull + wave, 0)) { ok = false; break; }
+            if (!wait_flag(vfull + wave, 0, polls)) { ok = false; break; }
             write_blocked<R>(stg, lane, x0, x1);
             if (lane == 0) __hip_atomic_store(&s_pair[wave], (long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): staging and pair index written
@@ -946,7 +947,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
             if (lane == 0) lds_set(vfull + wave, 1);
             q = qn;
         }
-        if (ok) ok = wait_flag(vfull + wave, 0);
+        if (ok) ok = wait_flag(vfull + wave, 0, polls);
         __builtin_amdgcn_s_waitcnt(0);
         compiler_fence();
         if (lane == 0) {
@@ -985,7 +986,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
         if (progress) {
             idle = 0;
         } else {
-            if (++idle >= kSpinLimit) {
+            if (++idle >= polls) {
                 if (lane == 0) *err = 1;
                 break;
             }

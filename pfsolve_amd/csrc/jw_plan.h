@@ -29,6 +29,7 @@ struct pfsolve_plan {
      * CU, and the device flag it raises when a bounded wait runs out */
     int split64, occupancy_split;
     int* d_err;
+    int wait_polls;           /* jw_run_desc.wait_polls of the split kernel's launches */
     hipEvent_t ev_start, ev_stop;   /* optional timing events around the main kernel */
 };
 

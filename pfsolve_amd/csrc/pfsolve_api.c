@@ -227,6 +227,7 @@ static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveL
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
     if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+    run->wait_polls = p->wait_polls;
     if (jwdev_main64s(run, src, cfg->buffer[0], p->R64, (int)grid, p->d_err, s) != 0)
         return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
@@ -292,6 +293,7 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
             if (grid < 1) grid = 1;
             if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
             if (p->split64) {
+                run.wait_polls = p->wait_polls;
                 if (jwdev_main64s(&run, &src, cfg->buffer[0], p->R64, (int)grid, p->d_err, s) != 0)
                     return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             } else if (jwdev_main64(&run, p->R64, p->waves64, (int)grid, s) != 0) {
@@ -613,6 +615,13 @@ int pfsolve_kernel_supported(int lanes, int rows_per_lane, int precision)
 }
 
 const char* pfsolve_last_error(void) { return jwdev_last_error(); }
+
+PfSolveResult pfsolve_set_wait_polls(PfSolveApplication* app, int32_t polls)
+{
+    if (!app || !app->localFFTPlan) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    app->localFFTPlan->wait_polls = polls;
+    return PFSOLVE_SUCCESS;
+}
 
 PfSolveResult pfsolve_set_kernel_events(PfSolveApplication* app, hipEvent_t start, hipEvent_t stop)
 {

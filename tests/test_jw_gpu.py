@@ -364,6 +364,36 @@ def test_wave_per_system_kernel_is_selected_for_aligned_fp64():
         assert plan.geometry().fast_rows_per_lane == 0
 
 
+def test_split_kernel_time_out_is_reported_and_recovers():
+    """The split kernel's bounded waits, made to give up at once from the host
+    (pfsolve_set_wait_polls with a negative bound): every wavefront still leaves,
+    PfSolveSync reports FAILED_TO_SYNCHRONIZE and clears the flag, and with the
+    default bound the next launch gives the oracle's result again."""
+    import torch
+    N, B = 512, 64
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 3, 1)
+    x = O.uniform(17, -1, 1, B * N)
+    dmat = torch.from_numpy(buf).cuda()
+    dx = torch.from_numpy(x.copy()).cuda()
+    L = pf.lib()
+    with pf.Plan(dmat.data_ptr(), dx.data_ptr(), size=[N, B, 1], jw_type=14, doublePrecision=1, scaleC=1.0,
+                 KL=2, KU=2, offsetV=offV, offsetT=offT, offsetM=offM) as plan:
+        assert plan.geometry().fast_store_waves >= 1
+        assert L.pfsolve_set_wait_polls(C.byref(plan.app), -1) == 0
+        plan.append()
+        assert pf.PfSolveSync(plan.app) == _abi.PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE
+        assert pf.PfSolveSync(plan.app) == 0   # the flag was cleared
+        assert L.pfsolve_set_wait_polls(C.byref(plan.app), 0) == 0
+        dx.copy_(torch.from_numpy(x))
+        plan.append()
+        assert pf.PfSolveSync(plan.app) == 0
+        got = dx.cpu().numpy()
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    want = O.jw_f128(p, buf, x)
+    check_systems(got, want, x, B, N, 0, N, 0, N, TOL64)
+
+
 def test_split_kernel_many_passes_per_wavefront():
     """enough pairs that every computing wavefront hands many outputs to its storer:
     N = 256 (R = 4) and N = 100 (R = 2), 40000 real systems, a sample of them against
