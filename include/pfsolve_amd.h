@@ -151,6 +151,9 @@ PfSolveResult pfsolve_batch_launch_graph(pfsolve_batch* b, hipStream_t stream);
  * inside the compute / store split kernel ran out) as
  * PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE, clearing it. */
 PfSolveResult pfsolve_batch_sync(pfsolve_batch* b, hipStream_t stream);
+/* Bound of the split queue kernel's waits, as pfsolve_set_wait_polls for plans
+ * (0: the default; negative: none, for tests of failure handling). */
+PfSolveResult pfsolve_batch_set_wait_polls(pfsolve_batch* b, int32_t polls);
 /* Autotuner: times every candidate (kernel, waves, unit size, grid, rows per
  * lane, queue order) on scratch right-hand sides of the batch's shapes (the
  * caller's systems are not touched), keeps the fastest per class, applies it

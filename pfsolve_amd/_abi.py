@@ -363,6 +363,8 @@ def declare_ext(lib: C.CDLL) -> None:
     lib.pfsolve_batch_tune.restype = C.c_int
     lib.pfsolve_batch_set_tuning.argtypes = [vp, P(PfSolveBatchTuning)]
     lib.pfsolve_batch_set_tuning.restype = C.c_int
+    lib.pfsolve_batch_set_wait_polls.argtypes = [vp, C.c_int32]
+    lib.pfsolve_batch_set_wait_polls.restype = C.c_int
     lib.pfsolve_batch_get_info.argtypes = [vp, P(PfSolveBatchInfo)]
     lib.pfsolve_batch_get_info.restype = C.c_int
     lib.pfsolve_batch_destroy.argtypes = [vp]

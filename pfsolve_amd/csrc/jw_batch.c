@@ -38,6 +38,7 @@ struct pfsolve_batch {
     /* launch groups: sets run with R_eff = 2 << g rows per lane */
     int64_t unit_first[NCLS], unit_count[NCLS];
     int group_waves[NCLS], group_grid[NCLS], group_kernel[NCLS];
+    int wait_polls;                   /* bound of the split kernel's waits (0: default) */
     int64_t total_units, total_systems;
     pfsolve_batch_tuning tuning;
     int max_RL;
@@ -484,7 +485,7 @@ static PfSolveResult launch_step(pfsolve_batch* b, const jw_batch_item* d_items,
         const int rc = b->group_kernel[g] == 2
                            ? jwdev_batch64s(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
                                             b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g],
-                                            b->d_counters + NCLS, stream)
+                                            b->d_counters + NCLS, b->wait_polls, stream)
                            : jwdev_batch64(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
                                            b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g], stream);
         if (rc != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
@@ -587,6 +588,13 @@ PfSolveResult pfsolve_batch_launch_graph(pfsolve_batch* b, hipStream_t stream)
 {
     if (!b || !b->graph_exec) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     return hipGraphLaunch(b->graph_exec, stream) == hipSuccess ? PFSOLVE_SUCCESS : PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+}
+
+PfSolveResult pfsolve_batch_set_wait_polls(pfsolve_batch* b, int32_t polls)
+{
+    if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    b->wait_polls = polls;
+    return PFSOLVE_SUCCESS;
 }
 
 PfSolveResult pfsolve_batch_sync(pfsolve_batch* b, hipStream_t stream)

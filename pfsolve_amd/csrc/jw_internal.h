@@ -158,12 +158,13 @@ int jwdev_zero_counters(int* d_counters, int n, void* stream);
 int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
                   int waves, int grid, void* stream);
 /* the same queue with the compute / store split (R <= 16): `ncomp` compute
- * wavefronts (0: default); *d_err is raised if a bounded wait ran out */
+ * wavefronts (0: default); *d_err is raised if a bounded wait ran out (after
+ * `wait_polls` polls: 0 the default, negative none) */
 int jwdev_batch64s_shape(int R, int ncomp, int* ncomp_out, int* nstore_out);
 int64_t jwdev_batch64s_lds_bytes(int R, int ncomp);
 int jwdev_batch64s_occupancy(int R, int ncomp);
 int jwdev_batch64s(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
-                   int ncomp, int grid, int* d_err, void* stream);
+                   int ncomp, int grid, int* d_err, int wait_polls, void* stream);
 int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream);
 int jwdev_block(const jw_block_desc* d, void* stream);
 int jwdev_gbmv(const jw_gbmv_desc* d, void* stream);

@@ -1092,8 +1092,11 @@ constexpr int bsplit_threads()
 template <int R>
 __global__ __launch_bounds__(bsplit_threads<R>()) void jw_batch64s_kernel(const jw_batch_item* items,
                                                                           const jw_batch_unit* units, int nunits,
-                                                                          int* counter, int ncomp, int* err)
+                                                                          int* counter, int ncomp, int* err,
+                                                                          int wait_polls)
 {
+    // polls before a bounded wait gives up: 0 the default, negative none
+    const int polls = wait_polls > 0 ? wait_polls : (wait_polls < 0 ? 0 : kSpinLimit);
     constexpr int NTAB = JW_TAB64_ELEMS(R);
     constexpr int STG = 64 * R;   // staging chunks (16 bytes) of a computing wavefront: two systems
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1138,7 +1141,7 @@ __global__ __launch_bounds__(bsplit_threads<R>()) void jw_batch64s_kernel(const 
                 unpack_pair<R>(pf, lane, g, 2 * q, x0, x1);
                 if (qn < un.pair1) load_pair_direct<R>(pf, lane, g, 2 * qn);
                 compute_pair<R, true>(tab, lane, g, x0, x1);
-                if (!wait_flag(&s_full[wave], 0)) { ok = false; break; }
+                if (!wait_flag(&s_full[wave], 0, polls)) { ok = false; break; }
                 write_blocked<R>(stg, lane, x0, x1);
                 if (lane == 0)
                     __hip_atomic_store(&s_pair[wave], (long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1147,7 +1150,7 @@ __global__ __launch_bounds__(bsplit_threads<R>()) void jw_batch64s_kernel(const 
                 if (lane == 0) lds_set(&s_full[wave], 1);
                 q = qn;
             }
-            if (ok) ok = wait_flag(&s_full[wave], 0);
+            if (ok) ok = wait_flag(&s_full[wave], 0, polls);
             __builtin_amdgcn_s_waitcnt(0);
             compiler_fence();
             if (lane == 0) {
@@ -1185,7 +1188,7 @@ __global__ __launch_bounds__(bsplit_threads<R>()) void jw_batch64s_kernel(const 
             if (progress) {
                 idle = 0;
             } else {
-                if (++idle >= kSpinLimit) {
+                if (++idle >= polls) {
                     if (lane == 0) {
                         *err = 1;
                         lds_set(&s_abort, 1);
@@ -1435,7 +1438,7 @@ int jwdev_batch64s_occupancy(int R, int ncomp)
 }
 
 int jwdev_batch64s(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
-                   int ncomp, int grid, int* d_err, void* stream)
+                   int ncomp, int grid, int* d_err, int wait_polls, void* stream)
 {
     int c = 0, st = 0;
     if (!jwdev_batch64s_shape(R, ncomp, &c, &st) || grid < 1) {
@@ -1447,7 +1450,7 @@ int jwdev_batch64s(const jw_batch_item* d_items, const jw_batch_unit* d_units, i
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -2;
     void* args[] = {const_cast<jw_batch_item**>(&d_items), const_cast<jw_batch_unit**>(&d_units), &nunits,
-                    &d_counter, &c, &d_err};
+                    &d_counter, &c, &d_err, &wait_polls};
     e = hipLaunchKernel(f, dim3(grid), dim3(64 * (c + st)), args, lds, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : -3;
 }

@@ -175,6 +175,30 @@ def test_batch_tunings(kw):
         np.testing.assert_array_equal(got, ref)
 
 
+def test_batch_split_kernel_time_out_is_reported_and_recovers():
+    """The split queue kernel's bounded waits, made to give up at once from the host
+    (pfsolve_batch_set_wait_polls with a negative bound): the step ends,
+    pfsolve_batch_sync reports FAILED_TO_SYNCHRONIZE once, and with the default bound
+    the next step equals the plain per-set form, bit for bit."""
+    t = torch()
+    L = pf.lib()
+    d = BatchData(SMALL_SETS, seed=8)
+    ref = run_batch(d, mode="run_plain")
+    stream = t.cuda.current_stream().cuda_stream
+    d.reset()
+    with pf.Batch(d.sets, tuning(kernel=[2, 2, 2, 2, 1])) as b:
+        assert L.pfsolve_batch_set_wait_polls(b.handle, -1) == 0
+        b.run(stream)
+        assert L.pfsolve_batch_sync(b.handle, stream) == _abi.PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE
+        assert L.pfsolve_batch_sync(b.handle, stream) == 0   # the flag was cleared
+        assert L.pfsolve_batch_set_wait_polls(b.handle, 0) == 0
+        d.reset()
+        b.run(stream)
+        b.sync(stream)
+        got = d.result()
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_batch_graph_replays():
     """the captured step replays as a full step every time (queue counters are reset
     inside the graph), with eager work and synchronisation between replays"""
