@@ -186,3 +186,11 @@ def test_append_without_plan():
     app = _abi.PfSolveApplication()
     assert pf.PfSolveAppend(app, 0, _abi.PfSolveLaunchParams()) == _abi.PFSOLVE_ERROR_PLAN_NOT_INITIALIZED
     assert pf.PfSolveSync(app) == 0
+
+
+def test_wait_bound_setters_need_a_plan_or_batch():
+    app = _abi.PfSolveApplication()
+    lib = pf.lib()
+    assert lib.pfsolve_set_wait_polls(C.byref(app), -1) == _abi.PFSOLVE_ERROR_PLAN_NOT_INITIALIZED
+    assert lib.pfsolve_batch_set_wait_polls(None, -1) == _abi.PFSOLVE_ERROR_PLAN_NOT_INITIALIZED
+    assert lib.pfsolve_batch_sync(None, None) == _abi.PFSOLVE_ERROR_PLAN_NOT_INITIALIZED
