@@ -20,9 +20,12 @@
  *   PfSolveResult          PfSolve/pfSolve_Structs/pfSolve_Structs.h:450-543
  *   RUNTIME_/BLOCK_/JWT_   PfSolve/pfSolve_Structs/pfSolve_Structs.h:73-99
  *
- * Fields of the reference configuration that belong to its FFT machinery
- * (Bluestein/Rader tuning, convolution, R2C/DCT) are kept for source
- * compatibility and ignored.
+ * PfSolveConfiguration holds every field of the reference's HIP build
+ * (VKFFT_BACKEND=2), in its order and at its offsets, so that a caller's
+ * assignments compile unchanged; the fields of the reference's FFT machinery
+ * (Bluestein/Rader tuning, convolution, R2C/DCT, register boost, paging) are
+ * accepted and ignored.  Extension fields are appended after the last
+ * reference field.
  */
 #ifndef PFSOLVE_H
 #define PFSOLVE_H
@@ -181,6 +184,17 @@ typedef struct {
     void* loadApplicationString;
     uint64_t disableSetLocale;
 
+    /* Bluestein / Rader tuning of the reference's FFT machinery: ignored */
+    uint64_t fixMaxRadixBluestein;
+    uint64_t forceBluesteinSequenceSize;
+    uint64_t useCustomBluesteinPaddingPattern;
+    uint64_t* primeSizes;
+    uint64_t* paddedSizes;
+    uint64_t fixMinRaderPrimeMult;
+    uint64_t fixMaxRaderPrimeMult;
+    uint64_t fixMinRaderPrimeFFT;
+    uint64_t fixMaxRaderPrimeFFT;
+
     int numConsecutiveJWIterations;
     int useMultipleInputBuffers;
 
@@ -189,9 +203,23 @@ typedef struct {
     uint64_t fft_zeropad_right[3];
     uint64_t frequencyZeroPadding;
 
+    /* convolution controls of the FFT machinery: ignored */
     uint64_t performConvolution;
+    uint64_t conjugateConvolution;
+    uint64_t crossPowerSpectrumNormalization;
     uint64_t coordinateFeatures;
+    uint64_t matrixConvolution;
+    uint64_t symmetricKernel;
     uint64_t numberKernels;
+    uint64_t kernelConvolution;
+
+    /* register over-utilisation and paging of the FFT machinery: ignored */
+    uint64_t registerBoost;
+    uint64_t registerBoostNonPow2;
+    uint64_t registerBoost4Step;
+    uint64_t swapTo3Stage4Step;
+    uint64_t devicePageSize;
+    uint64_t localPageSize;
 
     /* filled from the device at plan creation (may be overridden) */
     uint64_t computeCapabilityMajor;
@@ -203,8 +231,13 @@ typedef struct {
     uint64_t sharedMemorySize;
     uint64_t sharedMemorySizePow2;
     uint64_t warpSize;
+    uint64_t halfThreads;
+    uint64_t allocateTempBuffer;
+    uint64_t reorderFourStep;
     int64_t maxCodeLength;
     int64_t maxTempLength;
+    uint64_t autoCustomBluesteinPaddingPattern;
+    uint64_t useRaderUintLUT;
     uint64_t vendorID;
     hipEvent_t* stream_event;
     uint64_t streamCounter;
@@ -363,10 +396,31 @@ typedef enum PfSolveResult {
 /* Opaque plan state of one application. */
 struct pfsolve_plan;
 
+/* The reference's application (pfSolve_Structs.h:1281-1341, HIP build) field for
+ * field; the FFT machinery's Bluestein / Rader members stay zero. */
 typedef struct {
     PfSolveConfiguration configuration;
     struct pfsolve_plan* localFFTPlan;   /* opaque; NULL before init and after delete */
+    struct pfsolve_plan* localFFTPlan_inverse;   /* always NULL: a JW plan has no inverse */
     char kernelName[200];
+    uint64_t actualNumBatches;           /* size[2] of the plan */
+    uint64_t firstAxis;
+    uint64_t lastAxis;
+    uint64_t useBluesteinFFT[3];
+    void* bufferRaderUintLUT[3][4];
+    void* bufferBluestein[3];
+    void* bufferBluesteinFFT[3];
+    void* bufferBluesteinIFFT[3];
+    uint64_t bufferRaderUintLUTSize[3][4];
+    uint64_t bufferBluesteinSize[3];
+    void* applicationBluesteinString[3];
+    uint64_t applicationBluesteinStringSize[3];
+    uint64_t numRaderFFTPrimes;
+    uint64_t rader_primes[30];
+    uint64_t rader_buffer_size[30];
+    void* raderFFTkernel[30];
+    uint64_t applicationStringOffsetRader;
+    uint64_t currentApplicationStringPos;
     uint64_t applicationStringSize;
     void* saveApplicationString;
 } PfSolveApplication;

@@ -86,7 +86,8 @@ class Plan:
 
     ``fields`` are PfSolveConfiguration fields (size=[N, B, 1] etc.);
     ``matrices`` and ``rhs`` are device pointers (ints) of the plan's
-    `buffer` and `outputBuffer`; ``stream`` an optional hipStream_t (int).
+    `buffer` and `outputBuffer`; ``stream`` an optional hipStream_t (int), or a
+    list of them: launches then go round robin over the streams (num_streams).
     """
 
     def __init__(self, matrices: int, rhs: int, stream: int | None = None, **fields):
@@ -107,9 +108,10 @@ class Plan:
         cfg.buffer = C.cast(self._buf, _abi.vpp)
         cfg.outputBuffer = C.cast(self._out, _abi.vpp)
         if stream is not None:
-            self._stream = (C.c_void_p * 1)(stream)
+            streams = list(stream) if isinstance(stream, (list, tuple)) else [stream]
+            self._stream = (C.c_void_p * len(streams))(*streams)
             cfg.stream = C.cast(self._stream, _abi.vpp)
-            cfg.num_streams = 1
+            cfg.num_streams = len(streams)
         self.app = _abi.PfSolveApplication()
         check(initializePfSolve(self.app, cfg), "initializePfSolve")
         self.alive = True
@@ -130,6 +132,10 @@ class Plan:
                 setattr(lp, k, v)
         check(PfSolveAppend(self.app, 0, lp), "PfSolveAppend")
         del self._keep[:-4]
+
+    def sync(self) -> None:
+        """PfSolveSync: joins the plan's streams; raises if a kernel's bounded wait ran out"""
+        check(PfSolveSync(self.app), "PfSolveSync")
 
     def geometry(self) -> _abi.PfSolveGeometry:
         g = _abi.PfSolveGeometry()

@@ -147,6 +147,15 @@ class PfSolveConfiguration(C.Structure):
         ("loadApplicationFromString", u64),
         ("loadApplicationString", vp),
         ("disableSetLocale", u64),
+        ("fixMaxRadixBluestein", u64),
+        ("forceBluesteinSequenceSize", u64),
+        ("useCustomBluesteinPaddingPattern", u64),
+        ("primeSizes", u64p),
+        ("paddedSizes", u64p),
+        ("fixMinRaderPrimeMult", u64),
+        ("fixMaxRaderPrimeMult", u64),
+        ("fixMinRaderPrimeFFT", u64),
+        ("fixMaxRaderPrimeFFT", u64),
         ("numConsecutiveJWIterations", i32),
         ("useMultipleInputBuffers", i32),
         ("performZeropadding", u64 * 3),
@@ -154,8 +163,19 @@ class PfSolveConfiguration(C.Structure):
         ("fft_zeropad_right", u64 * 3),
         ("frequencyZeroPadding", u64),
         ("performConvolution", u64),
+        ("conjugateConvolution", u64),
+        ("crossPowerSpectrumNormalization", u64),
         ("coordinateFeatures", u64),
+        ("matrixConvolution", u64),
+        ("symmetricKernel", u64),
         ("numberKernels", u64),
+        ("kernelConvolution", u64),
+        ("registerBoost", u64),
+        ("registerBoostNonPow2", u64),
+        ("registerBoost4Step", u64),
+        ("swapTo3Stage4Step", u64),
+        ("devicePageSize", u64),
+        ("localPageSize", u64),
         ("computeCapabilityMajor", u64),
         ("computeCapabilityMinor", u64),
         ("maxComputeWorkGroupCount", u64 * 3),
@@ -165,8 +185,13 @@ class PfSolveConfiguration(C.Structure):
         ("sharedMemorySize", u64),
         ("sharedMemorySizePow2", u64),
         ("warpSize", u64),
+        ("halfThreads", u64),
+        ("allocateTempBuffer", u64),
+        ("reorderFourStep", u64),
         ("maxCodeLength", i64),
         ("maxTempLength", i64),
+        ("autoCustomBluesteinPaddingPattern", u64),
+        ("useRaderUintLUT", u64),
         ("vendorID", u64),
         ("stream_event", vpp),
         ("streamCounter", u64),
@@ -215,7 +240,26 @@ class PfSolveApplication(C.Structure):
     _fields_ = [
         ("configuration", PfSolveConfiguration),
         ("localFFTPlan", vp),
+        ("localFFTPlan_inverse", vp),
         ("kernelName", C.c_char * 200),
+        ("actualNumBatches", u64),
+        ("firstAxis", u64),
+        ("lastAxis", u64),
+        ("useBluesteinFFT", u64 * 3),
+        ("bufferRaderUintLUT", (vp * 4) * 3),
+        ("bufferBluestein", vp * 3),
+        ("bufferBluesteinFFT", vp * 3),
+        ("bufferBluesteinIFFT", vp * 3),
+        ("bufferRaderUintLUTSize", (u64 * 4) * 3),
+        ("bufferBluesteinSize", u64 * 3),
+        ("applicationBluesteinString", vp * 3),
+        ("applicationBluesteinStringSize", u64 * 3),
+        ("numRaderFFTPrimes", u64),
+        ("rader_primes", u64 * 30),
+        ("rader_buffer_size", u64 * 30),
+        ("raderFFTkernel", vp * 30),
+        ("applicationStringOffsetRader", u64),
+        ("currentApplicationStringPos", u64),
         ("applicationStringSize", u64),
         ("saveApplicationString", vp),
     ]

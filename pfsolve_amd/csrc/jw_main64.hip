@@ -965,6 +965,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
         bool all_done = true, progress = false;
         for (int w = j; w < ncomp; w += nstore) {
             if (lds_flag(vfull + w)) {
+                compiler_fence();   // the pair index and staging reads stay after the flag read
                 const long long qq = __hip_atomic_load(&s_pair[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 const int64_t q = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)qq) |
                                   ((int64_t)__builtin_amdgcn_readfirstlane((int)(qq >> 32)) << 32);
@@ -1170,6 +1171,7 @@ __global__ __launch_bounds__(bsplit_threads<R>()) void jw_batch64s_kernel(const 
             bool all_done = true, progress = false;
             for (int w = j; w < ncomp; w += nstore) {
                 if (lds_flag(&s_full[w])) {
+                    compiler_fence();   // the pair index and staging reads stay after the flag read
                     const long long qq =
                         __hip_atomic_load(&s_pair[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     const int64_t qs = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)qq) |

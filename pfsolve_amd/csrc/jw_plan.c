@@ -121,6 +121,7 @@ void jw_keep_range(int64_t zp0, int64_t zp1, int runtime, int64_t N, int64_t* lo
     int64_t a = zp0, b = zp1;
     if (!runtime && b == 0) b = N;
     if (a < 0) a = 0;
+    if (a > N) a = N;
     if (b > N) b = N;
     if (b < a) b = a;
     *lo = a;
@@ -254,14 +255,17 @@ void jw_index_extent(const jw_source_desc* s, const jw_run_desc* r, int64_t* mat
     *rhs_max = (r->nsys > 0 && hi > 0) ? (r->nsys - 1) * r->stride + r->offSol + hi - 1 : -1;
 }
 
-int jw_offsets_valid(const jw_source_desc* s, const jw_run_desc* r)
+PfSolveResult jw_check_offsets(const jw_source_desc* s, const jw_run_desc* r)
 {
+    /* matrix positions: every diagonal the step reads starts inside the matrix buffer */
     if (s->mdA < 0 || s->udA < 0 || s->ldA < 0 || s->offV < 0 || s->ldS < 0 || s->udS < 0 ||
-        s->mdM < 0 || s->udM < 0 || r->stride < 0 || r->offSol < 0)
-        return 0;
+        s->mdS < -1 || s->mdM < 0 || s->udM < 0)
+        return PFSOLVE_ERROR_EMPTY_bufferSize;
+    if (r->stride < 0 || r->offSol < 0) return PFSOLVE_ERROR_EMPTY_outputBufferSize;
     /* systems must not overlap when solved in place */
-    if (r->nsys > 1 && r->stride < (r->in_hi > r->out_hi ? r->in_hi : r->out_hi)) return 0;
-    return 1;
+    if (r->nsys > 1 && r->stride < (r->in_hi > r->out_hi ? r->in_hi : r->out_hi))
+        return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+    return PFSOLVE_SUCCESS;
 }
 
 PfSolveResult jw_resolve_block(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp,

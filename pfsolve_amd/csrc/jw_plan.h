@@ -19,8 +19,9 @@ struct pfsolve_plan {
     int64_t N;
     int L, R, waves;
     int cu_count, occupancy;
-    void* table;              /* device: coefficient table */
+    void* table;              /* device: coefficient table, one per stream slot */
     size_t table_bytes;
+    size_t table_stride;      /* bytes between the slots' tables */
     int64_t lds_bytes;
     /* wave-per-system FP64 path (table format 2): rows per lane (0 = not available),
      * waves per workgroup and workgroups per CU */
@@ -52,7 +53,7 @@ PfSolveResult jw_resolve_gbmv(const PfSolveConfiguration* cfg, const PfSolveLaun
                               int64_t nsys, jw_gbmv_desc* d);
 /* largest matrix / RHS element index a JW launch touches, and sanity of offsets */
 void jw_index_extent(const jw_source_desc* s, const jw_run_desc* r, int64_t* mat_max, int64_t* rhs_max);
-int jw_offsets_valid(const jw_source_desc* s, const jw_run_desc* r);
+PfSolveResult jw_check_offsets(const jw_source_desc* s, const jw_run_desc* r);
 /* keep range [lo, hi) of a zero-pad pair, clipped to [0, N) */
 void jw_keep_range(int64_t zp0, int64_t zp1, int runtime, int64_t N, int64_t* lo, int64_t* hi);
 

@@ -57,6 +57,7 @@ __host__ __device__ __forceinline__ int ilog2_rt(int v)
 // ------------------------------------------------------------------------
 constexpr int kPrologueThreads = 512;
 constexpr int kProRows = 4;   // rows per thread: R * L <= 2048
+constexpr int kMaxRT = 8;     // rows per thread of the pivot scan (256 threads): R * L <= 2048
 
 // LDS of prologue_body: 5 arrays of R*L elements (ld -> l -> l', ud -> h -> h', D, E, g)
 // five arrays of RL values; the last (g) is at least 16 values long, since the
@@ -258,23 +259,115 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
             if (r0 >= 1) gprev = A::div(A::add(xc, xe), A::add(xa, xb));
         }
         __syncthreads();   // the totals are read: g's array is free
+        // first pass: the exact recurrence over this thread's rows from the scanned pivot;
+        // the rows' ld / ud stay in registers (nothing reads another thread's raw ld again)
+        T pr[kMaxRT], ldv[kMaxRT], udv[kMaxRT];
+        const T udp0 = udprev;
         if (part) {
-            for (int i = 0; i < RT; i++) {
+#pragma unroll
+            for (int i = 0; i < kMaxRT; i++) {
+                pr[i] = A::one();
+                ldv[i] = A::zero();
+                udv[i] = A::zero();
                 const int r = r0 + i;
-                if (r >= RL) break;
-                const T ldr = sl[r], udr = sh[r];
-                T l = A::zero(), g = A::zero(), h = A::zero();
+                if (i >= RT || r >= RL) continue;
+                ldv[i] = sl[r];
+                udv[i] = sh[r];
+                T g = A::zero();
                 if (r < N) {
-                    // l_r = ld_r / p_{r-1},  p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r,  h_r = ud_r g_r
-                    l = (r >= 1) ? A::mul(ldr, gprev) : A::zero();
-                    const T p = A::sub(A::one(), A::mul(l, udprev));
-                    g = A::div(A::one(), p);
-                    h = A::mul(udr, g);
+                    // p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r
+                    const T l = (r >= 1) ? A::mul(ldv[i], gprev) : A::zero();
+                    pr[i] = A::sub(A::one(), A::mul(l, udprev));
+                    g = A::div(A::one(), pr[i]);
                     gprev = g;
                 }
-                udprev = udr;
-                sl[r] = l;
+                udprev = udv[i];
                 sg[r] = g;
+            }
+        }
+        __syncthreads();   // every first-pass g is in sg
+
+        // Refinement.  Near a parabolic fixed point of the pivot map (m_r -> 1/4, e.g. the
+        // Gram matrices of Jones-Worland connection matrices at small degree) the scanned
+        // products lose accuracy like eps k^1.5 over k rows, where the sequential recurrence
+        // loses eps k^0.5.  The pivots p~ of the first pass have residuals
+        //   rho_r = p~_r - (1 - m_r / p~_{r-1})   (rounding inside a thread, the scan's error
+        // at its first row), and their errors e_r = p~_r - p_r obey, to first order,
+        //   e_r = rho_r + c_r e_{r-1},  c_r = m_r / p~_{r-1}^2,  e_0 = rho_0 = 0:
+        // an affine recurrence, solved by one more scan over the same partition (its own
+        // rounding is relative to e, far below eps); p_r = p~_r - e_r.
+        T rho[kMaxRT], cc[kMaxRT];
+        T sc = A::one(), ss = A::zero();      // this thread's map e -> sc e + ss
+        if (part) {
+            T gm = (r0 >= 1 && r0 < RL) ? sg[r0 - 1] : A::one();
+            udprev = udp0;
+#pragma unroll
+            for (int i = 0; i < kMaxRT; i++) {
+                rho[i] = A::zero();
+                cc[i] = A::zero();
+                const int r = r0 + i;
+                if (i >= RT || r >= N) continue;
+                const T m = (r >= 1) ? A::mul(ldv[i], udprev) : A::zero();
+                const T mg = A::mul(m, gm);
+                rho[i] = A::add(A::sub(pr[i], A::one()), mg);
+                cc[i] = A::mul(mg, gm);
+                ss = A::madd(rho[i], cc[i], ss);
+                sc = A::mul(cc[i], sc);
+                gm = sg[r];
+                udprev = udv[i];
+            }
+            // inclusive scan of the maps inside the wavefront, wavefront totals to LDS
+            // (sl's first 8 values: every thread holds its raw ld in registers by now)
+            for (int k = 0; k < 6; k++) {
+                const int sft = 1 << k;
+                const int src = lane >= sft ? lane - sft : lane;
+                const T pc = lane_fetch(sc, src), ps = lane_fetch(ss, src);
+                if (lane >= sft) {
+                    ss = A::madd(ss, sc, ps);
+                    sc = A::mul(sc, pc);
+                }
+            }
+            if (lane == 63) {
+                sl[2 * wv] = sc;
+                sl[2 * wv + 1] = ss;
+            }
+            const int src = lane >= 1 ? lane - 1 : lane;
+            const T xs = lane_fetch(ss, src), xc = lane_fetch(sc, src);
+            ss = lane == 0 ? A::zero() : xs;
+            sc = lane == 0 ? A::one() : xc;
+        }
+        __syncthreads();
+        if (part) {
+            // error entering this thread's first row: earlier wavefronts' maps, then this
+            // wavefront's exclusive part, applied to e = 0
+            T ein = A::zero();
+            for (int w = 0; w < wv; w++) ein = A::madd(sl[2 * w + 1], sl[2 * w], ein);
+            ein = A::madd(ss, sc, ein);
+#pragma unroll
+            for (int i = 0; i < kMaxRT; i++) {
+                const int r = r0 + i;
+                if (i >= RT || r >= N) continue;
+                ein = A::madd(rho[i], cc[i], ein);
+                // a correction that is not finite (no dominance, products out of range)
+                // is not applied
+                const double eh = A::hi(ein);
+                const T p = (eh - eh == 0.0) ? A::sub(pr[i], ein) : pr[i];
+                sg[r] = A::div(A::one(), p);
+            }
+        }
+        __syncthreads();   // the refined g of every row (and of the row before) is in sg
+        if (part) {
+#pragma unroll
+            for (int i = 0; i < kMaxRT; i++) {
+                const int r = r0 + i;
+                if (i >= RT || r >= RL) continue;
+                // l_r = ld_r g_{r-1},  h_r = ud_r g_r  (zero outside the matrix)
+                T l = A::zero(), h = A::zero();
+                if (r < N) {
+                    if (r >= 1) l = A::mul(ldv[i], sg[r - 1]);
+                    h = A::mul(udv[i], sg[r]);
+                }
+                sl[r] = l;
                 sh[r] = h;
             }
         }

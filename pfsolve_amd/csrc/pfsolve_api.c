@@ -16,9 +16,22 @@
 
 int jwdev_main_occupancy(int L, int R, int precision, int waves);
 
-static hipStream_t plan_stream(const PfSolveApplication* app)
+/* Stream slot of the next launch: with num_streams > 1 launches go round robin over
+ * stream[0..num_streams) (pfSolve_DispatchPlan.h:337-362), each slot with its own event
+ * and its own coefficient table, so that launches in flight on different streams never
+ * share device scratch. */
+static uint64_t plan_slot(const PfSolveApplication* app)
 {
-    return app->configuration.stream ? app->configuration.stream[0] : (hipStream_t)0;
+    const PfSolveConfiguration* c = &app->configuration;
+    return (c->num_streams > 1 && c->stream) ? c->streamID % c->num_streams : 0;
+}
+static hipStream_t slot_stream(const PfSolveApplication* app, uint64_t slot)
+{
+    return app->configuration.stream ? app->configuration.stream[slot] : (hipStream_t)0;
+}
+static void* slot_table(const struct pfsolve_plan* p, uint64_t slot)
+{
+    return p->table ? (char*)p->table + slot * p->table_stride : NULL;
 }
 
 static PfSolveResult fill_geometry(const PfSolveConfiguration* cfg, pfsolve_geometry* g)
@@ -75,17 +88,41 @@ void deletePfSolve(PfSolveApplication* app)
         free(p);
     }
     app->localFFTPlan = NULL;
+    PfSolveConfiguration* c = &app->configuration;
+    if (c->stream_event) {
+        for (uint64_t i = 0; i < c->num_streams; i++)
+            if (c->stream_event[i]) (void)hipEventDestroy(c->stream_event[i]);
+        free(c->stream_event);
+        c->stream_event = NULL;
+    }
 }
 
 PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in)
 {
     if (!app) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
-    app->localFFTPlan = NULL;
+    /* members after the configuration start zero (the FFT machinery's stay so) */
+    memset((char*)app + offsetof(PfSolveApplication, localFFTPlan), 0,
+           sizeof(*app) - offsetof(PfSolveApplication, localFFTPlan));
     /* HIP branch of setConfigurationPfSolve: a device pointer is required (pfSolve_InitializeApp.h:656-659) */
     if (in.device == NULL) return PFSOLVE_ERROR_INVALID_DEVICE;
     PfSolveResult res = jw_configure(&in, &app->configuration);
-    if (res != PFSOLVE_SUCCESS) { deletePfSolve(app); return res; }
     PfSolveConfiguration* cfg = &app->configuration;
+    /* stream bookkeeping belongs to the plan (pfSolve_InitializeApp.h:661-663, 738-748) */
+    cfg->stream_event = NULL;
+    cfg->streamID = 0;
+    cfg->streamCounter = 0;
+    if (res != PFSOLVE_SUCCESS) { deletePfSolve(app); return res; }
+    if (cfg->num_streams == 0) cfg->num_streams = 1;
+    if (cfg->num_streams > 1) {
+        if (!cfg->stream) { deletePfSolve(app); return PFSOLVE_ERROR_INVALID_QUEUE; }
+        cfg->stream_event = (hipEvent_t*)calloc(cfg->num_streams, sizeof(hipEvent_t));
+        if (!cfg->stream_event) { deletePfSolve(app); return PFSOLVE_ERROR_MALLOC_FAILED; }
+        for (uint64_t i = 0; i < cfg->num_streams; i++)
+            if (hipEventCreateWithFlags(&cfg->stream_event[i], hipEventDisableTiming) != hipSuccess) {
+                deletePfSolve(app);
+                return PFSOLVE_ERROR_FAILED_TO_CREATE_EVENT;
+            }
+    }
 
     struct pfsolve_plan* p = (struct pfsolve_plan*)calloc(1, sizeof(*p));
     if (!p) return PFSOLVE_ERROR_MALLOC_FAILED;
@@ -145,7 +182,12 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
         }
         if (hipMalloc((void**)&p->d_err, 256) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
         if (hipMemset(p->d_err, 0, 256) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_COPY; }
-        if (hipMalloc(&p->table, p->table_bytes) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
+        /* one table per stream slot */
+        p->table_stride = (p->table_bytes + 255) / 256 * 256;
+        if (hipMalloc(&p->table, p->table_stride * cfg->num_streams) != hipSuccess) {
+            deletePfSolve(app);
+            return PFSOLVE_ERROR_FAILED_TO_ALLOCATE;
+        }
         if (!long_n) {
             p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
             if (p->occupancy < 1) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY; }
@@ -220,7 +262,6 @@ static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveL
         if (cfg->bufferSize && mmax >= 0 && (uint64_t)((mmax + 1) * 8) > cfg->bufferSize[0])
             return PFSOLVE_ERROR_EMPTY_bufferSize;
     }
-    run->table = p->table;
     if (ncx <= 0) return PFSOLVE_SUCCESS;
     int64_t need = (ncx + p->split64 - 1) / p->split64;
     int64_t grid = (int64_t)p->cu_count * p->occupancy_split;
@@ -234,12 +275,11 @@ static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveL
     return PFSOLVE_SUCCESS;
 }
 
-PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchParams* lp)
+static PfSolveResult append_impl(PfSolveApplication* app, PfSolveLaunchParams* lp, uint64_t slot)
 {
-    (void)inverse;
-    if (!app || !app->localFFTPlan) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     struct pfsolve_plan* p = app->localFFTPlan;
     PfSolveConfiguration* cfg = &app->configuration;
+    void* const table = slot_table(p, slot);
     /* buffer swap and checks: PfSolve/pfSolve_PlanManagement/pfSolve_API_handles/pfSolve_UpdateBuffers.h:616-699 */
     if (lp) {
         if (lp->buffer) cfg->buffer = lp->buffer;
@@ -253,7 +293,7 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
     if (cfg->outputBuffer == NULL || cfg->outputBuffer[0] == NULL) return PFSOLVE_ERROR_EMPTY_outputBuffer;
     /* batchSize > 0 replaces size[1] (pfSolve_UpdateBuffers.h:1558) */
     if (lp && lp->batchSize > 0) cfg->size[1] = lp->batchSize;
-    hipStream_t s = plan_stream(app);
+    hipStream_t s = slot_stream(app, slot);
 
     if (p->kind == PF_KIND_JW) {
         const int64_t nsys = (int64_t)(cfg->size[1] * cfg->size[2]);
@@ -261,10 +301,13 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
         jw_run_desc run;
         PfSolveResult r = jw_resolve_launch(cfg, lp, nsys, &src, &run);
         if (r != PFSOLVE_SUCCESS) return r;
+        /* offsets and strides: no matrix diagonal or system may start before its buffer */
+        r = jw_check_offsets(&src, &run);
+        if (r != PFSOLVE_SUCCESS) return r;
+        run.table = table;
         if (cfg->inputBlock || cfg->outputBlock) return append_fused_blocks(app, lp, &src, &run, nsys, s);
         /* index guard: every element the kernels touch must lie inside the
          * buffers whose sizes the caller declared (bufferSize/outputBufferSize) */
-        if (!jw_offsets_valid(&src, &run)) return PFSOLVE_ERROR_EMPTY_outputBufferSize;
         {
             int64_t mmax, rmax;
             jw_index_extent(&src, &run, &mmax, &rmax);
@@ -275,14 +318,13 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
                 return PFSOLVE_ERROR_EMPTY_outputBufferSize;
         }
         run.rhs = cfg->outputBuffer[0];
-        run.table = p->table;
         const uintptr_t base = (uintptr_t)run.rhs;
         run.vec2 = p->precision == JW_PREC_F64 && (base % 16) == 0 && (run.stride % 2) == 0 && (run.offSol % 2) == 0;
         if (p->R64 && run.vec2) {
             /* wave-per-system kernel: table format 2, built by a prologue launch or,
              * in the compute / store split kernel, by every workgroup in LDS */
             if (nsys <= 0) return PFSOLVE_SUCCESS;
-            if (!p->split64 && jwdev_prologue(&src, cfg->buffer[0], p->table, 64, p->R64, p->precision, 2, s) != 0)
+            if (!p->split64 && jwdev_prologue(&src, cfg->buffer[0], table, 64, p->R64, p->precision, 2, s) != 0)
                 return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             /* wavefront passes: pairs of systems, single systems for R = 32 */
             const int64_t npass = p->R64 == 32 ? nsys : (nsys + 1) / 2;
@@ -305,7 +347,7 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
         /* long systems run only on the wave-per-system kernel, which needs 16-byte
          * aligned systems (even stride and offset, aligned buffer) */
         if (p->L == 0) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
-        if (jwdev_prologue(&src, cfg->buffer[0], p->table, p->L, p->R, p->precision, 1, s) != 0)
+        if (jwdev_prologue(&src, cfg->buffer[0], table, p->L, p->R, p->precision, 1, s) != 0)
             return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
         if (nsys <= 0) return PFSOLVE_SUCCESS;
         const int G = 64 / p->L;
@@ -369,14 +411,41 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
     return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
 }
 
-/* Waits for the plan's stream (pfSolve_RunApp.h:28-57 joins the plan's streams) and
- * reports a raised device error flag (a bounded wait inside a kernel ran out). */
+PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchParams* lp)
+{
+    (void)inverse;
+    if (!app || !app->localFFTPlan) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    PfSolveConfiguration* cfg = &app->configuration;
+    const uint64_t slot = plan_slot(app);
+    PfSolveResult r = append_impl(app, lp, slot);
+    if (r != PFSOLVE_SUCCESS || !cfg->stream_event) return r;
+    /* round robin: the slot's event marks its last launch, the next launch takes the
+     * next stream (pfSolve_DispatchPlan.h:355-362) */
+    if (hipEventRecord(cfg->stream_event[slot], slot_stream(app, slot)) != hipSuccess)
+        return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+    cfg->streamCounter++;
+    cfg->streamID = cfg->streamCounter % cfg->num_streams;
+    return PFSOLVE_SUCCESS;
+}
+
+/* Joins the plan's work and reports a raised device error flag (a bounded wait inside a
+ * kernel ran out).  With num_streams > 1 it waits for every stream's event and restarts
+ * the round robin, as pfSolve_RunApp.h:28-57 does; with one stream it synchronises that
+ * stream (the reference returns at once there; the wait is what lets the flag be read). */
 PfSolveResult PfSolveSync(PfSolveApplication* app)
 {
     if (!app) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     struct pfsolve_plan* p = app->localFFTPlan;
     if (!p) return PFSOLVE_SUCCESS;   /* as the reference: nothing to join */
-    if (hipStreamSynchronize(plan_stream(app)) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+    PfSolveConfiguration* cfg = &app->configuration;
+    if (cfg->stream_event) {
+        for (uint64_t i = 0; i < cfg->num_streams; i++)
+            if (hipEventSynchronize(cfg->stream_event[i]) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+        cfg->streamCounter = 0;
+        cfg->streamID = 0;
+    } else if (hipStreamSynchronize(slot_stream(app, 0)) != hipSuccess) {
+        return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
+    }
     if (p->d_err) {
         int e = 0;
         if (hipMemcpy(&e, p->d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
@@ -540,6 +609,11 @@ static const field_off k_cfg_fields[] = {
     OFF(PfSolveConfiguration, inputBufferStride), OFF(PfSolveConfiguration, outputBufferStride),
     OFF(PfSolveConfiguration, keepShaderCode), OFF(PfSolveConfiguration, disableCaching),
     OFF(PfSolveConfiguration, numConsecutiveJWIterations), OFF(PfSolveConfiguration, useMultipleInputBuffers),
+    OFF(PfSolveConfiguration, fixMaxRadixBluestein), OFF(PfSolveConfiguration, fixMaxRaderPrimeFFT),
+    OFF(PfSolveConfiguration, kernelConvolution), OFF(PfSolveConfiguration, registerBoost),
+    OFF(PfSolveConfiguration, localPageSize), OFF(PfSolveConfiguration, halfThreads),
+    OFF(PfSolveConfiguration, reorderFourStep), OFF(PfSolveConfiguration, useRaderUintLUT),
+    OFF(PfSolveConfiguration, vendorID), OFF(PfSolveConfiguration, useStrict32BitAddress),
     OFF(PfSolveConfiguration, offsetT), OFF(PfSolveConfiguration, autotune),
     OFF(PfSolveConfiguration, kernelVariant), OFF(PfSolveConfiguration, inputBlock),
     OFF(PfSolveConfiguration, outputBlock), OFF(PfSolveConfiguration, inputBlockLshift),

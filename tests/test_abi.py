@@ -194,3 +194,54 @@ def test_wait_bound_setters_need_a_plan_or_batch():
     assert lib.pfsolve_set_wait_polls(C.byref(app), -1) == _abi.PFSOLVE_ERROR_PLAN_NOT_INITIALIZED
     assert lib.pfsolve_batch_set_wait_polls(None, -1) == _abi.PFSOLVE_ERROR_PLAN_NOT_INITIALIZED
     assert lib.pfsolve_batch_sync(None, None) == _abi.PFSOLVE_ERROR_PLAN_NOT_INITIALIZED
+
+
+def _ref_fields(ref, which):
+    ref.ref_field.restype = C.c_char_p
+    ref.ref_field.argtypes = [C.c_int, C.c_long, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+    ref.ref_struct_size.restype = C.c_long
+    ref.ref_field_count.restype = C.c_long
+    out = []
+    for i in range(ref.ref_field_count(which)):
+        o, s = C.c_long(), C.c_long()
+        out.append((ref.ref_field(which, i, C.byref(o), C.byref(s)).decode(), o.value, s.value))
+    return out, ref.ref_struct_size(which)
+
+
+@pytest.mark.parametrize("which,struct,first_ext", [(0, "PfSolveConfiguration", "offsetT"),
+                                                   (1, "PfSolveLaunchParams", "offsetT"),
+                                                   (2, "PfSolveApplication", None)])
+def test_struct_layout_matches_reference_hip_build(which, struct, first_ext):
+    """Every member of the reference's HIP build of the struct (compiled from the reference
+    headers under oracle/_ref) exists here under the same name, at the same offset and with the
+    same size; the reference's members are listed completely (no gap of 8 or more bytes between
+    them or before the end of the struct); the extension members start right after them (in the
+    launch parameters, in the tail padding that the long double scaleC gives the struct)."""
+    if not os.path.exists(O.REF_SO):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    ref = C.CDLL(O.REF_SO)
+    fields, size = _ref_fields(ref, which)
+    end = 0
+    for name, off, sz in fields:
+        assert off - end < 8, ("member missing from the reference list before", name)
+        end = off + sz
+    assert size - end < (16 if which == 1 else 8)
+    mine = getattr(_abi, struct)
+    # the application embeds the configuration, whose extension members shift what follows
+    shift = C.sizeof(_abi.PfSolveConfiguration) - ref.ref_struct_size(0) if which == 2 else 0
+    for name, off, sz in fields:
+        assert hasattr(mine, name), name
+        d = getattr(mine, name)
+        if name == "configuration":
+            assert (d.offset, d.size) == (off, sz + shift)
+        else:
+            assert (d.offset, d.size) == (off + shift if off else 0, sz), (name, d.offset, d.size, off, sz)
+    if first_ext:
+        assert getattr(mine, first_ext).offset == end
+    else:
+        assert C.sizeof(mine) == size + shift
+    # the C header agrees with the mirror
+    lib = pf.lib()
+    sizes = {0: lib.pfsolve_sizeof_configuration, 1: lib.pfsolve_sizeof_launch_params,
+             2: lib.pfsolve_sizeof_application}
+    assert sizes[which]() == C.sizeof(mine)

@@ -315,3 +315,10 @@ def test_fused_blocks_refusals():
         _abi.PFSOLVE_ERROR_EMPTY_outputBufferSize
     assert append_code(dict(r, outputBufferStride=[65]), dict(inputBuffer=inp.data_ptr())) == \
         _abi.PFSOLVE_ERROR_EMPTY_outputBufferSize
+    # matrix blocks before the start of the matrix buffer: refused on the host, nothing launches
+    # (a baked negative offset, and runtime offsets whose unsigned value wraps to a negative one)
+    assert append_code(dict(f, offsetM=-8), dict(inputBuffer=inp.data_ptr())) == _abi.PFSOLVE_ERROR_EMPTY_bufferSize
+    assert append_code(dict(f, offsetV=-1), dict(inputBuffer=inp.data_ptr())) == _abi.PFSOLVE_ERROR_EMPTY_bufferSize
+    rt = dict(f, jw_control_bitmask=_abi.RUNTIME_OFFSETT | _abi.RUNTIME_OFFSETM)
+    for off in (dict(offsetT=-64, offsetM=0), dict(offsetT=0, offsetM=(1 << 64) - 16)):
+        assert append_code(rt, dict(inputBuffer=inp.data_ptr(), **off)) == _abi.PFSOLVE_ERROR_EMPTY_bufferSize

@@ -466,3 +466,42 @@ def test_reference_modes_wave_per_system(jw_type, ub, N):
                   run_scaleC=1.25, run_izp=[1, N - 1])
     want = O.jw_f128(p, buf, x)
     check_systems(got, want, x, B, S, 0, N, 0, N, TOL64)
+
+
+# ---------------------------------------------------------------- several streams
+@pytest.mark.parametrize("N,variant", [(100, 1), (512, 0), (1000, 0)])
+def test_round_robin_streams(N, variant):
+    """num_streams = 3: launches go round robin over the caller's streams (the stream
+    index advances per launch and PfSolveSync restarts it), each launch with its own
+    right-hand sides and its own matrix set (runtime offsets), all in flight together;
+    every result matches the oracle, as each slot has its own coefficient table."""
+    import torch
+    B, K = 6, 7
+    sets = [fused_buffers(N, 300 + k, 1) for k in range(2)]
+    # both matrix sets in one buffer, the second behind the first
+    span = sets[0][1].size
+    mat = np.concatenate([sets[0][1], sets[1][1]])
+    offs = [sets[0][2], tuple(o + span for o in sets[1][2])]
+    xs = [O.uniform(400 + k, -1, 1, B * N) for k in range(K)]
+    dmat = torch.from_numpy(mat).cuda()
+    drhs = [torch.from_numpy(x.copy()).cuda() for x in xs]
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    torch.cuda.synchronize()
+    rt = _abi.RUNTIME_OFFSETM | _abi.RUNTIME_OFFSETV | _abi.RUNTIME_OFFSETT
+    with pf.Plan(dmat.data_ptr(), drhs[0].data_ptr(), stream=[s.cuda_stream for s in streams],
+                 size=[N, B, 1], jw_type=14, doublePrecision=1, scaleC=1.0, KL=2, KU=2,
+                 jw_control_bitmask=rt, kernelVariant=variant) as plan:
+        seen = []
+        for k in range(K):
+            oV, oT, oM = offs[k % 2]
+            seen.append(plan.app.configuration.streamID)
+            plan.append(outputBuffer=drhs[k].data_ptr(), offsetV=oV, offsetT=oT, offsetM=oM)
+        plan.sync()
+        assert seen == [k % 3 for k in range(K)]
+        assert plan.app.configuration.streamID == 0
+    for k in range(K):
+        oV, oT, oM = offs[k % 2]
+        p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0, mask=rt,
+                      run_offsetV=oV, run_offsetT=oT, run_offsetM=oM)
+        want = O.jw_f128(p, mat, xs[k])
+        check_systems(drhs[k].cpu().numpy(), want, xs[k], B, N, 0, N, 0, N, TOL64)

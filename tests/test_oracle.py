@@ -422,3 +422,126 @@ def test_reference_tridiagonal_fixture_residual():
     assert np.max(np.abs(res)) / np.max(np.abs(x)) < 1e-14
     y64 = O.jw_f64(p, buf, x)
     assert O.relerr_norm(y64, y) < 1e-14
+
+
+# ---------------------------------------------------------------- dgbmv against BLAS
+@pytest.mark.parametrize("KL,KU", [(2, 2), (1, 0), (0, 3), (3, 1), (0, 0), (4, 2)])
+@pytest.mark.parametrize("N", [1, 2, 5, 17, 64])
+def test_dgbmv_matches_blas_dgbmv(KL, KU, N):
+    """The band matvec of the dgbmv plan (PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/
+    pfSolve_MatVecMul.h:383-446: y_r = sum_j AB[j*N + c] x_c, c = r + KU - j) is LAPACK's
+    dgbmv on a band array stored row by row (AB[j*N + c] = A(c - KU + j, c)): checked against
+    the BLAS routine itself (scipy.linalg.blas.dgbmv), with the plan's keep-ranges, offsets and
+    strides applied around it.  A band read in the wrong direction (r - KL + j) or a swapped
+    KL/KU fails for every KL != KU case."""
+    from scipy.linalg.blas import dgbmv
+    LDA = KL + KU + 1
+    B, S, offS, offM = 3, N + 5, 2, 7
+    buf = O.uniform(40 + N, -1, 1, offM + LDA * N + 3)
+    ab = buf[offM:offM + LDA * N].reshape(LDA, N)
+    x = O.uniform(41 + N, -1, 1, B * S + offS + 4)
+    izp, ozp = (N // 3, N), (1 if N > 1 else 0, N - N // 4)
+    p = O.problem(jw_type=10, N=N, B=B, mask=RT["M"] | RT["SOL"] | RT["OSTR"] | RT["IZP"] | RT["OZP"],
+                  cfg_outStride=N, run_offsetM=offM, run_offsetSolution=offS, run_outStride=S,
+                  run_izp=list(izp), run_ozp=list(ozp))
+    for fn, tol in (("pfo_dgbmv_f64", 4e-16), ("pfo_dgbmv_f128", 4e-16)):
+        out = x.copy()
+        assert getattr(lib(), fn)(C.byref(p), LDA, KU, KL, O.P(buf), O.P(out)) == 0
+        for b in range(B):
+            base = b * S + offS
+            xin = x[base:base + N].copy()
+            xin[:izp[0]] = 0
+            xin[izp[1]:] = 0
+            # scipy's wrapper wants at least LDA rows; below that the dense definition stands in
+            want = dgbmv(N, N, KL, KU, 1.0, ab, xin) if N >= LDA else _band_dense(ab.ravel(), N, KL, KU) @ xin
+            np.testing.assert_allclose(want, _band_dense(ab.ravel(), N, KL, KU) @ xin, rtol=0, atol=1e-15 * LDA)
+            keep = np.zeros(N, bool)
+            keep[ozp[0]:ozp[1]] = True
+            got = out[base:base + N]
+            np.testing.assert_allclose(got[keep], want[keep], rtol=0, atol=tol * LDA)
+            np.testing.assert_array_equal(got[~keep], x[base:base + N][~keep])
+        mask = np.ones(x.size, bool)
+        for b in range(B):
+            mask[b * S + offS:b * S + offS + N] = False
+        np.testing.assert_array_equal(out[mask], x[mask])
+
+
+def _band_dense(ab, N, KL, KU):
+    """Dense A from LAPACK's band definition A(i, c) = AB(KU + i - c, c) for
+    max(0, c - KU) <= i <= min(N - 1, c + KL)."""
+    A = np.zeros((N, N))
+    for c in range(N):
+        for i in range(max(0, c - KU), min(N - 1, c + KL) + 1):
+            A[i, c] = ab[(KU + i - c) * N + c]
+    return A
+
+
+def test_dgbmv_dense_band_definition():
+    """Same band, dense brute force from LAPACK's definition, against the __float128 oracle."""
+    N, KL, KU = 9, 3, 1
+    LDA = KL + KU + 1
+    ab = O.uniform(11, -1, 1, LDA * N)
+    A = _band_dense(ab, N, KL, KU)
+    x = O.uniform(12, -1, 1, N)
+    p = O.problem(jw_type=10, N=N, B=1, cfg_outStride=N)
+    out = x.copy()
+    assert lib().pfo_dgbmv_f128(C.byref(p), LDA, KU, KL, O.P(ab), O.P(out)) == 0
+    np.testing.assert_allclose(out, A @ x, rtol=0, atol=1e-15)
+    assert np.count_nonzero(A) == sum(min(N - 1, c + KL) - max(0, c - KU) + 1 for c in range(N))
+
+
+def test_block_layout_6_real_to_imaginary():
+    """Layout 6 reads a real system into the imaginary part of a complex element (re = 0)
+    (pfSolve_ReadWrite.h:2342-2344 read side, :2330-2331 write side): layout 5 of the real
+    parts plus accumulated layout 6 of the imaginary parts rebuilds the complex data, and
+    layouts 2 / 3 take it apart again.  A swapped re/im fails both checks."""
+    N, B = 21, 4
+    z = O.uniform(3, -1, 1, 2 * N * B)
+    zc = z.reshape(B, N, 2)
+    re = np.ascontiguousarray(zc[:, :, 0].ravel())
+    im = np.ascontiguousarray(zc[:, :, 1].ravel())
+    only_im = _block(6, N, B, im, np.full(2 * N * B, 9.0)).reshape(B, N, 2)
+    assert not only_im[..., 0].any()
+    np.testing.assert_array_equal(only_im[..., 1], zc[:, :, 1])
+    both = _block(20 + 6, N, B, im, _block(5, N, B, re, np.zeros(2 * N * B)))
+    np.testing.assert_array_equal(both, z)
+    np.testing.assert_array_equal(_block(3, N, B, both, np.zeros(N * B)), im)
+    # with the ScaleC operator on the way in: the imaginary part is scaled, re stays 0
+    sc = _block(106, N, B, im, np.zeros(2 * N * B), scale=0.75).reshape(B, N, 2)
+    np.testing.assert_array_equal(sc[..., 1], 0.75 * zc[:, :, 1])
+    assert not sc[..., 0].any()
+
+
+# ---------------------------------------------------------------- the reference's own systems
+@pytest.mark.parametrize("jw,ub", [(0, 1), (0, 0), (2, 1), (2, 0)])
+@pytest.mark.parametrize("N,l", [(64, 128), (200, 400), (200, 1), (65, 3)])
+@pytest.mark.parametrize("rt", [0, 1])
+def test_connection_matrix_modes_against_dense(jw, ub, N, l, rt):
+    """Jones-Worland connection matrices (mu/nu closed forms of the reference harness) in the
+    matvec + solve (jw 10) and solve-only (jw 12) modes: the oracle's __float128 step against a
+    dense LAPACK solve of the same operator, and its FP64 step against both."""
+    import jw_systems as J
+    offM, offV = (11, 3 * N + 5) if rt else (0, 0)
+    buf, A, S = J.mode_buffer(jw, ub, N, bool(rt), offM, offV, l, natural=False)
+    x = O.uniform(9 + N, -1, 1, N)
+    p = O.problem(jw_type=10 + jw, upperBanded=ub, N=N, B=1, mask=(RT["M"] | RT["V"]) if rt else 0,
+                  cfg_outStride=N, cfg_scaleC=1.0, run_offsetM=offM, run_offsetV=offV)
+    want = np.linalg.solve(S, A @ x)
+    assert O.relerr_norm(O.jw_f128(p, buf, x), want) < 1e-14
+    assert O.relerr_norm(O.jw_f64(p, buf, x), want) < 1e-14
+
+
+@pytest.mark.parametrize("N,l", [(64, 128), (300, 1), (513, 8)])
+def test_gram_tridiagonal_oracle_residual(N, l):
+    """T = C^T C of a connection matrix (critical pivot recurrence for small l): the oracle's
+    __float128 step solves the raw system to a residual at FP64 rounding of the stored
+    coefficients, and its FP64 step agrees with it to the condition number's allowance."""
+    import jw_systems as J
+    a, b, c = J.gram_tridiagonal(N, l)
+    buf = J.tridiag_buffer(a, b, c, natural=False)
+    x = O.uniform(3 + N, -1, 1, N)
+    p = O.problem(jw_type=13, upperBanded=2, N=N, B=1, cfg_outStride=N, cfg_scaleC=1.0)
+    yq = O.jw_f128(p, buf, x)
+    assert J.tridiag_residual(a, b, c, yq, x) < 1e-14 * max(1.0, np.max(np.abs(yq)) * np.max(np.abs(b)))
+    cond = np.linalg.cond(np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1), np.inf)
+    assert O.relerr_norm(O.jw_f64(p, buf, x), yq) < 4 * cond * 2.0 ** -53
