@@ -285,6 +285,9 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 sg[r] = g;
             }
         }
+        // sh[0] becomes the refinement flag: its raw ud is in thread 0's registers by now,
+        // and the one other reader (the next thread's udprev) read it before the scan
+        if (tid == 0) sh[0] = A::zero();
         __syncthreads();   // every first-pass g is in sg
 
         // Refinement.  Near a parabolic fixed point of the pivot map (m_r -> 1/4, e.g. the
@@ -296,9 +299,20 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         //   e_r = rho_r + c_r e_{r-1},  c_r = m_r / p~_{r-1}^2,  e_0 = rho_0 = 0:
         // an affine recurrence, solved by one more scan over the same partition (its own
         // rounding is relative to e, far below eps); p_r = p~_r - e_r.
+        // Inside a thread rho is one rounding; the scan's error shows at a thread's first
+        // row.  The refinement runs only when some first-row residual exceeds a few ulps
+        // (the flag is raised by any thread, read by all: the choice is workgroup-uniform
+        // and a function of the matrix alone, so every kernel still builds the same table).
+        if (part && r0 >= 1 && r0 < N) {
+            const double tol = sizeof(T) == 8 ? 0x1p-48 : 0x1p-100;
+            const T rho0 = A::add(A::sub(pr[0], A::one()), A::mul(A::mul(ldv[0], udp0), sg[r0 - 1]));
+            if (!(fabs(A::hi(rho0)) <= tol * fabs(A::hi(pr[0])))) sh[0] = A::one();
+        }
+        __syncthreads();
+        const bool refine = A::hi(sh[0]) != 0.0;
         T rho[kMaxRT], cc[kMaxRT];
         T sc = A::one(), ss = A::zero();      // this thread's map e -> sc e + ss
-        if (part) {
+        if (refine && part) {
             T gm = (r0 >= 1 && r0 < RL) ? sg[r0 - 1] : A::one();
             udprev = udp0;
 #pragma unroll
@@ -336,8 +350,8 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
             ss = lane == 0 ? A::zero() : xs;
             sc = lane == 0 ? A::one() : xc;
         }
-        __syncthreads();
-        if (part) {
+        if (refine) __syncthreads();
+        if (refine && part) {
             // error entering this thread's first row: earlier wavefronts' maps, then this
             // wavefront's exclusive part, applied to e = 0
             T ein = A::zero();
@@ -355,7 +369,9 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 sg[r] = A::div(A::one(), p);
             }
         }
-        __syncthreads();   // the refined g of every row (and of the row before) is in sg
+        // refined: the corrected g of every row (and of the row before) is in sg; not
+        // refined: every thread has read the flag before thread 0 writes h_0 over it
+        __syncthreads();
         if (part) {
 #pragma unroll
             for (int i = 0; i < kMaxRT; i++) {
