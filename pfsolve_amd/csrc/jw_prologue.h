@@ -304,7 +304,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         // (the flag is raised by any thread, read by all: the choice is workgroup-uniform
         // and a function of the matrix alone, so every kernel still builds the same table).
         if (part && r0 >= 1 && r0 < N) {
-            const double tol = sizeof(T) == 8 ? 0x1p-48 : 0x1p-100;
+            const double tol = sizeof(T) == 8 ? 0x1p-51 : 0x1p-103;
             const T rho0 = A::add(A::sub(pr[0], A::one()), A::mul(A::mul(ldv[0], udp0), sg[r0 - 1]));
             if (!(fabs(A::hi(rho0)) <= tol * fabs(A::hi(pr[0])))) sh[0] = A::one();
         }

@@ -245,3 +245,12 @@ def test_struct_layout_matches_reference_hip_build(which, struct, first_ext):
     sizes = {0: lib.pfsolve_sizeof_configuration, 1: lib.pfsolve_sizeof_launch_params,
              2: lib.pfsolve_sizeof_application}
     assert sizes[which]() == C.sizeof(mine)
+
+
+def test_c_caller_compiles_against_the_header(tmp_path):
+    """A C11 caller written the way the reference harness drives a plan (fields set by
+    name, including ignored FFT-machinery fields) compiles warning-free against
+    include/pfSolve.h and links libpfsolve_gfx950.so (run on the GPU in test_c_caller_gpu)."""
+    import c_caller
+    exe = c_caller.build(str(tmp_path))
+    assert os.path.exists(exe)

@@ -76,14 +76,21 @@ class Case:
                      outputBlockScale=self.out_scale)
         return f
 
-    def run_fused(self):
+    def run_fused(self, runtime_izp=False):
+        """runtime_izp: the input keep-range comes from the launch parameters
+        (RUNTIME_INPUTZEROPAD, as the benchmark passes it), the plan's own is unset"""
         t = torch()
         dmat = t.from_numpy(self.mat).cuda()
         din = t.from_numpy(self.x).cuda()
         dout = t.from_numpy(self.prev).cuda()
         stream = t.cuda.current_stream().cuda_stream
-        with pf.Plan(dmat.data_ptr(), dout.data_ptr(), stream=stream, **self.fused_fields()) as p:
-            p.append(inputBuffer=din.data_ptr())
+        fields = self.fused_fields()
+        launch = dict(inputBuffer=din.data_ptr())
+        if runtime_izp:
+            fields.update(jw_control_bitmask=_abi.RUNTIME_INPUTZEROPAD, inputZeropad=[0, 0])
+            launch.update(inputZeropad=self.izp)
+        with pf.Plan(dmat.data_ptr(), dout.data_ptr(), stream=stream, **fields) as p:
+            p.append(**launch)
             t.cuda.synchronize()
         return dout.cpu().numpy()
 
@@ -264,14 +271,17 @@ def test_fused_blocks_full_size_equals_chain():
 
 def test_fused_blocks_headline_workload():
     """N = 512, 65536 complex systems read through block 104 (ScaleC 0.75), input
-    keep-range [480, 512), packed real systems out: bit for bit against the chain
-    (block 104 -> JW step), sampled systems against the oracle composition"""
+    keep-range [480, 512) baked and as a runtime launch parameter, packed real systems
+    out: bit for bit against the chain (block 104 -> JW step), sampled systems against
+    the oracle composition"""
     c = Case(512, 65536, 1, 0, 0, seed=9, izp=(480, 512), real_out=True, in_scale=0.75)
     got = c.run_fused()
     chain = c.run_chain()
     out = c.outside_mask()
     np.testing.assert_array_equal(got[~out], chain[~out])
     np.testing.assert_array_equal(got[out], c.prev[out])
+    # the benchmark's form: the same keep-range as a runtime launch parameter
+    np.testing.assert_array_equal(c.run_fused(runtime_izp=True), got)
     small = Case(512, 3, 1, 0, 0, seed=9, izp=(480, 512), real_out=True, in_scale=0.75)
     small.x = c.x[:small.x.size].copy()
     small.prev = c.prev[:small.prev.size].copy()
