@@ -60,6 +60,25 @@ int pfsolve_kernel_supported(int lanes, int rows_per_lane, int precision);
 /* last device-side error text of the calling thread */
 const char* pfsolve_last_error(void);
 
+/* One plan's step without allocating: the work of initializePfSolve(configuration) +
+ * PfSolveAppend(launchParams) + deletePfSolve, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference, whose plans always allocate.  The
+ * coefficient table and the device error flag live in a static device array of the
+ * library (pfsolve_noalloc_scratch_bytes(), 144 KiB + 256 B per GPU, held until the
+ * process ends); configurations whose table would not fit (none of the supported ones
+ * today) return PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER.  Nothing is allocated on the
+ * device or the host (no hipMalloc, hipFree, hipEventCreate, malloc).  Runs on
+ * configuration->stream[0] (or the null stream); num_streams > 1 is refused
+ * (PFSOLVE_ERROR_INVALID_QUEUE).  All calls on one GPU share its scratch: calls whose
+ * launches overlap on that GPU (different streams, or host threads) race; calls in
+ * one stream are ordered.  The error flag is cleared by each call, so a bounded wait
+ * that ran out is not reported (PfSolveSync has no plan to read it from).  Results are
+ * bit-identical to PfSolveAppend of the same configuration and launch parameters. */
+PfSolveResult pfsolve_append_noalloc(const PfSolveConfiguration* configuration,
+                                     const PfSolveLaunchParams* launchParams);
+size_t pfsolve_noalloc_scratch_bytes(void);
+
 /* Bounded waits of the compute / store split kernel: polls before a wait gives up
  * and raises the device error flag that PfSolveSync reports (0: the default, 2^22;
  * negative: none, so every launch takes the time-out path at once and still

@@ -388,6 +388,10 @@ def declare(lib: C.CDLL) -> None:
 
 
 def declare_ext(lib: C.CDLL) -> None:
+    lib.pfsolve_append_noalloc.argtypes = [C.POINTER(PfSolveConfiguration), C.POINTER(PfSolveLaunchParams)]
+    lib.pfsolve_append_noalloc.restype = C.c_int
+    lib.pfsolve_noalloc_scratch_bytes.argtypes = []
+    lib.pfsolve_noalloc_scratch_bytes.restype = C.c_size_t
     lib.pfsolve_set_kernel_events.argtypes = [C.POINTER(PfSolveApplication), C.c_void_p, C.c_void_p]
     lib.pfsolve_set_kernel_events.restype = C.c_int
     lib.pfsolve_set_wait_polls.argtypes = [C.POINTER(PfSolveApplication), C.c_int32]

@@ -31,6 +31,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <mutex>
+
 #include "jw_common.h"
 #include "jw_prologue.h"
 
@@ -447,7 +449,40 @@ static const void* pick_main(int prec, int L, int R)
     return nullptr;
 }
 
+// ------------------------------------------------------------------------
+// device scratch of the allocation-free entry point (pfsolve_append_noalloc):
+// the coefficient table of one launch and its device error flag, one copy per
+// GPU for the life of the process.  Host code never passes `scratch` itself to
+// a kernel: its device address per GPU comes from hipGetSymbolAddress.
+// ------------------------------------------------------------------------
+constexpr size_t kNoAllocTableBytes = 144 * 1024;   // largest table: R = 32, 140 KiB
+constexpr size_t kNoAllocBytes = kNoAllocTableBytes + 256;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static void* g_scratch_ptr[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
 extern "C" {
+
+/* Device address of this GPU's scratch (the current device), resolved once per device. */
+int jwdev_noalloc_scratch(void** table, int** err, size_t* table_bytes)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { set_err("hipGetDevice", e); return -1; }
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return -1;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) { set_err("hipGetSymbolAddress(scratch)", e); return -2; }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *table = g_scratch_ptr[dev];
+    *err = reinterpret_cast<int*>(static_cast<unsigned char*>(g_scratch_ptr[dev]) + kNoAllocTableBytes);
+    *table_bytes = kNoAllocTableBytes;
+    return 0;
+}
 
 const char* jwdev_last_error(void) { return g_err; }
 

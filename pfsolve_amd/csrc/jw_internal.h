@@ -23,6 +23,7 @@
 #ifndef PFSOLVE_JW_INTERNAL_H
 #define PFSOLVE_JW_INTERNAL_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -170,6 +171,8 @@ int jwdev_block(const jw_block_desc* d, void* stream);
 int jwdev_gbmv(const jw_gbmv_desc* d, void* stream);
 int jwdev_device_info(int* cu_count, int* lds_per_block, int* arch_major, int* arch_minor);
 const char* jwdev_last_error(void);
+/* this GPU's scratch of the allocation-free entry point: table, error flag, table capacity */
+int jwdev_noalloc_scratch(void** table, int** err, size_t* table_bytes);
 
 #ifdef __cplusplus
 }

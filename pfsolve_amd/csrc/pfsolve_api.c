@@ -97,6 +97,83 @@ void deletePfSolve(PfSolveApplication* app)
     }
 }
 
+/* Everything a plan needs that allocates nothing: kind, precision, device limits,
+ * geometry and kernel choice (pfSolve_InitializeApp.h:1480-1644 decides these while it
+ * generates code).  The caller provides the device table and error flag. */
+static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
+{
+    PfSolveConfiguration* cfg = &app->configuration;
+    PfSolveResult res = PFSOLVE_SUCCESS;
+    p->kind = jw_plan_kind(cfg);
+    p->precision = jw_precision(cfg);
+    p->N = cfg->M_size;
+
+    int cu = 0, lds_blk = 0, maj = 0, min = 0;
+    if (jwdev_device_info(&cu, &lds_blk, &maj, &min) != 0) return PFSOLVE_ERROR_INVALID_DEVICE;
+    p->cu_count = cu;
+    cfg->computeCapabilityMajor = (uint64_t)maj;
+    cfg->computeCapabilityMinor = (uint64_t)min;
+    cfg->sharedMemorySize = (uint64_t)lds_blk;
+    cfg->maxThreadsNum = 1024;
+    if (p->kind == PF_KIND_JW) {
+        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_jw_%llu_%llu_%d_%d_%lld_%llu_%d_%d",
+                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->upperBanded,
+                 cfg->jw_type, (long long)cfg->jw_control_bitmask, (unsigned long long)cfg->Msplit[0],
+                 cfg->numConsecutiveJWIterations, cfg->useMultipleInputBuffers);
+        pfsolve_geometry g;
+        res = fill_geometry(cfg, &g);
+        if (res != PFSOLVE_SUCCESS) return res;
+        p->L = g.lanes;
+        p->R = g.rows_per_lane;
+        p->waves = g.waves;
+        p->lds_bytes = g.lds_bytes;
+        p->table_bytes = (size_t)g.table_bytes;
+        const int kp = p->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD;
+        const int long_n = p->L == 0;   /* N > 1024: the wave-per-system long kernel only */
+        if (long_n && cfg->kernelVariant == 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        if (!long_n && !jwdev_supported(p->L, p->R, kp)) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        /* wave-per-system path: FP64, 64 lanes, R a power of two in [2, 32] */
+        if (p->precision == JW_PREC_F64 && cfg->kernelVariant != 1) {
+            int r64 = 2;
+            while (r64 * 64 < p->N) r64 *= 2;
+            if (r64 <= 32) {
+                p->R64 = r64;
+                p->waves64 = jwdev_main64_waves(r64);
+                p->occupancy64 = p->waves64 ? jwdev_main64_occupancy(r64, p->waves64) : 0;
+                if (p->occupancy64 < 1) p->R64 = 0;
+            }
+            if (p->R64) {
+                size_t b2 = (size_t)jwdev_table64_elems(p->R64) * 8;
+                if (b2 > p->table_bytes) p->table_bytes = b2;
+            }
+        }
+        if ((cfg->kernelVariant == 2 || long_n) && !p->R64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        /* compute / store split form of the wave-per-system kernel (R <= 8) */
+        if (p->R64 && p->R64 <= 8) {
+            int c = 0, st = 0;
+            if (jwdev_main64s_shape(p->R64, &c, &st)) {
+                p->occupancy_split = jwdev_main64s_occupancy(p->R64);
+                if (p->occupancy_split >= 1) p->split64 = c;
+            }
+        }
+        p->table_stride = (p->table_bytes + 255) / 256 * 256;
+        if (!long_n) {
+            p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
+            if (p->occupancy < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
+        }
+    } else if (p->kind == PF_KIND_BLOCK) {
+        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_block_%llu_%llu_%d_%d_%lld",
+                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->block, cfg->lshift,
+                 (long long)cfg->jw_control_bitmask);
+    } else {
+        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_JW_dgbmv_%llu_%llu_%d_%d_%d_%lld",
+                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->LDA, cfg->KU, cfg->KL,
+                 (long long)cfg->jw_control_bitmask);
+    }
+    return PFSOLVE_SUCCESS;
+}
+
+
 PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in)
 {
     if (!app) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
@@ -125,81 +202,18 @@ PfSolveResult initializePfSolve(PfSolveApplication* app, PfSolveConfiguration in
     }
 
     struct pfsolve_plan* p = (struct pfsolve_plan*)calloc(1, sizeof(*p));
-    if (!p) return PFSOLVE_ERROR_MALLOC_FAILED;
+    if (!p) { deletePfSolve(app); return PFSOLVE_ERROR_MALLOC_FAILED; }
     app->localFFTPlan = p;
-    p->kind = jw_plan_kind(cfg);
-    p->precision = jw_precision(cfg);
-    p->N = cfg->M_size;
-
-    int cu = 0, lds_blk = 0, maj = 0, min = 0;
-    if (jwdev_device_info(&cu, &lds_blk, &maj, &min) != 0) { deletePfSolve(app); return PFSOLVE_ERROR_INVALID_DEVICE; }
-    p->cu_count = cu;
-    cfg->computeCapabilityMajor = (uint64_t)maj;
-    cfg->computeCapabilityMinor = (uint64_t)min;
-    cfg->sharedMemorySize = (uint64_t)lds_blk;
-    cfg->maxThreadsNum = 1024;
-
+    res = plan_setup(app, p);
+    if (res != PFSOLVE_SUCCESS) { deletePfSolve(app); return res; }
     if (p->kind == PF_KIND_JW) {
-        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_jw_%llu_%llu_%d_%d_%lld_%llu_%d_%d",
-                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->upperBanded,
-                 cfg->jw_type, (long long)cfg->jw_control_bitmask, (unsigned long long)cfg->Msplit[0],
-                 cfg->numConsecutiveJWIterations, cfg->useMultipleInputBuffers);
-        pfsolve_geometry g;
-        res = fill_geometry(cfg, &g);
-        if (res != PFSOLVE_SUCCESS) { deletePfSolve(app); return res; }
-        p->L = g.lanes;
-        p->R = g.rows_per_lane;
-        p->waves = g.waves;
-        p->lds_bytes = g.lds_bytes;
-        p->table_bytes = (size_t)g.table_bytes;
-        const int kp = p->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD;
-        const int long_n = p->L == 0;   /* N > 1024: the wave-per-system long kernel only */
-        if (long_n && cfg->kernelVariant == 1) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
-        if (!long_n && !jwdev_supported(p->L, p->R, kp)) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
-        /* wave-per-system path: FP64, 64 lanes, R a power of two in [2, 32] */
-        if (p->precision == JW_PREC_F64 && cfg->kernelVariant != 1) {
-            int r64 = 2;
-            while (r64 * 64 < p->N) r64 *= 2;
-            if (r64 <= 32) {
-                p->R64 = r64;
-                p->waves64 = jwdev_main64_waves(r64);
-                p->occupancy64 = p->waves64 ? jwdev_main64_occupancy(r64, p->waves64) : 0;
-                if (p->occupancy64 < 1) p->R64 = 0;
-            }
-            if (p->R64) {
-                size_t b2 = (size_t)jwdev_table64_elems(p->R64) * 8;
-                if (b2 > p->table_bytes) p->table_bytes = b2;
-            }
-        }
-        if ((cfg->kernelVariant == 2 || long_n) && !p->R64) { deletePfSolve(app); return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH; }
-        /* compute / store split form of the wave-per-system kernel (R <= 8) */
-        if (p->R64 && p->R64 <= 8) {
-            int c = 0, st = 0;
-            if (jwdev_main64s_shape(p->R64, &c, &st)) {
-                p->occupancy_split = jwdev_main64s_occupancy(p->R64);
-                if (p->occupancy_split >= 1) p->split64 = c;
-            }
-        }
         if (hipMalloc((void**)&p->d_err, 256) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_ALLOCATE; }
         if (hipMemset(p->d_err, 0, 256) != hipSuccess) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_COPY; }
         /* one table per stream slot */
-        p->table_stride = (p->table_bytes + 255) / 256 * 256;
         if (hipMalloc(&p->table, p->table_stride * cfg->num_streams) != hipSuccess) {
             deletePfSolve(app);
             return PFSOLVE_ERROR_FAILED_TO_ALLOCATE;
         }
-        if (!long_n) {
-            p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
-            if (p->occupancy < 1) { deletePfSolve(app); return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY; }
-        }
-    } else if (p->kind == PF_KIND_BLOCK) {
-        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_block_%llu_%llu_%d_%d_%lld",
-                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->block, cfg->lshift,
-                 (long long)cfg->jw_control_bitmask);
-    } else {
-        snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_JW_dgbmv_%llu_%llu_%d_%d_%d_%lld",
-                 (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->LDA, cfg->KU, cfg->KL,
-                 (long long)cfg->jw_control_bitmask);
     }
     return PFSOLVE_SUCCESS;
 }
@@ -275,7 +289,7 @@ static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveL
     return PFSOLVE_SUCCESS;
 }
 
-static PfSolveResult append_impl(PfSolveApplication* app, PfSolveLaunchParams* lp, uint64_t slot)
+static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchParams* lp, uint64_t slot)
 {
     struct pfsolve_plan* p = app->localFFTPlan;
     PfSolveConfiguration* cfg = &app->configuration;
@@ -427,6 +441,43 @@ PfSolveResult PfSolveAppend(PfSolveApplication* app, int inverse, PfSolveLaunchP
     cfg->streamID = cfg->streamCounter % cfg->num_streams;
     return PFSOLVE_SUCCESS;
 }
+
+/* initializePfSolve + PfSolveAppend + deletePfSolve with the plan on the stack and the
+ * device table and error flag in the library's static device scratch: no allocation. */
+PfSolveResult pfsolve_append_noalloc(const PfSolveConfiguration* configuration,
+                                     const PfSolveLaunchParams* launchParams)
+{
+    if (!configuration) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
+    if (configuration->device == NULL) return PFSOLVE_ERROR_INVALID_DEVICE;
+    if (configuration->num_streams > 1) return PFSOLVE_ERROR_INVALID_QUEUE;
+    PfSolveApplication app;
+    memset(&app, 0, sizeof(app));
+    PfSolveResult res = jw_configure(configuration, &app.configuration);
+    if (res != PFSOLVE_SUCCESS) return res;
+    app.configuration.stream_event = NULL;
+    app.configuration.streamID = 0;
+    app.configuration.streamCounter = 0;
+    app.configuration.num_streams = 1;
+    struct pfsolve_plan plan;
+    memset(&plan, 0, sizeof(plan));
+    res = plan_setup(&app, &plan);
+    if (res != PFSOLVE_SUCCESS) return res;
+    if (plan.kind == PF_KIND_JW) {
+        void* table = NULL;
+        int* err = NULL;
+        size_t cap = 0;
+        if (jwdev_noalloc_scratch(&table, &err, &cap) != 0) return PFSOLVE_ERROR_INVALID_DEVICE;
+        if (plan.table_stride > cap) return PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER;
+        plan.table = table;
+        plan.d_err = err;
+        /* the flag starts at zero for this call (a fill kernel: callers may capture it) */
+        if (jwdev_zero_counters(err, 1, slot_stream(&app, 0)) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    }
+    app.localFFTPlan = &plan;
+    return append_impl(&app, launchParams, 0);
+}
+
+size_t pfsolve_noalloc_scratch_bytes(void) { return 144 * 1024 + 256; }
 
 /* Joins the plan's work and reports a raised device error flag (a bounded wait inside a
  * kernel ran out).  With num_streams > 1 it waits for every stream's event and restarts
