@@ -196,6 +196,65 @@ static int resolve_problem(const pfo_jw_problem* p, jw_resolved* rs)
     return 0;
 }
 
+/* One iteration of a chained step (numConsecutiveJWIterations = K > 1,
+ * PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel2/pfSolve_JonesWorland.h:108-330): iteration i
+ * has N_i = N -/+ i rows (upperBanded 1: shrinking, 0: growing, :110-113); its matvec is
+ * rectangular, N_i rows by N_{i-1} columns; the matvec diagonal is applied only in the first
+ * (upperBanded 1) or the last (upperBanded 0) iteration and is an implicit 1 elsewhere
+ * (pfSolve_MatVecMul.h:74,98; loads pfSolve_ReadWrite.h:1650-1660); baked positions change
+ * with i (pfSolve_JonesWorland.h:159-168,189-198 for the matvec, :291-296,320-324 for the
+ * solve), runtime ones use the current N_i.  Matrices are stored in reorder_i0 order of
+ * N_i rows with the register layout of the largest system (Nalign = N, or N + K - 1 when
+ * growing; pfSolve_Plan_JonesWorland.h:277-282).  Modes restated: jw 10, 11, 12 with
+ * upperBanded 0 or 1. */
+typedef struct jw_iter {
+    int64_t N, Ncols;
+    int mv, sv, unit;
+    int64_t mdA, udA, ldA, ldS, udS;
+    int64_t* P;
+} jw_iter;
+
+static int resolve_iteration(const pfo_jw_problem* p, int K, int i, jw_iter* it)
+{
+    memset(it, 0, sizeof(*it));
+    const int natural = p->jw_type > 1000;
+    const int jw = (p->jw_type % 1000) % 10;
+    const int ub = p->upperBanded;
+    if ((p->jw_type % 1000) < 10 || (p->jw_type % 1000) > 12 || (ub != 0 && ub != 1) || K < 1) return -5;
+    const int64_t N0 = p->N;
+    it->N = ub ? N0 - i : N0 + i;
+    if (it->N < 1) return -6;
+    it->Ncols = i == 0 ? N0 : (ub ? it->N + 1 : it->N - 1);
+    const int64_t Nalign = ub ? N0 : N0 + K - 1;
+    const int64_t m = p->mask;
+    const int rtM = (m & PFO_RUNTIME_OFFSETM) != 0, rtV = (m & PFO_RUNTIME_OFFSETV) != 0;
+    const int64_t offM = rtM ? p->run_offsetM : p->cfg_offsetM;
+    const int64_t offV = rtV ? p->run_offsetV : p->cfg_offsetV;
+    const int64_t Ni = it->N;
+    it->P = (int64_t*)malloc(sizeof(int64_t) * (size_t)Ni);
+    if (!it->P) return -100;
+    const int permuted = !natural && pfo_jw_uses_parallel_thomas(N0, K, p->jw_type);
+    for (int64_t r = 0; r < Ni; r++) it->P[r] = permuted ? pfo_reorder_i0(r, Ni, Nalign, p->Msplit0) : r;
+    if (jw == 0 || jw == 1) {
+        if (ub) {
+            it->mv = MV_UPPER;
+            if (rtM) { it->udA = offM; it->mdA = offM + Ni; it->unit = i != 0; }
+            else if (i == 0) { it->mdA = 0; it->udA = Ni; }
+            else { it->udA = 0; it->unit = 1; }
+        } else {
+            it->mv = MV_LOWER;
+            if (rtM) { it->mdA = offM; it->ldA = offM + Ni; it->unit = i != K - 1; }
+            else if (i == K - 1) { it->ldA = 0; it->mdA = Ni; }
+            else { it->ldA = 0; it->unit = 1; }
+        }
+    }
+    if (jw == 0 || jw == 2) {
+        if (ub) { it->sv = SV_LOWER; it->ldS = rtV ? offV : (i == 0 ? 2 * Ni : Ni); }
+        else    { it->sv = SV_UPPER; it->udS = rtV ? offV : (i == K - 1 ? 2 * Ni : Ni); }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* double-double arithmetic (Dekker 1971; Knuth TAOCP 4.2.2 TwoSum;    */
 /* Joldes, Muller, Popescu, ACM TOMS 44(2) 2017, Alg. 6, 12, 17)       */
@@ -466,6 +525,20 @@ int pfo_jw_step_f128(const pfo_jw_problem* p, const double* buffer, double* out,
 int pfo_jw_step_dd(const pfo_jw_problem* p, const pfo_dd* buffer, pfo_dd* out)
 {
     return jw_step_dd(p, buffer, out, 0, p->B);
+}
+
+int pfo_jw_chain_f64(const pfo_jw_problem* p, int K, const double* const* buffers, double* out)
+{
+    return jw_chain_f64(p, K, buffers, out, 0, p->B);
+}
+
+int pfo_jw_chain_f128(const pfo_jw_problem* p, int K, const double* const* buffers, double* out)
+{
+    g_q_lo_out = NULL;
+    g_q_base = out;
+    int r = jw_chain_f128(p, K, buffers, out, 0, p->B);
+    g_q_base = NULL;
+    return r;
 }
 
 int pfo_jw_step_f128_dd(const pfo_jw_problem* p, const pfo_dd* buffer, pfo_dd* out)

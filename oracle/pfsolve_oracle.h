@@ -91,6 +91,12 @@ int pfo_jw_step_dd(const pfo_jw_problem* p, const pfo_dd* buffer, pfo_dd* out);
 /* __float128 evaluation of DD inputs; the result is rounded to DD. */
 int pfo_jw_step_f128_dd(const pfo_jw_problem* p, const pfo_dd* buffer, pfo_dd* out);
 
+/* Chained step (numConsecutiveJWIterations = K, jw 10/11/12, upperBanded 0/1): iteration i
+ * reads its matrices from buffers[i] (the same pointer K times without
+ * useMultipleInputBuffers); p->N is the first system's size. */
+int pfo_jw_chain_f64(const pfo_jw_problem* p, int K, const double* const* buffers, double* out);
+int pfo_jw_chain_f128(const pfo_jw_problem* p, int K, const double* const* buffers, double* out);
+
 /* dgbmv plan: y = A_band x in place; A at offsetM (band-major, LDA rows). */
 int pfo_dgbmv_f64(const pfo_jw_problem* p, int LDA, int KU, int KL, const double* buffer, double* out);
 int pfo_dgbmv_f128(const pfo_jw_problem* p, int LDA, int KU, int KL, const double* buffer, double* out);

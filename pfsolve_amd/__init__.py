@@ -86,7 +86,8 @@ class Plan:
 
     ``fields`` are PfSolveConfiguration fields (size=[N, B, 1] etc.);
     ``matrices`` and ``rhs`` are device pointers (ints) of the plan's
-    `buffer` and `outputBuffer`; ``stream`` an optional hipStream_t (int), or a
+    `buffer` and `outputBuffer` (``matrices`` a list of pointers for
+    useMultipleInputBuffers: one buffer per chained iteration); ``stream`` an optional hipStream_t (int), or a
     list of them: launches then go round robin over the streams (num_streams).
     """
 
@@ -103,7 +104,8 @@ class Plan:
                 setattr(cfg, k, v)
         self._dev = C.c_int(0)
         cfg.device = C.pointer(self._dev)
-        self._buf = (C.c_void_p * 1)(matrices)
+        mats = list(matrices) if isinstance(matrices, (list, tuple)) else [matrices]
+        self._buf = (C.c_void_p * len(mats))(*mats)
         self._out = (C.c_void_p * 1)(rhs)
         cfg.buffer = C.cast(self._buf, _abi.vpp)
         cfg.outputBuffer = C.cast(self._out, _abi.vpp)

@@ -159,11 +159,14 @@ constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v / 2); }
 
 // reorder_i0 (storage position of logical row i), single-upload case N <= 3840:
 // PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:66-142
-__device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N)
+__device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N, int64_t Nalign = 0)
 {
-    int64_t ws = ((N + 1023) / 1024) * 64;
-    if (N < 512) ws = 64;
-    int64_t R = (N + ws - 1) / ws;
+    // rows per thread of the register layout: from Nalign (the largest system of a
+    // chained step), the split of the N stored rows over them: from N
+    const int64_t NA = Nalign > 0 ? Nalign : N;
+    int64_t ws = ((NA + 1023) / 1024) * 64;
+    if (NA < 512) ws = 64;
+    int64_t R = (NA + ws - 1) / ws;
     if ((R & 1) == 0) R++;
     int64_t ceilq = (N + R - 1) / R;
     int64_t k = i % R;

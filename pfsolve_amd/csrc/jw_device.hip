@@ -57,6 +57,16 @@ __global__ __launch_bounds__(NT) void jw_prologue_batch_kernel(const jw_prologue
     prologue_body<T>(j.src, j.matrices, reinterpret_cast<T*>(j.table), j.L, j.R, j.format, pro_smem);
 }
 
+// chained step: one workgroup per iteration, each building its format-2 table (64 lanes,
+// R rows per lane, R <= 16 so 256 threads cover the rows); the jobs come by value
+__global__ __launch_bounds__(256) void jw_prologue_chain_kernel(jw_chain_jobs j)
+{
+    extern __shared__ __align__(16) unsigned char pro_smem[];
+    const int k = blockIdx.x;
+    double* table = reinterpret_cast<double*>(static_cast<unsigned char*>(j.table) + (int64_t)k * j.tab_stride);
+    prologue_body<double>(j.src[k], j.matrices[k], table, 64, j.R, 2, pro_smem);
+}
+
 // ------------------------------------------------------------------------
 // main kernel
 // ------------------------------------------------------------------------
@@ -538,6 +548,24 @@ int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, i
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { set_err("prologue launch", e); return -1; }
+    return 0;
+}
+
+int jwdev_prologue_chain(const jw_chain_jobs* jobs, void* stream)
+{
+    const int RL = jobs->R * 64;
+    if (jobs->K < 1 || jobs->K > JW_CHAIN_MAX || RL > kProRows * 256) {
+        snprintf(g_err, sizeof(g_err), "chain prologue: bad job count or rows");
+        return -1;
+    }
+    const size_t lds = prologue_lds_bytes(RL, 8);
+    const void* fn = reinterpret_cast<const void*>(&jw_prologue_chain_kernel);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) { set_err("hipFuncSetAttribute(chain prologue)", e); return -2; }
+    hipLaunchKernelGGL(jw_prologue_chain_kernel, dim3(jobs->K), dim3(256), lds, reinterpret_cast<hipStream_t>(stream),
+                       *jobs);
+    e = hipGetLastError();
+    if (e != hipSuccess) { set_err("chain prologue launch", e); return -1; }
     return 0;
 }
 

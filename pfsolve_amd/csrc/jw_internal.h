@@ -57,7 +57,23 @@ typedef struct jw_source_desc {
     int64_t mdM, udM;
     int32_t Msplit0;
     int32_t mem_f64;          /* DD compute with FP64 storage */
+    /* chained iterations: columns of the (rectangular) matvec, 0 = N; the row count of
+     * the register layout the storage permutation uses, 0 = N; matvec diagonal an
+     * implicit 1 (not read) */
+    int64_t Ncols, Nalign;
+    int32_t unit_diag, pad_chain;
 } jw_source_desc;
+
+/* Tables of a chained step (numConsecutiveJWIterations = K <= JW_CHAIN_MAX), built by one
+ * launch with one workgroup per iteration; passed by value (no device copy of the jobs). */
+#define JW_CHAIN_MAX 16
+typedef struct jw_chain_jobs {
+    jw_source_desc src[JW_CHAIN_MAX];
+    const void* matrices[JW_CHAIN_MAX];
+    void* table;              /* iteration k's format-2 table at table + k * tab_stride bytes */
+    int64_t tab_stride;
+    int32_t K, R;
+} jw_chain_jobs;
 
 /* One launch of the main kernel over a batch of right-hand sides. */
 typedef struct jw_run_desc {
@@ -167,6 +183,11 @@ int jwdev_batch64s_occupancy(int R, int ncomp);
 int jwdev_batch64s(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
                    int ncomp, int grid, int* d_err, int wait_polls, void* stream);
 int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream);
+/* chained steps: the K tables (one launch), then the kernel that keeps each pair of systems
+ * in registers through the K iterations (FP64, 16-byte aligned pairs, R in {2,4,8,16}) */
+int jwdev_prologue_chain(const jw_chain_jobs* jobs, void* stream);
+int jwdev_chain64_occupancy(int R);
+int jwdev_chain64(const jw_run_desc* r, const void* tables, int64_t tab_elems, int K, int R, int grid, void* stream);
 int jwdev_block(const jw_block_desc* d, void* stream);
 int jwdev_gbmv(const jw_gbmv_desc* d, void* stream);
 int jwdev_device_info(int* cu_count, int* lds_per_block, int* arch_major, int* arch_minor);

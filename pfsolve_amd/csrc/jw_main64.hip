@@ -501,6 +501,48 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc
 }
 
 // ---------------------------------------------------------------------------
+// Chained steps (numConsecutiveJWIterations = K): a wavefront owns a pair of systems
+// through all K iterations, whose right-hand sides stay in its registers in between (one
+// read and one write of HBM per K-step instead of K).  Iteration k applies its own
+// format-2 table; the K tables stay in global memory (every wavefront reads them, so they
+// are served from L2); LDS holds each wavefront's staging.  The input keep-range applies
+// to the first iteration's rows, the output keep-range to the last one's.
+// ---------------------------------------------------------------------------
+template <int R>
+constexpr int chain_waves() { return 4; }
+
+template <int R, bool ALL3>
+__global__ __launch_bounds__(256) void jw_chain64_kernel(jw_run_desc d, const double* tables, int64_t tab_elems,
+                                                         int K)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* stg = reinterpret_cast<double2*>(smem) + (size_t)wave * 64 * R;
+    zero_staging<R>(stg, lane);
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): zeros written before any DMA lands
+    compiler_fence();
+    const int64_t npass = (d.nsys + 1) >> 1;
+    for (int64_t q = (int64_t)blockIdx.x * nwaves + wave; q < npass; q += (int64_t)gridDim.x * nwaves) {
+        const jw_pass64 g = make_pass64(reinterpret_cast<double*>(d.rhs), d.nsys, d.stride, d.offSol,
+                                        (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
+                                        d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
+        dma_pair<R>(stg, lane, g, 2 * q);
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the DMA writes have landed in LDS
+        compiler_fence();
+        double x0[R], x1[R];
+        read_pair<R>(stg, lane, g, 2 * q, x0, x1);
+        for (int k = 0; k < K; k++) compute_pair<R, ALL3>(tables + (int64_t)k * tab_elems, lane, g, x0, x1);
+        write_blocked<R>(stg, lane, x0, x1);
+        store_pair<R>(stg, lane, g, 2 * q);
+        rezero_input<R>(stg, lane, g);
+        __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): LDS traffic done before the next DMA lands
+        compiler_fence();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Long systems (R = 32 rows per lane, N <= 2048): one system per wavefront pass.
 // The 140 KiB table stays in global memory (every wavefront of the grid reads
 // it, so it is served from L2) and coefficients are streamed row by row instead
@@ -1362,6 +1404,53 @@ int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream)
     if (e != hipSuccess) return -2;
     void* args[] = {const_cast<jw_run_desc*>(r)};
     e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? 0 : -3;
+}
+
+static const void* pick_chain(int R, bool all3)
+{
+    switch (R) {
+    case 2: return all3 ? reinterpret_cast<const void*>(&jw_chain64_kernel<2, true>)
+                        : reinterpret_cast<const void*>(&jw_chain64_kernel<2, false>);
+    case 4: return all3 ? reinterpret_cast<const void*>(&jw_chain64_kernel<4, true>)
+                        : reinterpret_cast<const void*>(&jw_chain64_kernel<4, false>);
+    case 8: return all3 ? reinterpret_cast<const void*>(&jw_chain64_kernel<8, true>)
+                        : reinterpret_cast<const void*>(&jw_chain64_kernel<8, false>);
+    case 16: return all3 ? reinterpret_cast<const void*>(&jw_chain64_kernel<16, true>)
+                         : reinterpret_cast<const void*>(&jw_chain64_kernel<16, false>);
+    default: return nullptr;
+    }
+}
+
+static int64_t chain_lds_bytes(int R) { return 2LL * 64 * R * 8 * 4; }   /* 4 wavefronts' stagings */
+
+/* workgroups (of 4 wavefronts) per CU of the chained-step kernel */
+int jwdev_chain64_occupancy(int R)
+{
+    int best = -1;
+    for (int v = 0; v < 2; v++) {
+        const void* f = pick_chain(R, v == 0);
+        if (!f) return 0;
+        const int lds = (int)chain_lds_bytes(R);
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, lds) != hipSuccess) return 0;
+        if (best < 0 || nb < best) best = nb;
+    }
+    return best;
+}
+
+int jwdev_chain64(const jw_run_desc* r, const void* tables, int64_t tab_elems, int K, int R, int grid, void* stream)
+{
+    const void* f = pick_chain(R, r->fwd && r->b1 && r->b2);
+    if (!f || K < 1) { snprintf(g_err64, sizeof(g_err64), "no chained-step kernel for R=%d", R); return -1; }
+    const size_t lds = (size_t)chain_lds_bytes(R);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return -2;
+    jw_run_desc rr = *r;
+    const double* t = static_cast<const double*>(tables);
+    void* args[] = {&rr, &t, &tab_elems, &K};
+    e = hipLaunchKernel(f, dim3(grid), dim3(256), args, lds, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : -3;
 }
 

@@ -10,6 +10,7 @@
 #define JW_MAX_N_F64 2048   /* N > 1024: wave-per-system long kernel only */
 #define JW_MAX_N_DD 512
 #define JW_MAX_N_FUSED_BLOCKS 512   /* fused block I/O runs on the compute / store split kernel */
+#define JW_MAX_N_CHAIN 1024         /* chained steps: R <= 16 rows per lane on one wavefront */
 
 int jw_plan_kind(const PfSolveConfiguration* cfg)
 {
@@ -59,8 +60,20 @@ PfSolveResult jw_configure(const PfSolveConfiguration* in, PfSolveConfiguration*
         const int jwt = out->jw_type % 1000;
         if (jwt < 10 || jwt > 14) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;   /* ALT (hundreds digit) not built */
         const int jw = jwt % 10;
-        if (out->numConsecutiveJWIterations != 1) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+        const int K = out->numConsecutiveJWIterations;
         if (out->upperBanded < 0 || out->upperBanded > 2) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+        if (out->useMultipleInputBuffers != 0 && out->useMultipleInputBuffers != K) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+        if (K != 1) {
+            /* chained steps (pfSolve_JonesWorland.h:108-374): the bidiagonal JW modes, FP64,
+             * every iteration's system (N -/+ i rows) on one wavefront of the chained kernel */
+            if (K < 1 || K > JW_CHAIN_MAX || jw > 2 || out->upperBanded > 1) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+            if (prec != JW_PREC_F64 || out->inputBlock || out->outputBlock || out->kernelVariant == 1)
+                return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+            const int64_t nmax = out->upperBanded ? N : N + K - 1;
+            const int64_t nmin = out->upperBanded ? N - K + 1 : N;
+            if (nmin < 1) return PFSOLVE_ERROR_EMPTY_size;
+            if (nmax > JW_MAX_N_CHAIN) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        }
         if (jw != 4 && out->upperBanded == 2 && jw != 3) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
         if (jw == 4) {
             if (out->KL == 0 && out->KU == 0) { out->KL = 2; out->KU = 2; }
@@ -231,6 +244,60 @@ PfSolveResult jw_resolve_launch(const PfSolveConfiguration* cfg, const PfSolveLa
     return PFSOLVE_SUCCESS;
 }
 
+/* Iteration i of a chained step (numConsecutiveJWIterations = K > 1), for the bidiagonal
+ * modes jw 10/11/12 with upperBanded 0/1
+ * (PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel2/pfSolve_JonesWorland.h:108-330):
+ *  - N_i = N - i (upperBanded 1) or N + i (upperBanded 0), :110-113; the matvec of
+ *    iteration i > 0 has the previous iteration's N_{i-1} columns;
+ *  - the matvec diagonal is read and applied in the first (upperBanded 1) or the last
+ *    (upperBanded 0) iteration only, an implicit 1 elsewhere (pfSolve_MatVecMul.h:74,98,
+ *    pfSolve_ReadWrite.h:1650);
+ *  - baked positions: upper [D @0, U @N_i] in iteration 0, then [U @0]; lower [L @0,
+ *    D @N_i] in the last iteration, [L @0] before (:159-168, 189-198); the solve at 2 N_i in
+ *    the first (lower solve) / last (upper solve) iteration, at N_i otherwise (:291-296,
+ *    320-324); runtime offsets are the same every iteration, with N_i for the diagonal;
+ *  - storage in reorder_i0 order of N_i rows in the register layout of the largest system
+ *    (pfSolve_Plan_JonesWorland.h:277-282). */
+PfSolveResult jw_resolve_chain_iteration(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp, int i,
+                                         jw_source_desc* src)
+{
+    if (!lp) lp = &g_zero_lp;
+    memset(src, 0, sizeof(*src));
+    const int K = cfg->numConsecutiveJWIterations;
+    const int64_t N0 = cfg->M_size;
+    const int ub = cfg->upperBanded;
+    const int jw = (cfg->jw_type % 1000) % 10;
+    const int64_t m = cfg->jw_control_bitmask;
+    const int rtM = (m & RUNTIME_OFFSETM) != 0, rtV = (m & RUNTIME_OFFSETV) != 0;
+    const int64_t offM = eff(m, RUNTIME_OFFSETM, cfg->offsetM, lp->offsetM);
+    const int64_t offV = eff(m, RUNTIME_OFFSETV, cfg->offsetV, lp->offsetV);
+    const int64_t Ni = ub ? N0 - i : N0 + i;
+    src->N = Ni;
+    src->Ncols = i == 0 ? N0 : (ub ? Ni + 1 : Ni - 1);
+    src->Nalign = ub ? N0 : N0 + K - 1;
+    src->mdS = -1;
+    src->Msplit0 = (int32_t)cfg->Msplit[0];
+    src->permuted = cfg->jw_type < 1000;   /* K > 1: the parallel layout at every size (Plan_JonesWorland.h:305-306) */
+    if (jw == 0 || jw == 1) {
+        const int diag_here = ub ? i == 0 : i == K - 1;
+        src->unit_diag = !diag_here;
+        if (ub) {
+            src->band_kind = JW_BAND_UPPER;
+            src->udA = rtM ? offM : (i == 0 ? Ni : 0);
+            src->mdA = rtM ? offM + Ni : 0;
+        } else {
+            src->band_kind = JW_BAND_LOWER;
+            src->ldA = rtM ? offM + Ni : 0;
+            src->mdA = rtM ? offM : Ni;
+        }
+    }
+    if (jw == 0 || jw == 2) {
+        if (ub) { src->solve_kind = JW_SOLVE_LOWER; src->ldS = rtV ? offV : (i == 0 ? 2 * Ni : Ni); }
+        else    { src->solve_kind = JW_SOLVE_UPPER; src->udS = rtV ? offV : (i == K - 1 ? 2 * Ni : Ni); }
+    }
+    return PFSOLVE_SUCCESS;
+}
+
 /* Largest element index the prologue reads from the matrix buffer and the main
  * kernel touches in the RHS buffer (-1: none). */
 void jw_index_extent(const jw_source_desc* s, const jw_run_desc* r, int64_t* mat_max, int64_t* rhs_max)
@@ -240,8 +307,8 @@ void jw_index_extent(const jw_source_desc* s, const jw_run_desc* r, int64_t* mat
 #define UPD(base) do { if ((base) + N - 1 > m) m = (base) + N - 1; } while (0)
     switch (s->band_kind) {
     case JW_BAND_DIAG: UPD(s->mdA); break;
-    case JW_BAND_UPPER: UPD(s->mdA); UPD(s->udA); break;
-    case JW_BAND_LOWER: UPD(s->mdA); UPD(s->ldA); break;
+    case JW_BAND_UPPER: if (!s->unit_diag) UPD(s->mdA); UPD(s->udA); break;
+    case JW_BAND_LOWER: if (!s->unit_diag) UPD(s->mdA); UPD(s->ldA); break;
     case JW_BAND_GB: UPD(s->offV + (int64_t)(s->KL + s->KU) * N); break;
     default: break;
     }

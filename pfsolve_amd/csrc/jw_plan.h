@@ -32,6 +32,10 @@ struct pfsolve_plan {
     int* d_err;
     int wait_polls;           /* jw_run_desc.wait_polls of the split kernel's launches */
     hipEvent_t ev_start, ev_stop;   /* optional timing events around the main kernel */
+    /* chained steps: iterations (1 = none), rows per lane of the chained kernel, its
+     * workgroups per CU; a slot's table area holds K format-2 tables of chain_tab bytes */
+    int chain_K, chain_R, chain_occ;
+    size_t chain_tab;
 };
 
 /* Applies the reference's configuration defaults (pfSolve_InitializeApp.h:981-1212)
@@ -51,6 +55,9 @@ PfSolveResult jw_resolve_block(const PfSolveConfiguration* cfg, const PfSolveLau
                                int64_t nsys, jw_block_desc* d);
 PfSolveResult jw_resolve_gbmv(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp,
                               int64_t nsys, jw_gbmv_desc* d);
+/* iteration i of a chained step (numConsecutiveJWIterations > 1) */
+PfSolveResult jw_resolve_chain_iteration(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp, int i,
+                                         jw_source_desc* src);
 /* largest matrix / RHS element index a JW launch touches, and sanity of offsets */
 void jw_index_extent(const jw_source_desc* s, const jw_run_desc* r, int64_t* mat_max, int64_t* rhs_max);
 PfSolveResult jw_check_offsets(const jw_source_desc* s, const jw_run_desc* r);

@@ -75,6 +75,8 @@ __device__ __forceinline__ void row_band(const jw_source_desc& d, const void* ma
 {
     using A = Ar<T>;
     const int mf = d.mem_f64;
+    // columns of the matvec: N, or in a chained step the previous iteration's rows
+    const int64_t ncols = d.Ncols > 0 ? d.Ncols : N;
 #pragma unroll
     for (int k = 0; k < 5; k++) v[k] = A::zero();
     switch (d.band_kind) {
@@ -82,12 +84,12 @@ __device__ __forceinline__ void row_band(const jw_source_desc& d, const void* ma
         v[2] = mem_load<T>(mat, d.mdA + P, mf);
         break;
     case JW_BAND_UPPER:
-        v[2] = mem_load<T>(mat, d.mdA + P, mf);
-        if (r + 1 < N) v[3] = mem_load<T>(mat, d.udA + P, mf);
+        v[2] = d.unit_diag ? A::one() : mem_load<T>(mat, d.mdA + P, mf);
+        if (r + 1 < ncols) v[3] = mem_load<T>(mat, d.udA + P, mf);
         break;
     case JW_BAND_LOWER:
-        v[2] = mem_load<T>(mat, d.mdA + P, mf);
-        if (r >= 1) v[1] = mem_load<T>(mat, d.ldA + P, mf);
+        v[2] = d.unit_diag ? A::one() : mem_load<T>(mat, d.mdA + P, mf);
+        if (r >= 1 && r - 1 < ncols) v[1] = mem_load<T>(mat, d.ldA + P, mf);
         break;
     case JW_BAND_GB:
 #pragma unroll
@@ -137,7 +139,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
 #pragma unroll
         for (int k = 0; k < 5; k++) vb[j][k] = A::zero();
         if (r < N) {
-            const int64_t P = d.permuted ? reorder_pos(r, N) : (int64_t)r;
+            const int64_t P = d.permuted ? reorder_pos(r, N, d.Nalign) : (int64_t)r;
             row_band<T>(d, mat, r, N, P, vb[j]);
             if (has_f && r >= 1) ldv = mem_load<T>(mat, d.ldS + P, mf);
             if (has_b1 && r + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);

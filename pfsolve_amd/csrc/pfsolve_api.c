@@ -107,6 +107,7 @@ static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
     p->kind = jw_plan_kind(cfg);
     p->precision = jw_precision(cfg);
     p->N = cfg->M_size;
+    p->chain_K = 1;
 
     int cu = 0, lds_blk = 0, maj = 0, min = 0;
     if (jwdev_device_info(&cu, &lds_blk, &maj, &min) != 0) return PFSOLVE_ERROR_INVALID_DEVICE;
@@ -120,6 +121,24 @@ static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
                  (unsigned long long)cfg->size[0], (unsigned long long)cfg->size[1], cfg->upperBanded,
                  cfg->jw_type, (long long)cfg->jw_control_bitmask, (unsigned long long)cfg->Msplit[0],
                  cfg->numConsecutiveJWIterations, cfg->useMultipleInputBuffers);
+        if (cfg->numConsecutiveJWIterations > 1) {
+            /* chained steps: K format-2 tables (one per iteration) per stream slot and the
+             * chained kernel, R rows per lane for the largest iteration */
+            const int K = cfg->numConsecutiveJWIterations;
+            const int64_t nmax = cfg->upperBanded ? p->N : p->N + K - 1;
+            int r64 = 2;
+            while (r64 * 64 < nmax) r64 *= 2;
+            p->chain_K = K;
+            p->chain_R = r64;
+            p->chain_occ = jwdev_chain64_occupancy(r64);
+            if (p->chain_occ < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
+            p->chain_tab = (size_t)jwdev_table64_elems(r64) * 8;
+            p->L = 64;
+            p->R64 = r64;
+            p->table_bytes = (size_t)K * p->chain_tab;
+            p->table_stride = (p->table_bytes + 255) / 256 * 256;
+            return PFSOLVE_SUCCESS;
+        }
         pfsolve_geometry g;
         res = fill_geometry(cfg, &g);
         if (res != PFSOLVE_SUCCESS) return res;
@@ -289,6 +308,67 @@ static PfSolveResult append_fused_blocks(PfSolveApplication* app, const PfSolveL
     return PFSOLVE_SUCCESS;
 }
 
+/* Chained steps (numConsecutiveJWIterations = K > 1): the K iterations' tables in one
+ * launch, then one launch of the chained kernel that keeps each pair of systems in
+ * registers through the K iterations (pfSolve_JonesWorland.h:95-374).  Iteration i reads its
+ * matrices from buffer[i] with useMultipleInputBuffers, from buffer[0] otherwise
+ * (pfSolve_DispatchPlan.h:231-235); with useMultipleInputBuffers a declared bufferSize holds
+ * one size per buffer. */
+static PfSolveResult append_chain(PfSolveApplication* app, const PfSolveLaunchParams* lp, jw_run_desc* run,
+                                  hipStream_t s, void* table)
+{
+    struct pfsolve_plan* p = app->localFFTPlan;
+    PfSolveConfiguration* cfg = &app->configuration;
+    const int K = p->chain_K;
+    const int multi = cfg->useMultipleInputBuffers != 0;
+    if (!lp) lp = &(const PfSolveLaunchParams){0};
+    /* the output keep-range is that of the last iteration's rows */
+    const int64_t m = cfg->jw_control_bitmask;
+    const int64_t nlast = cfg->upperBanded ? p->N - K + 1 : p->N + K - 1;
+    const int ozrt = (m & RUNTIME_OUTPUTZEROPAD) != 0;
+    jw_keep_range(ozrt ? (int64_t)lp->outputZeropad[0] : cfg->outputZeropad[0],
+                  ozrt ? (int64_t)lp->outputZeropad[1] : cfg->outputZeropad[1], ozrt, nlast, &run->out_lo,
+                  &run->out_hi);
+    jw_chain_jobs jobs;
+    memset(&jobs, 0, sizeof(jobs));
+    for (int i = 0; i < K; i++) {
+        jw_source_desc* src = &jobs.src[i];
+        PfSolveResult r = jw_resolve_chain_iteration(cfg, lp, i, src);
+        if (r != PFSOLVE_SUCCESS) return r;
+        r = jw_check_offsets(src, run);
+        if (r != PFSOLVE_SUCCESS) return r;
+        const void* mat = cfg->buffer[multi ? i : 0];
+        if (!mat) return PFSOLVE_ERROR_EMPTY_buffer;
+        jobs.matrices[i] = mat;
+        int64_t mmax, rmax;
+        jw_index_extent(src, run, &mmax, &rmax);
+        if (cfg->bufferSize && mmax >= 0 && (uint64_t)((mmax + 1) * 8) > cfg->bufferSize[multi ? i : 0])
+            return PFSOLVE_ERROR_EMPTY_bufferSize;
+        if (i == 0 && cfg->outputBufferSize && rmax >= 0 && (uint64_t)((rmax + 1) * 8) > cfg->outputBufferSize[0])
+            return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+    }
+    run->rhs = cfg->outputBuffer[0];
+    run->vec2 = ((uintptr_t)run->rhs % 16) == 0 && (run->stride % 2) == 0 && (run->offSol % 2) == 0;
+    if (!run->vec2) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;   /* 16-byte aligned pairs of systems */
+    const int64_t nsys = run->nsys;
+    if (nsys <= 0) return PFSOLVE_SUCCESS;
+    jobs.table = table;
+    jobs.tab_stride = (int64_t)p->chain_tab;
+    jobs.K = K;
+    jobs.R = p->chain_R;
+    if (jwdev_prologue_chain(&jobs, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    const int64_t npass = (nsys + 1) / 2;
+    int64_t need = (npass + 3) / 4;
+    int64_t grid = (int64_t)p->cu_count * p->chain_occ;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+    if (jwdev_chain64(run, table, (int64_t)(p->chain_tab / 8), K, p->chain_R, (int)grid, s) != 0)
+        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+    return PFSOLVE_SUCCESS;
+}
+
 static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchParams* lp, uint64_t slot)
 {
     struct pfsolve_plan* p = app->localFFTPlan;
@@ -315,6 +395,7 @@ static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchPar
         jw_run_desc run;
         PfSolveResult r = jw_resolve_launch(cfg, lp, nsys, &src, &run);
         if (r != PFSOLVE_SUCCESS) return r;
+        if (p->chain_K > 1) return append_chain(app, lp, &run, s, table);
         /* offsets and strides: no matrix diagonal or system may start before its buffer */
         r = jw_check_offsets(&src, &run);
         if (r != PFSOLVE_SUCCESS) return r;

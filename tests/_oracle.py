@@ -183,3 +183,19 @@ def relerr_norm(a: np.ndarray, b: np.ndarray) -> float:
     den = float(np.max(np.abs(b)))
     num = float(np.max(np.abs(a - b)))
     return num / den if den > 0 else num
+
+
+def _chain_fn(name):
+    f = getattr(lib(), name)
+    f.argtypes = [C.POINTER(JwProblem), C.c_int, C.POINTER(dptr), dptr]
+    return f
+
+
+def jw_chain(p: JwProblem, K: int, bufs, rhs: np.ndarray, f128: bool = True) -> np.ndarray:
+    """chained step (numConsecutiveJWIterations = K) of the oracle: bufs[i] are the
+    matrices of iteration i (one array K times without multiple input buffers)"""
+    out = rhs.copy()
+    arr = (dptr * K)(*[P(b) for b in bufs])
+    rc = _chain_fn("pfo_jw_chain_f128" if f128 else "pfo_jw_chain_f64")(C.byref(p), K, arr, P(out))
+    assert rc == 0, rc
+    return out

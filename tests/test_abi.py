@@ -254,3 +254,24 @@ def test_c_caller_compiles_against_the_header(tmp_path):
     import c_caller
     exe = c_caller.build(str(tmp_path))
     assert os.path.exists(exe)
+
+
+def test_chained_iterations_validation():
+    """numConsecutiveJWIterations K > 1: the bidiagonal modes (jw 10/11/12, upperBanded 0/1)
+    in FP64 with every iteration's system within 1024 rows; useMultipleInputBuffers is 0 or K"""
+    ok = dict(N=512, jw_type=10, upperBanded=1, numConsecutiveJWIterations=4)
+    assert validate(cfg(**ok))[0] == 0
+    assert validate(cfg(**dict(ok, useMultipleInputBuffers=4)))[0] == 0
+    assert validate(cfg(**dict(ok, useMultipleInputBuffers=3)))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(**dict(ok, jw_type=1012, upperBanded=0)))[0] == 0
+    assert validate(cfg(**dict(ok, jw_type=11, upperBanded=0)))[0] == 0
+    assert validate(cfg(**dict(ok, jw_type=13, upperBanded=2)))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(**dict(ok, jw_type=14, upperBanded=0)))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(**dict(ok, doublePrecision=0, quadDoubleDoublePrecision=1)))[0] == \
+        _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(**dict(ok, numConsecutiveJWIterations=17)))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    # growing systems: the last one has N + K - 1 rows
+    assert validate(cfg(**dict(ok, N=1021, upperBanded=0)))[0] == 0
+    assert validate(cfg(**dict(ok, N=1022, upperBanded=0)))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    # shrinking systems keep at least one row
+    assert validate(cfg(**dict(ok, N=3, upperBanded=1)))[0] == _abi.PFSOLVE_ERROR_EMPTY_size

@@ -545,3 +545,49 @@ def test_gram_tridiagonal_oracle_residual(N, l):
     assert J.tridiag_residual(a, b, c, yq, x) < 1e-14 * max(1.0, np.max(np.abs(yq)) * np.max(np.abs(b)))
     cond = np.linalg.cond(np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1), np.inf)
     assert O.relerr_norm(O.jw_f64(p, buf, x), yq) < 4 * cond * 2.0 ** -53
+
+
+# ---------------------------------------------------------------- chained iterations
+@pytest.mark.parametrize("ub", [1, 0])
+@pytest.mark.parametrize("jw", [0, 1, 2])
+@pytest.mark.parametrize("N,K", [(9, 1), (9, 2), (40, 4), (64, 8), (100, 3), (200, 5)])
+@pytest.mark.parametrize("rt", [0, 1])
+def test_chain_against_dense_composition(ub, jw, N, K, rt):
+    """The oracle's chained step (numConsecutiveJWIterations = K) against the composition of
+    dense rectangular operators solved by LAPACK: iteration i maps N_{i-1} values to
+    N_i = N -/+ i, its matvec diagonal is 1 except in the first (shrinking) or last (growing)
+    iteration, the solve's row-0 scaleC applies after every solve; input keep-range on the
+    first read, output keep-range on the last write; junk everywhere the layout leaves free.
+    K = 1 is also checked against the single-step oracle."""
+    import jw_chain as JC
+    B, S = 3, N + K + 4
+    bufs, dense, Ns = JC.build(N, K, jw, ub, rt, seed=10 * N + K + jw)
+    x = O.uniform(N + 7 * K, -1, 1, B * S + 3)
+    izp, ozp = (1, N), (0, max(1, Ns[-1] - 1))
+    mask = RT["SOL"] | RT["OSTR"] | RT["IZP"] | RT["OZP"] | RT["SC"] | ((RT["M"] | RT["V"]) if rt else 0)
+    p = O.problem(jw_type=10 + jw, upperBanded=ub, N=N, B=B, mask=mask, cfg_outStride=N, cfg_scaleC=1.0,
+                  run_offsetM=7, run_offsetV=0, run_offsetSolution=2, run_outStride=S,
+                  run_izp=list(izp), run_ozp=list(ozp), run_scaleC=0.8)
+    if rt:   # runtime offsetV is one value for every iteration: the solve block sits behind the rest
+        bufs, dense, Ns = JC.build(N, K, jw, ub, rt, seed=10 * N + K + jw, offV=7 + 2 * (N + K))
+        p.run_offsetV = 7 + 2 * (N + K)
+    got = O.jw_chain(p, K, bufs, x)
+    got64 = O.jw_chain(p, K, bufs, x, f128=False)
+    for b in range(B):
+        base = b * S + 2
+        x0 = x[base:base + N].copy()
+        x0[:izp[0]] = 0
+        want = JC.apply_dense(dense, Ns, x0, 0.8)
+        keep = np.zeros(Ns[-1], bool)
+        keep[ozp[0]:ozp[1]] = True
+        g = got[base:base + Ns[-1]]
+        assert O.relerr_norm(g[keep], want[keep]) < 1e-14
+        assert O.relerr_norm(got64[base:base + Ns[-1]][keep], want[keep]) < 1e-13
+        np.testing.assert_array_equal(g[~keep], x[base:base + Ns[-1]][~keep])
+    mask_out = np.ones(x.size, bool)
+    for b in range(B):
+        mask_out[b * S + 2 + ozp[0]:b * S + 2 + ozp[1]] = False
+    np.testing.assert_array_equal(got[mask_out], x[mask_out])
+    if K == 1:
+        single = O.jw_f128(p, bufs[0], x)
+        np.testing.assert_array_equal(got, single)
