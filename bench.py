@@ -385,6 +385,111 @@ def bench_dd(torch, pf, steps: int, warmup: int, N: int = 256, B: int = 16384) -
     }
 
 
+def chain_matrices(N: int, K: int, reorder) -> list:
+    """Matrices of a chained shrinking step (jw 10, upperBanded 1): iteration i maps N_{i-1}
+    values to N_i = N - i through the Jones-Worland connection matrix of degree 2N (upper
+    bidiagonal, diagonal mu(r) in the first iteration only, superdiagonal nu(r+1)) and solves
+    the lower unit-bidiagonal normalised connection matrix; reference storage per iteration
+    ([D | U | L_S] at [0, N_0, 2 N_0) first, then [U | L_S] at [0, N_i)), reorder_i0 order of
+    N_i rows in the layout of N rows."""
+    def mu(n, b):
+        return np.sqrt(2 * (n + b) * (n - 0.5 + b) / (2 * n - 0.5 + b) / (2 * n + 0.5 + b))
+
+    def nu(n, b):
+        return np.sqrt(2 * (n + 1) * (n + 0.5) / (2 * n + 0.5 + b) / (2 * n + 1.5 + b))
+    bufs = []
+    for i in range(K):
+        Ni = N - i
+        P = np.array([reorder(r, Ni, N, 0) for r in range(Ni)], dtype=np.int64)
+        r = np.arange(Ni, dtype=np.float64)
+        dA, eA = mu(r, 2 * N + 1.5), nu(r + 1, 2 * N + 1.5)
+        dS = mu(r, 2 * N + 0.5)
+        lS = np.zeros(Ni)
+        lS[1:] = nu(r[1:], 2 * N + 0.5) / dS[1:]
+        buf = np.zeros(3 * Ni)
+        if i == 0:
+            buf[P] = dA
+            buf[Ni + P] = eA
+            buf[2 * Ni + P] = lS
+        else:
+            buf[P] = eA
+            buf[Ni + P] = lS
+        bufs.append(buf)
+    return bufs
+
+
+def bench_chain(torch, pf, steps: int, warmup: int, N: int = 512, ncx: int = 65536, K: int = 4) -> dict:
+    """Chained JW steps (numConsecutiveJWIterations = K): K shrinking Jones-Worland
+    iterations on 65536 complex systems (XXII), one launch that keeps every pair of systems
+    in registers through the K iterations, against the same K iterations as K separate
+    single-step launches (each reading and writing every system)."""
+    lib = pf.lib()
+    nreal = 2 * ncx
+    bufs = chain_matrices(N, K, lib.pfsolve_reorder_i0)
+    dmats = [torch.from_numpy(b).cuda() for b in bufs]
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.rand(nreal * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    stream = torch.cuda.Stream()
+    plan = pf.Plan([m.data_ptr() for m in dmats], x.data_ptr(), stream=stream.cuda_stream, size=[N, nreal, 1],
+                   jw_type=10, upperBanded=1, doublePrecision=1, scaleC=1.0, numConsecutiveJWIterations=K,
+                   useMultipleInputBuffers=K, outputBufferStride=[N])
+    ms, per, _, _ = time_plan_steps(torch, lib, plan, stream, warmup, steps, plan.append)
+    # the chained kernel alone (without the tables' launch): events around each kernel
+    kst = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ken = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    for e in kst + ken:
+        e.record(stream)
+    stream.synchronize()
+    with torch.cuda.stream(stream):
+        for i in range(steps):
+            lib.pfsolve_set_kernel_events(C.byref(plan.app), C.c_void_p(kst[i].cuda_event),
+                                          C.c_void_p(ken[i].cuda_event))
+            plan.append()
+    stream.synchronize()
+    lib.pfsolve_set_kernel_events(C.byref(plan.app), None, None)
+    kms = float(np.mean([kst[i].elapsed_time(ken[i]) for i in range(steps)]))
+    plan.close()
+    assert torch.isfinite(x).all().item(), "non-finite chained output"
+    # unchained: K single-step plans (iteration i: N_i rows, its own matrices), each a
+    # full read and write of the right-hand sides
+    plans = []
+    for i in range(K):
+        Ni = N - i
+        # a single step reads [D | U | L_S]: iterations after the first get a unit diagonal block
+        b = np.concatenate([np.ones(Ni), bufs[i]]) if i > 0 else bufs[0]
+        dm = torch.from_numpy(np.ascontiguousarray(b)).cuda()
+        dmats.append(dm)
+        plans.append(pf.Plan(dm.data_ptr(), x.data_ptr(), stream=stream.cuda_stream, size=[Ni, nreal, 1],
+                             jw_type=10, upperBanded=1, doublePrecision=1, scaleC=1.0, outputBufferStride=[N]))
+
+    def unchained():
+        for p in plans:
+            p.append()
+    ms_un, _, _, _ = time_plan_steps(torch, lib, plans[0], stream, warmup, steps, unchained)
+    for p in plans:
+        p.close()
+    nlast = N - K + 1
+    alg = 8 * nreal * (N + nlast) + sum(8 * b.size for b in bufs)
+    alg_un = sum(8 * nreal * 2 * (N - i) for i in range(K)) + sum(8 * b.size for b in bufs)
+    return {
+        "workload": f"chained JW step, {K} iterations (jw_type 10, upperBanded 1: N = {N} -> {nlast} rows), "
+                    f"Jones-Worland connection matrices of degree 2N, {ncx} complex systems (XXII), FP64",
+        "value": ncx / (ms * 1e-3), "unit": "complex systems/s (K-step)", "ms_per_step": ms,
+        "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+        "kernel": f"jw_chain64_kernel<{plan_rows(N)}, false>", "kernel_ms": kms,
+        "bytes_per_k_step": alg, "hbm_frac": alg / (kms * 1e-3) / HBM_PEAK,
+        "unchained_ms_per_step": ms_un, "unchained_bytes_per_k_step": alg_un,
+        "speedup_vs_unchained": ms_un / ms,
+    }
+
+
+def plan_rows(N: int) -> int:
+    r = 2
+    while r * 64 < N:
+        r *= 2
+    return r
+
+
 def bench_fused_blocks(torch, pf, steps: int, warmup: int, N: int = 512, ncx: int = 65536) -> dict:
     """Block operators fused around the step: complex systems read through block
     layout 4 with operator ScaleD (204), the step, written through layout 7 with
@@ -530,6 +635,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-size batch measurement")
     ap.add_argument("--no-dd", action="store_true", help="skip the double-double measurement")
     ap.add_argument("--no-fused-blocks", action="store_true", help="skip the fused block I/O measurement")
+    ap.add_argument("--no-chain", action="store_true", help="skip the chained-iterations measurement")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the serving stream (60 = the full stream; 0 skips it)")
     ap.add_argument("--no-stream-reference", action="store_true",
@@ -674,6 +780,8 @@ def main() -> int:
         result["dd"] = bench_dd(torch, pf, max(5, K // 2), 2)
     if not args.no_fused_blocks:
         result["fused_blocks"] = bench_fused_blocks(torch, pf, max(10, K // 2), 3)
+    if not args.no_chain:
+        result["chain"] = bench_chain(torch, pf, max(10, K // 2), 3)
     if args.serve_seconds > 0:
         result["serve"] = bench_serve(torch, pf, args.serve_seconds)
     if args.calibrate or not args.no_stream_reference:
