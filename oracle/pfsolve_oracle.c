@@ -74,6 +74,46 @@ int64_t pfo_reorder_i0(int64_t i, int64_t N, int64_t Nalign, int64_t Msplit0)
     return ret + ret_off;
 }
 
+/* rows per lane and lanes of the ALT layout (HIP branch of get_ALT_stride and
+ * reorder_i_alt, PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:30-64,144-219) */
+static void alt_geometry(int64_t size1, int64_t* W, int64_t* R)
+{
+    int64_t ws = ((size1 + 1023) / 1024) * 64;
+    if (size1 < 512) ws = 64;
+    if (ws < 64) ws = 64;
+    if (ws % 64) ws = 64 * ((ws + 63) / 64);
+    int64_t r = (size1 + ws - 1) / ws;
+    if (r % 2 == 0) r++;
+    *W = ws;
+    *R = r;
+}
+
+/* block stride of one diagonal in the ALT storage: rows per lane x lanes */
+int64_t pfo_alt_stride(int64_t Nalign, int64_t Msplit0)
+{
+    if (Msplit0 == 0) Msplit0 = 704;
+    int64_t size1 = Nalign > 3840 ? Msplit0 : Nalign;
+    int64_t W, R;
+    alt_geometry(size1, &W, &R);
+    return W * R;
+}
+
+/* storage position of row i in the ALT layout: row i belongs to the subsystem of its
+ * parity (lanes [0, W/2) even, [W/2, W) odd), as row k = i/2 of that subsystem, held by
+ * lane k / R of the half in register k % R */
+int64_t pfo_reorder_i_alt(int64_t i, int64_t N, int64_t Nalign, int64_t Msplit0)
+{
+    if (Msplit0 == 0) Msplit0 = 704;
+    int64_t size0 = N, size1 = Nalign ? Nalign : N;
+    if (N > 3840) { size0 = Msplit0; size1 = Msplit0; }
+    int64_t W, R;
+    alt_geometry(size1, &W, &R);
+    const int64_t ret_off = (i / size0) * size0;
+    const int64_t li = i % size0;
+    const int64_t k = li / 2;
+    return (k % R) * W + (li % 2) * (W / 2) + k / R + ret_off;
+}
+
 /* PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:274-283,311-347 */
 int pfo_jw_registers_per_thread(int64_t N, int K, int upperBanded)
 {
@@ -127,6 +167,7 @@ typedef struct jw_resolved {
     int izp_rt, ozp_rt, scale_rt;
     double scaleC;
     int64_t* P;                     /* storage permutation of matrix diagonals */
+    int alt;                        /* ALT: rows couple with rows +-2 (two interleaved systems) */
 } jw_resolved;
 
 static void free_resolved(jw_resolved* r) { free(r->P); r->P = NULL; }
@@ -139,10 +180,18 @@ static int resolve_problem(const pfo_jw_problem* p, jw_resolved* rs)
     if (N < 1) return -1;
     int natural = p->jw_type > 1000;
     int jwt = p->jw_type % 1000;
-    if (jwt < 10 || jwt > 14) return -2;       /* hundreds digit = ALT: not restated */
+    /* hundreds digit 1 = ALT (pfSolve_Plan_JonesWorland.h:265) */
+    const int alt = jwt / 100 == 1;
+    if (alt) jwt -= 100;
+    if (jwt < 10 || jwt > 14) return -2;
     int jw = jwt % 10;
     const int64_t m = p->mask;
     int rtM = (m & PFO_RUNTIME_OFFSETM) != 0, rtV = (m & PFO_RUNTIME_OFFSETV) != 0;
+    /* ALT restated for the parallel layout with baked offsets: each baked position
+     * (0, N, 2N) becomes a multiple of the ALT block stride (pfSolve_JonesWorland.h:205-222) */
+    if (alt && (natural || rtM || rtV || jw == 4)) return -8;
+    rs->alt = alt;
+    const int64_t NB = alt ? pfo_alt_stride(N, p->Msplit0) : N;   /* block stride of baked positions */
     int64_t offM = rtM ? p->run_offsetM : p->cfg_offsetM;
     int64_t offV = rtV ? p->run_offsetV : p->cfg_offsetV;
     int64_t offT = (m & PFO_RUNTIME_OFFSETT) ? p->run_offsetT : p->cfg_offsetT;
@@ -160,7 +209,8 @@ static int resolve_problem(const pfo_jw_problem* p, jw_resolved* rs)
     rs->P = (int64_t*)malloc(sizeof(int64_t) * (size_t)N);
     if (!rs->P) return -100;
     int permuted = !natural && pfo_jw_uses_parallel_thomas(N, 1, p->jw_type);
-    for (int64_t r = 0; r < N; r++) rs->P[r] = permuted ? pfo_reorder_i0(r, N, N, p->Msplit0) : r;
+    for (int64_t r = 0; r < N; r++)
+        rs->P[r] = alt ? pfo_reorder_i_alt(r, N, N, p->Msplit0) : (permuted ? pfo_reorder_i0(r, N, N, p->Msplit0) : r);
 
     const int ub = p->upperBanded;
     if (jw == 4) {
@@ -174,24 +224,24 @@ static int resolve_problem(const pfo_jw_problem* p, jw_resolved* rs)
     }
     if (ub == 2) {
         if (jw != 3) { free_resolved(rs); return -3; }   /* only diagonal matvec + tri solve is defined */
-        rs->ldS = 0; rs->mdA = N; rs->udS = 2 * N;
+        rs->ldS = 0; rs->mdA = NB; rs->udS = 2 * NB;
         rs->mv = MV_DIAG; rs->sv = SV_TRI;
         return 0;
     }
     if (ub != 0 && ub != 1) { free_resolved(rs); return -4; }
     if (jw == 0 || jw == 1) {
-        if (ub) { rs->mv = MV_UPPER; rs->udA = rtM ? offM : N; rs->mdA = rtM ? offM + N : 0; }
-        else    { rs->mv = MV_LOWER; rs->mdA = rtM ? offM : N; rs->ldA = rtM ? offM + N : 0; }
+        if (ub) { rs->mv = MV_UPPER; rs->udA = rtM ? offM : NB; rs->mdA = rtM ? offM + N : 0; }
+        else    { rs->mv = MV_LOWER; rs->mdA = rtM ? offM : NB; rs->ldA = rtM ? offM + N : 0; }
     }
     if (jw == 0 || jw == 2) {
-        if (ub) { rs->sv = SV_LOWER; rs->ldS = rtV ? offV : 2 * N; }
-        else    { rs->sv = SV_UPPER; rs->udS = rtV ? offV : 2 * N; }
+        if (ub) { rs->sv = SV_LOWER; rs->ldS = rtV ? offV : 2 * NB; }
+        else    { rs->sv = SV_UPPER; rs->udS = rtV ? offV : 2 * NB; }
     }
     if (jw == 3) {
         /* upperBanded is toggled for the load (pfSolve_JonesWorland.h:124-127,202-205) */
         rs->mv = MV_DIAG;
-        if (ub) { rs->sv = SV_LOWER; rs->mdA = rtM ? offM : N; rs->ldS = rtM ? offM + N : 0; }
-        else    { rs->sv = SV_UPPER; rs->udS = rtM ? offM : N; rs->mdA = rtM ? offM + N : 0; }
+        if (ub) { rs->sv = SV_LOWER; rs->mdA = rtM ? offM : NB; rs->ldS = rtM ? offM + N : 0; }
+        else    { rs->sv = SV_UPPER; rs->udS = rtM ? offM : NB; rs->mdA = rtM ? offM + N : 0; }
     }
     return 0;
 }

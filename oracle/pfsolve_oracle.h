@@ -44,6 +44,9 @@ void pfo_fill_uniform(uint64_t seed, double a, double b, double* out, int64_t n)
 
 /* ---- layout maths ---- */
 int64_t pfo_reorder_i0(int64_t i, int64_t N, int64_t Nalign, int64_t Msplit0);
+/* ALT layout (jw_type hundreds digit 1): block stride of a diagonal and position of row i */
+int64_t pfo_alt_stride(int64_t Nalign, int64_t Msplit0);
+int64_t pfo_reorder_i_alt(int64_t i, int64_t N, int64_t Nalign, int64_t Msplit0);
 int pfo_jw_registers_per_thread(int64_t N, int K, int upperBanded);
 int pfo_jw_uses_parallel_thomas(int64_t N, int K, int jw_type);
 /* 1 when row is read/written; runtime=1 means the [zp0,zp1) pair came from the launch params */

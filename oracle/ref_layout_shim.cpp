@@ -6,6 +6,7 @@
 // reference's own code.
 //   reorder_i0             PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:66-142
 //   get_ALT_stride         PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:30-64
+//   reorder_i_alt          PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:144-219
 //   getPfSolveErrorString  PfSolve/pfSolve_Structs/pfSolve_Structs.h:545-733
 //   PfSolveGetVersion      PfSolve/pfSolve.h:101-103
 //   PfSolveConfiguration, PfSolveLaunchParams, PfSolveApplication (HIP build)
@@ -84,6 +85,10 @@ int ref_reorder_i0(int i, int M_size, int M_size_align, int Msplit0)
     return reorder_i0(i, M_size, M_size_align, Msplit0);
 }
 int ref_get_ALT_stride(int M_size_align, int Msplit0) { return get_ALT_stride(M_size_align, Msplit0); }
+int ref_reorder_i_alt(int i, int M_size, int M_size_align, int Msplit0)
+{
+    return reorder_i_alt(i, M_size, M_size_align, Msplit0);
+}
 const char* ref_error_string(int code) { return getPfSolveErrorString((PfSolveResult)code); }
 int ref_version(void) { return PfSolveGetVersion(); }
 

@@ -180,6 +180,20 @@ __device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N, int64_t Nal
 }
 
 
+// Storage position of row i in the ALT layout (the reference's reorder_i_alt,
+// PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:144-219, N <= 3840):
+// row i is row k = i/2 of the subsystem of its parity, held in register k % R by lane k / R
+// of the parity's half wavefront (W lanes, R rows per lane of the layout of N rows).
+__device__ __forceinline__ int64_t alt_pos(int64_t i, int64_t N)
+{
+    int64_t W = ((N + 1023) / 1024) * 64;
+    if (N < 512) W = 64;
+    int64_t R = (N + W - 1) / W;
+    if ((R & 1) == 0) R++;
+    const int64_t k = i >> 1;
+    return (k % R) * W + (i & 1) * (W >> 1) + k / R;
+}
+
 // ------------------------------------------------------------------------
 // DPP moves of a double (two 32-bit v_mov_b32_dpp); lanes without a source
 // receive 0 (bound_ctrl).  CTRL: row_shl:n = 0x100+n, row_shr:n = 0x110+n,

@@ -137,11 +137,12 @@ __global__ __launch_bounds__(512) void jw_main_kernel(jw_run_desc d)
                     continue;
                 }
             }
+            const int64_t rs = d.row_step > 1 ? d.row_step : 1;
 #pragma unroll
             for (int i = 0; i < R; i++) {
                 const int r = r0 + i;
                 const bool k = ok && r >= d.in_lo && r < d.in_hi;
-                x[s][i] = k ? mem_load<T>(d.rhs, base + r, mf) : A::zero();
+                x[s][i] = k ? mem_load<T>(d.rhs, base + r * rs, mf) : A::zero();
             }
         }
 
@@ -315,10 +316,11 @@ __global__ __launch_bounds__(512) void jw_main_kernel(jw_run_desc d)
                     continue;
                 }
             }
+            const int64_t rs = d.row_step > 1 ? d.row_step : 1;
 #pragma unroll
             for (int i = 0; i < R; i++) {
                 const int r = r0 + i;
-                if (ok && r >= d.out_lo && r < d.out_hi) mem_store<T>(d.rhs, base + r, x[s][i], mf);
+                if (ok && r >= d.out_lo && r < d.out_hi) mem_store<T>(d.rhs, base + r * rs, x[s][i], mf);
             }
         }
     }
@@ -604,6 +606,8 @@ int jwdev_prologue_batch(const jw_prologue_job* d_jobs, int njobs, int max_RL, i
 
 int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream)
 {
+    /* a launch without its right-hand sides or table is a host bug: an error, not a fault */
+    if (!r->rhs || !r->table) { snprintf(g_err, sizeof(g_err), "main: no rhs or table"); return -1; }
     const void* f = pick_main(precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD, L, R);
     if (!f) { snprintf(g_err, sizeof(g_err), "no kernel for L=%d R=%d prec=%d", L, R, precision); return -1; }
     const size_t lds = (size_t)jwdev_lds_bytes(L, R, precision);

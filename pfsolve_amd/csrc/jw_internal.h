@@ -62,6 +62,10 @@ typedef struct jw_source_desc {
      * implicit 1 (not read) */
     int64_t Ncols, Nalign;
     int32_t unit_diag, pad_chain;
+    /* ALT variant: this descriptor is the subsystem of the rows of parity alt_parity of
+     * systems of alt_N rows, stored in reorder_i_alt order (alt = 1) */
+    int32_t alt, alt_parity;
+    int64_t alt_N;
 } jw_source_desc;
 
 /* Tables of a chained step (numConsecutiveJWIterations = K <= JW_CHAIN_MAX), built by one
@@ -102,7 +106,9 @@ typedef struct jw_run_desc {
     /* polls before a bounded wait of the compute / store split kernel gives up:
      * 0 the default (2^22), negative none (the time-out path at once) */
     int32_t wait_polls;
-    int32_t pad_wait;
+    /* elements between consecutive rows of a system (0 or 1: contiguous; 2: one parity
+     * subsystem of the ALT variant) -- general kernel only */
+    int32_t row_step;
 } jw_run_desc;
 
 /* One matrix set for the batched prologue (one workgroup each). */

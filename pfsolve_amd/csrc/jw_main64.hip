@@ -1397,6 +1397,7 @@ int jwdev_main64_occupancy(int R, int waves)
 
 int jwdev_main64(const jw_run_desc* r, int R, int waves, int grid, void* stream)
 {
+    if (!r->rhs || !r->table) { snprintf(g_err64, sizeof(g_err64), "main64: no rhs or table"); return -1; }
     const void* f = pick64(R, r->fwd && r->b1 && r->b2);
     if (!f) { snprintf(g_err64, sizeof(g_err64), "no 64-lane kernel for R=%d", R); return -1; }
     const size_t lds = (size_t)jwdev_main64_lds_bytes(R, waves);
@@ -1442,6 +1443,7 @@ int jwdev_chain64_occupancy(int R)
 
 int jwdev_chain64(const jw_run_desc* r, const void* tables, int64_t tab_elems, int K, int R, int grid, void* stream)
 {
+    if (!r->rhs || !tables) { snprintf(g_err64, sizeof(g_err64), "chain64: no rhs or tables"); return -1; }
     const void* f = pick_chain(R, r->fwd && r->b1 && r->b2);
     if (!f || K < 1) { snprintf(g_err64, sizeof(g_err64), "no chained-step kernel for R=%d", R); return -1; }
     const size_t lds = (size_t)chain_lds_bytes(R);
@@ -1581,6 +1583,7 @@ int jwdev_main64s_occupancy(int R)
 int jwdev_main64s(const jw_run_desc* r, const jw_source_desc* src, const void* matrices, int R, int grid, int* d_err,
                   void* stream)
 {
+    if (!r->rhs || !matrices || !d_err) { snprintf(g_err64, sizeof(g_err64), "main64s: no rhs, matrices or flag"); return -1; }
     int c = 0, st = 0;
     if (!jwdev_main64s_shape(R, &c, &st)) { snprintf(g_err64, sizeof(g_err64), "no split kernel for R=%d", R); return -1; }
     const void* f = pick_split(R, r->fwd && r->b1 && r->b2, (int)r->fused_io);

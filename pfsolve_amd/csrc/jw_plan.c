@@ -57,9 +57,20 @@ PfSolveResult jw_configure(const PfSolveConfiguration* in, PfSolveConfiguration*
     const int64_t N = out->M_size;
 
     if (kind == PF_KIND_JW) {
-        const int jwt = out->jw_type % 1000;
-        if (jwt < 10 || jwt > 14) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;   /* ALT (hundreds digit) not built */
+        const int alt = (out->jw_type % 1000) / 100 == 1;     /* hundreds digit: ALT */
+        const int jwt = alt ? out->jw_type % 100 : out->jw_type % 1000;
+        if (jwt < 10 || jwt > 14 || (out->jw_type % 1000) / 100 > 1) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
         const int jw = jwt % 10;
+        if (alt) {
+            /* ALT (even and odd rows two interleaved systems, pfSolve_Plan_JonesWorland.h:265,
+             * pfSolve_TridiagonalSolve.h:1622-1646): the JW modes in the parallel layout with baked
+             * matrix positions, one iteration; each parity's system on the general kernel */
+            if (jw == 4 || out->jw_type > 1000 || out->numConsecutiveJWIterations != 1 ||
+                (out->jw_control_bitmask & (RUNTIME_OFFSETM | RUNTIME_OFFSETV)) || out->inputBlock ||
+                out->outputBlock || out->kernelVariant == 2)
+                return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
+            if ((N + 1) / 2 > (prec == JW_PREC_F64 ? 1024 : JW_MAX_N_DD)) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        }
         const int K = out->numConsecutiveJWIterations;
         if (out->upperBanded < 0 || out->upperBanded > 2) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
         if (out->useMultipleInputBuffers != 0 && out->useMultipleInputBuffers != K) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
@@ -79,8 +90,8 @@ PfSolveResult jw_configure(const PfSolveConfiguration* in, PfSolveConfiguration*
             if (out->KL == 0 && out->KU == 0) { out->KL = 2; out->KU = 2; }
             if (out->KL < 0 || out->KU < 0 || out->KL > 2 || out->KU > 2) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
         }
-        if (prec == JW_PREC_F64 && N > JW_MAX_N_F64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
-        if (prec != JW_PREC_F64 && N > JW_MAX_N_DD) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        if (!alt && prec == JW_PREC_F64 && N > JW_MAX_N_F64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        if (!alt && prec != JW_PREC_F64 && N > JW_MAX_N_DD) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         /* block operators fused around the step: layout 4 on the input (complex in, no
          * accumulate), and on the output layout 7 (complex out, optional accumulate)
          * or none (the step's real systems) */
@@ -141,6 +152,19 @@ void jw_keep_range(int64_t zp0, int64_t zp1, int runtime, int64_t N, int64_t* lo
     *hi = b;
 }
 
+/* Block stride of one diagonal in the ALT storage: the reference's get_ALT_stride
+ * (PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:30-64, HIP branch):
+ * rows per lane (odd) times lanes of the layout of N rows. */
+int64_t jw_alt_stride(int64_t N, uint64_t Msplit0)
+{
+    int64_t size1 = N > 3840 ? (int64_t)(Msplit0 ? Msplit0 : 704) : N;
+    int64_t W = ((size1 + 1023) / 1024) * 64;
+    if (size1 < 512) W = 64;
+    int64_t R = (size1 + W - 1) / W;
+    if (R % 2 == 0) R++;
+    return R * W;
+}
+
 static int64_t eff(int64_t mask, int64_t bit, int64_t cfgv, uint64_t runv)
 {
     return (mask & bit) ? (int64_t)runv : cfgv;
@@ -160,11 +184,16 @@ PfSolveResult jw_resolve_launch(const PfSolveConfiguration* cfg, const PfSolveLa
     const int64_t offM = eff(m, RUNTIME_OFFSETM, cfg->offsetM, lp->offsetM);
     const int64_t offV = eff(m, RUNTIME_OFFSETV, cfg->offsetV, lp->offsetV);
     const int64_t offT = eff(m, RUNTIME_OFFSETT, cfg->offsetT, lp->offsetT);
-    const int jwt = cfg->jw_type % 1000;
+    const int alt = (cfg->jw_type % 1000) / 100 == 1;
+    const int jwt = cfg->jw_type % 100;
     const int jw = jwt % 10;
     const int ub = cfg->upperBanded;
+    /* baked positions 0, N, 2N; in the ALT variant multiples of its block stride
+     * (pfSolve_JonesWorland.h:205-222) */
+    const int64_t NB = alt ? jw_alt_stride(N, cfg->Msplit[0]) : N;
 
     src->N = N;
+    src->alt = alt;
     src->mdS = -1;
     src->Msplit0 = (int32_t)cfg->Msplit[0];
     src->permuted = cfg->jw_type < 1000 && !(N <= 64 && cfg->numConsecutiveJWIterations == 1);
@@ -186,30 +215,30 @@ PfSolveResult jw_resolve_launch(const PfSolveConfiguration* cfg, const PfSolveLa
     } else if (ub == 2) {
         src->band_kind = JW_BAND_DIAG;
         src->ldS = 0;
-        src->mdA = N;
-        src->udS = 2 * N;
+        src->mdA = NB;
+        src->udS = 2 * NB;
         src->solve_kind = JW_SOLVE_TRI;
     } else {
         if (jw == 0 || jw == 1) {
             if (ub) {
                 src->band_kind = JW_BAND_UPPER;
-                src->udA = rtM ? offM : N;
+                src->udA = rtM ? offM : NB;
                 src->mdA = rtM ? offM + N : 0;
             } else {
                 src->band_kind = JW_BAND_LOWER;
-                src->mdA = rtM ? offM : N;
+                src->mdA = rtM ? offM : NB;
                 src->ldA = rtM ? offM + N : 0;
             }
         }
         if (jw == 0 || jw == 2) {
-            if (ub) { src->solve_kind = JW_SOLVE_LOWER; src->ldS = rtV ? offV : 2 * N; }
-            else    { src->solve_kind = JW_SOLVE_UPPER; src->udS = rtV ? offV : 2 * N; }
+            if (ub) { src->solve_kind = JW_SOLVE_LOWER; src->ldS = rtV ? offV : 2 * NB; }
+            else    { src->solve_kind = JW_SOLVE_UPPER; src->udS = rtV ? offV : 2 * NB; }
         }
         if (jw == 3) {
             /* upperBanded toggled for the load (pfSolve_JonesWorland.h:124-127,202-205) */
             src->band_kind = JW_BAND_DIAG;
-            if (ub) { src->solve_kind = JW_SOLVE_LOWER; src->mdA = rtM ? offM : N; src->ldS = rtM ? offM + N : 0; }
-            else    { src->solve_kind = JW_SOLVE_UPPER; src->udS = rtM ? offM : N; src->mdA = rtM ? offM + N : 0; }
+            if (ub) { src->solve_kind = JW_SOLVE_LOWER; src->mdA = rtM ? offM : NB; src->ldS = rtM ? offM + N : 0; }
+            else    { src->solve_kind = JW_SOLVE_UPPER; src->udS = rtM ? offM : NB; src->mdA = rtM ? offM + N : 0; }
         }
     }
 

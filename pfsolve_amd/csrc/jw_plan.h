@@ -36,6 +36,9 @@ struct pfsolve_plan {
      * workgroups per CU; a slot's table area holds K format-2 tables of chain_tab bytes */
     int chain_K, chain_R, chain_occ;
     size_t chain_tab;
+    /* ALT variant: two parity subsystems, one format-1 table of alt_tab bytes each */
+    int alt;
+    size_t alt_tab;
 };
 
 /* Applies the reference's configuration defaults (pfSolve_InitializeApp.h:981-1212)
@@ -55,6 +58,8 @@ PfSolveResult jw_resolve_block(const PfSolveConfiguration* cfg, const PfSolveLau
                                int64_t nsys, jw_block_desc* d);
 PfSolveResult jw_resolve_gbmv(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp,
                               int64_t nsys, jw_gbmv_desc* d);
+/* block stride of a diagonal in the ALT storage (get_ALT_stride) */
+int64_t jw_alt_stride(int64_t N, uint64_t Msplit0);
 /* iteration i of a chained step (numConsecutiveJWIterations > 1) */
 PfSolveResult jw_resolve_chain_iteration(const PfSolveConfiguration* cfg, const PfSolveLaunchParams* lp, int i,
                                          jw_source_desc* src);

@@ -139,7 +139,8 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
 #pragma unroll
         for (int k = 0; k < 5; k++) vb[j][k] = A::zero();
         if (r < N) {
-            const int64_t P = d.permuted ? reorder_pos(r, N, d.Nalign) : (int64_t)r;
+            const int64_t P = d.alt ? alt_pos(2 * (int64_t)r + d.alt_parity, d.alt_N)
+                                    : (d.permuted ? reorder_pos(r, N, d.Nalign) : (int64_t)r);
             row_band<T>(d, mat, r, N, P, vb[j]);
             if (has_f && r >= 1) ldv = mem_load<T>(mat, d.ldS + P, mf);
             if (has_b1 && r + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);

@@ -41,7 +41,9 @@ static PfSolveResult fill_geometry(const PfSolveConfiguration* cfg, pfsolve_geom
     g->waves = 4;
     if (jw_plan_kind(cfg) != PF_KIND_JW) return PFSOLVE_SUCCESS;
     int L = 0, R = 0;
-    PfSolveResult r = jw_select_geometry(cfg->M_size, g->precision, &L, &R);
+    /* the ALT variant solves two subsystems of ceil(N/2) and floor(N/2) rows */
+    const int alt = (cfg->jw_type % 1000) / 100 == 1;
+    PfSolveResult r = jw_select_geometry(alt ? (cfg->M_size + 1) / 2 : cfg->M_size, g->precision, &L, &R);
     if (r != PFSOLVE_SUCCESS) return r;
     if (R > 16) {
         /* long systems: only the wave-per-system kernel (no general-kernel geometry) */
@@ -151,6 +153,18 @@ static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
         const int long_n = p->L == 0;   /* N > 1024: the wave-per-system long kernel only */
         if (long_n && cfg->kernelVariant == 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         if (!long_n && !jwdev_supported(p->L, p->R, kp)) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        if ((cfg->jw_type % 1000) / 100 == 1) {
+            /* ALT: each parity subsystem on the general kernel (rows two elements apart), one
+             * format-1 table per parity */
+            if (long_n) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+            p->alt = 1;
+            p->alt_tab = (p->table_bytes + 255) / 256 * 256;
+            p->table_bytes = 2 * p->alt_tab;
+            p->table_stride = (p->table_bytes + 255) / 256 * 256;
+            p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
+            if (p->occupancy < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
+            return PFSOLVE_SUCCESS;
+        }
         /* wave-per-system path: FP64, 64 lanes, R a power of two in [2, 32] */
         if (p->precision == JW_PREC_F64 && cfg->kernelVariant != 1) {
             int r64 = 2;
@@ -369,6 +383,79 @@ static PfSolveResult append_chain(PfSolveApplication* app, const PfSolveLaunchPa
     return PFSOLVE_SUCCESS;
 }
 
+/* ALT variant (jw_type hundreds digit 1): rows couple with rows two apart, so the even and
+ * the odd rows of every system are two independent systems of ceil(N/2) and floor(N/2)
+ * rows (pfSolve_TridiagonalSolve.h:1622-1646, pfSolve_MatVecMul.h:295-325); their matrix
+ * entries are interleaved in reorder_i_alt order (pfSolve_Plan_JonesWorland.h:144-219).
+ * Each parity: a table from its entries, then the general kernel on its rows (two elements
+ * apart); row 0 of the system (the even subsystem's first row) takes the row-0 scaleC. */
+static PfSolveResult append_alt(PfSolveApplication* app, const jw_source_desc* full, const jw_run_desc* frun,
+                                hipStream_t s, void* table)
+{
+    struct pfsolve_plan* p = app->localFFTPlan;
+    PfSolveConfiguration* cfg = &app->configuration;
+    PfSolveResult r = jw_check_offsets(full, frun);
+    if (r != PFSOLVE_SUCCESS) return r;
+    const int64_t N = p->N;
+    const int64_t NB = jw_alt_stride(N, cfg->Msplit[0]);
+    {
+        /* matrix extent: every diagonal is a block of NB entries */
+        int64_t mmax = -1;
+        const int64_t bases[5] = {full->mdA, full->udA, full->ldA, full->ldS, full->udS};
+        const int used[5] = {full->band_kind != JW_BAND_NONE,
+                             full->band_kind == JW_BAND_UPPER, full->band_kind == JW_BAND_LOWER,
+                             full->solve_kind == JW_SOLVE_LOWER || full->solve_kind == JW_SOLVE_TRI,
+                             full->solve_kind == JW_SOLVE_UPPER || full->solve_kind == JW_SOLVE_TRI};
+        for (int k = 0; k < 5; k++)
+            if (used[k] && bases[k] + NB - 1 > mmax) mmax = bases[k] + NB - 1;
+        if (cfg->bufferSize && mmax >= 0 && (uint64_t)((mmax + 1) * (p->precision == JW_PREC_DD ? 16 : 8)) >
+                                                 cfg->bufferSize[0])
+            return PFSOLVE_ERROR_EMPTY_bufferSize;
+        int64_t dummy, rmax;
+        jw_index_extent(full, frun, &dummy, &rmax);
+        const int64_t elt = p->precision == JW_PREC_DD ? 16 : 8;
+        if (cfg->outputBufferSize && rmax >= 0 && (uint64_t)((rmax + 1) * elt) > cfg->outputBufferSize[0])
+            return PFSOLVE_ERROR_EMPTY_outputBufferSize;
+    }
+    if (frun->nsys <= 0) return PFSOLVE_SUCCESS;
+    for (int par = 0; par < 2; par++) {
+        const int64_t n = (N + 1 - par) / 2;
+        if (n <= 0) continue;
+        jw_source_desc src = *full;
+        src.N = n;
+        src.alt = 1;
+        src.alt_parity = par;
+        src.alt_N = N;
+        jw_run_desc run = *frun;
+        run.rhs = cfg->outputBuffer[0];
+        run.N = (int32_t)n;
+        run.row_step = 2;
+        run.vec2 = 0;
+        run.offSol = frun->offSol + par;
+        /* rows r = 2k + par inside [lo, hi) <=> k in [ceil((lo - par) / 2), ceil((hi - par) / 2)) */
+        run.in_lo = frun->in_lo > par ? (frun->in_lo - par + 1) / 2 : 0;
+        run.in_hi = frun->in_hi > par ? (frun->in_hi - par + 1) / 2 : 0;
+        run.out_lo = frun->out_lo > par ? (frun->out_lo - par + 1) / 2 : 0;
+        run.out_hi = frun->out_hi > par ? (frun->out_hi - par + 1) / 2 : 0;
+        if (run.in_hi < run.in_lo) run.in_hi = run.in_lo;
+        if (run.out_hi < run.out_lo) run.out_hi = run.out_lo;
+        if (par == 1) run.apply_scale = 0;
+        void* tab = (char*)table + (size_t)par * p->alt_tab;
+        run.table = tab;
+        if (jwdev_prologue(&src, cfg->buffer[0], tab, p->L, p->R, p->precision, 1, s) != 0)
+            return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+        const int G = 64 / p->L;
+        const int64_t npass = (run.nsys + 2 * G - 1) / (2 * G);
+        int64_t need = (npass + p->waves - 1) / p->waves;
+        int64_t grid = (int64_t)p->cu_count * p->occupancy;
+        if (grid > need) grid = need;
+        if (grid < 1) grid = 1;
+        if (jwdev_main(&run, p->L, p->R, p->precision, p->waves, (int)grid, s) != 0)
+            return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    }
+    return PFSOLVE_SUCCESS;
+}
+
 static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchParams* lp, uint64_t slot)
 {
     struct pfsolve_plan* p = app->localFFTPlan;
@@ -396,6 +483,7 @@ static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchPar
         PfSolveResult r = jw_resolve_launch(cfg, lp, nsys, &src, &run);
         if (r != PFSOLVE_SUCCESS) return r;
         if (p->chain_K > 1) return append_chain(app, lp, &run, s, table);
+        if (p->alt) return append_alt(app, &src, &run, s, table);
         /* offsets and strides: no matrix diagonal or system may start before its buffer */
         r = jw_check_offsets(&src, &run);
         if (r != PFSOLVE_SUCCESS) return r;

@@ -199,3 +199,17 @@ def jw_chain(p: JwProblem, K: int, bufs, rhs: np.ndarray, f128: bool = True) -> 
     rc = _chain_fn("pfo_jw_chain_f128" if f128 else "pfo_jw_chain_f64")(C.byref(p), K, arr, P(out))
     assert rc == 0, rc
     return out
+
+
+def alt_pos(i, N, Nalign=0):
+    L = lib()
+    L.pfo_reorder_i_alt.argtypes = [i64, i64, i64, i64]
+    L.pfo_reorder_i_alt.restype = i64
+    return L.pfo_reorder_i_alt(i, N, Nalign or N, 0)
+
+
+def alt_stride(N):
+    L = lib()
+    L.pfo_alt_stride.argtypes = [i64, i64]
+    L.pfo_alt_stride.restype = i64
+    return L.pfo_alt_stride(N, 0)

@@ -106,7 +106,8 @@ def test_validation_error_codes():
     assert validate(cfg(N=0, jw_type=13, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_EMPTY_size
     assert validate(cfg(jw_type=0))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX          # no plan kind
     assert validate(cfg(jw_type=13, block=104))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
-    assert validate(cfg(jw_type=113, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX  # ALT
+    assert validate(cfg(jw_type=113, upperBanded=2))[0] == 0   # ALT
+    assert validate(cfg(jw_type=213, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
     assert validate(cfg(jw_type=10, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
     assert validate(cfg(jw_type=13, upperBanded=3))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
     assert validate(cfg(jw_type=13, upperBanded=2, doublePrecision=0))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
@@ -275,3 +276,27 @@ def test_chained_iterations_validation():
     assert validate(cfg(**dict(ok, N=1022, upperBanded=0)))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
     # shrinking systems keep at least one row
     assert validate(cfg(**dict(ok, N=3, upperBanded=1)))[0] == _abi.PFSOLVE_ERROR_EMPTY_size
+
+
+def test_alt_validation():
+    """ALT (jw_type hundreds digit 1): jw 10-13 with baked matrix positions, one iteration,
+    parallel (permuted) layout; each parity subsystem within the general kernel's sizes"""
+    assert validate(cfg(N=2048, jw_type=110, upperBanded=1))[0] == 0
+    assert validate(cfg(N=2049, jw_type=110, upperBanded=1))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(N=1024, jw_type=113, upperBanded=2, doublePrecision=0,
+                        quadDoubleDoublePrecision=1))[0] == 0
+    assert validate(cfg(N=100, jw_type=114))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(N=100, jw_type=1110, upperBanded=1))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(N=100, jw_type=110, upperBanded=1,
+                        jw_control_bitmask=_abi.RUNTIME_OFFSETM))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(N=100, jw_type=110, upperBanded=1, numConsecutiveJWIterations=2))[0] == \
+        _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+    assert validate(cfg(N=100, jw_type=210, upperBanded=1))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
+
+
+def test_alt_stride_library_matches_oracle():
+    lib = pf.lib()
+    lib.jw_alt_stride.argtypes = [C.c_int64, C.c_uint64]
+    lib.jw_alt_stride.restype = C.c_int64
+    for N in [1, 2, 63, 64, 65, 511, 512, 513, 1023, 1024, 1025, 2048, 3840, 5000]:
+        assert lib.jw_alt_stride(N, 0) == O.alt_stride(N), N

@@ -591,3 +591,68 @@ def test_chain_against_dense_composition(ub, jw, N, K, rt):
     if K == 1:
         single = O.jw_f128(p, bufs[0], x)
         np.testing.assert_array_equal(got, single)
+
+
+# ---------------------------------------------------------------- ALT variant
+def test_alt_layout_matches_reference_build():
+    """reorder_i_alt and get_ALT_stride of the reference headers (compiled under oracle/_ref)"""
+    if not os.path.exists(O.REF_SO):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    ref = C.CDLL(O.REF_SO)
+    ref.ref_reorder_i_alt.argtypes = [C.c_int] * 4
+    ref.ref_get_ALT_stride.argtypes = [C.c_int] * 2
+    for N in list(range(1, 140)) + [255, 256, 511, 512, 513, 777, 1023, 1024, 1025, 2048, 3000]:
+        assert O.alt_stride(N) == ref.ref_get_ALT_stride(N, 0), N
+        pos = [O.alt_pos(i, N) for i in range(N)]
+        assert pos == [ref.ref_reorder_i_alt(i, N, N, 0) for i in range(N)], N
+        assert len(set(pos)) == N and max(pos) < O.alt_stride(N)
+
+
+@pytest.mark.parametrize("jw,ub", [(0, 1), (0, 0), (1, 1), (2, 0), (3, 2), (3, 1), (3, 0)])
+@pytest.mark.parametrize("N", [1, 2, 3, 8, 63, 64, 130, 257])
+def test_alt_modes_against_dense(jw, ub, N):
+    """ALT (jw_type 110-113): rows couple with rows +-2; baked positions are multiples of the
+    ALT block stride, entries in reorder_i_alt order; row-0 scaleC on row 0 only.  Checked
+    against a dense LAPACK solve of the stride-2 operator built from the same values."""
+    S = O.alt_stride(N)
+    P = np.array([O.alt_pos(i, N) for i in range(N)], dtype=np.int64)
+    rng = np.random.default_rng(N + 10 * jw + ub)
+    buf = rng.uniform(-3, 3, 3 * S + 4)                   # junk outside the matrix
+    A = np.eye(N)
+    Sm = np.eye(N)
+    if jw in (0, 1):
+        d, e = rng.uniform(0.5, 1.5, N), rng.uniform(-0.5, 0.5, N)
+        if ub:   # [D @0 | U @S]
+            buf[P], buf[S + P] = d, e
+            A = np.diag(d) + np.diag(e[:-2], 2) if N > 2 else np.diag(d)
+        else:    # [L @0 | D @S]
+            buf[S + P], buf[P] = d, e
+            A = np.diag(d) + (np.diag(e[2:], -2) if N > 2 else 0)
+    if jw in (0, 2):
+        s = rng.uniform(-0.45, 0.45, N)
+        buf[2 * S + P] = s
+        if N > 2:
+            Sm = np.eye(N) + (np.diag(s[2:], -2) if ub else np.diag(s[:-2], 2))
+    if jw == 3:
+        d = rng.uniform(0.5, 1.5, N)
+        lo, up = rng.uniform(-0.3, 0.3, N), rng.uniform(-0.3, 0.3, N)
+        A = np.diag(d)
+        if ub == 2:   # [L @0 | D @S | U @2S]
+            buf[P], buf[S + P], buf[2 * S + P] = lo, d, up
+            if N > 2:
+                Sm = np.eye(N) + np.diag(lo[2:], -2) + np.diag(up[:-2], 2)
+        elif ub == 1:   # [D @S | L @0]
+            buf[S + P], buf[P] = d, lo
+            if N > 2:
+                Sm = np.eye(N) + np.diag(lo[2:], -2)
+        else:           # [U @S | D @0]
+            buf[S + P], buf[P] = up, d
+            if N > 2:
+                Sm = np.eye(N) + np.diag(up[:-2], 2)
+    x = O.uniform(N + 3, -1, 1, N)
+    p = O.problem(jw_type=110 + jw, upperBanded=ub, N=N, B=1, cfg_outStride=N, cfg_scaleC=0.5)
+    want = np.linalg.solve(Sm, A @ x)
+    if jw != 1:
+        want[0] *= 0.5
+    assert O.relerr_norm(O.jw_f128(p, buf, x), want) < 1e-14
+    assert O.relerr_norm(O.jw_f64(p, buf, x), want) < 1e-14
