@@ -66,8 +66,9 @@ const char* pfsolve_last_error(void);
  * hipMalloc).  No counterpart in the reference, whose plans always allocate.  The
  * coefficient table and the device error flag live in a static device array of the
  * library (pfsolve_noalloc_scratch_bytes(), 144 KiB + 256 B per GPU, held until the
- * process ends); configurations whose table would not fit (none of the supported ones
- * today) return PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER.  Nothing is allocated on the
+ * process ends); configurations whose table would not fit (long systems, FP64 N > 2048
+ * or double-double N > 512, whose per-chunk tables and reduced data are larger) return
+ * PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER.  Nothing is allocated on the
  * device or the host (no hipMalloc, hipFree, hipEventCreate, malloc).  Runs on
  * configuration->stream[0] (or the null stream); num_streams > 1 is refused
  * (PFSOLVE_ERROR_INVALID_QUEUE).  All calls on one GPU share its scratch: calls whose

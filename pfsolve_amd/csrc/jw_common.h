@@ -157,26 +157,39 @@ template <> __device__ __forceinline__ void mem_store<double>(void* base, int64_
 
 constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v / 2); }
 
-// reorder_i0 (storage position of logical row i), single-upload case N <= 3840:
-// PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:66-142
-__device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N, int64_t Nalign = 0)
+// reorder_i0 (storage position of logical row i,
+// PfSolve/pfSolve_PlanManagement/pfSolve_Plans/pfSolve_Plan_JonesWorland.h:66-142, HIP branch).
+// Up to 3840 rows the whole diagonal is one block; longer diagonals are blocks of Msplit0
+// rows (the last one shorter), each permuted on its own.  Inside a block of n rows laid out
+// as R rows per lane (R odd, from the block size or Nalign), row k is held in register
+// k % R of lane k / R, and registers are stored one after the other, register j holding
+// ceil(n/R) rows for j < n % R and floor(n/R) after.
+__device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N, int64_t Nalign = 0, int64_t Msplit0 = 704)
 {
     // rows per thread of the register layout: from Nalign (the largest system of a
-    // chained step), the split of the N stored rows over them: from N
-    const int64_t NA = Nalign > 0 ? Nalign : N;
+    // chained step) or the block size, the split of the stored rows over them: from N
+    int64_t blk = N;
+    int64_t NA = Nalign > 0 ? Nalign : N;
+    if (N > 3840) {
+        blk = Msplit0 > 0 ? Msplit0 : 704;
+        NA = blk;
+    }
     int64_t ws = ((NA + 1023) / 1024) * 64;
     if (NA < 512) ws = 64;
     int64_t R = (NA + ws - 1) / ws;
     if ((R & 1) == 0) R++;
-    int64_t ceilq = (N + R - 1) / R;
-    int64_t k = i % R;
-    int64_t rem = N % R;
-    int64_t ret = i / R;
+    const int64_t base = (i / blk) * blk;
+    const int64_t n = base + blk > N ? N - base : blk;
+    const int64_t li = i - base;
+    const int64_t ceilq = (n + R - 1) / R;
+    const int64_t k = li % R;
+    const int64_t rem = n % R;
+    int64_t ret = li / R;
     if (k < rem)
         ret += k * ceilq;
     else
-        ret += rem * ceilq + (k - rem) * (N / R);
-    return ret;
+        ret += rem * ceilq + (k - rem) * (n / R);
+    return base + ret;
 }
 
 

@@ -497,6 +497,7 @@ int jwdev_noalloc_scratch(void** table, int** err, size_t* table_bytes)
 }
 
 const char* jwdev_last_error(void) { return g_err; }
+void jwdev_set_error(const char* msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
 
 int jwdev_supported(int L, int R, int precision) { return pick_main(precision, L, R) != nullptr; }
 

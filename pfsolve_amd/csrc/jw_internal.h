@@ -111,6 +111,31 @@ typedef struct jw_run_desc {
     int32_t row_step;
 } jw_run_desc;
 
+/* Long systems (FP64 N > 2048, double-double N > 512): every system is cut into chunks of
+ * C = 64 R rows (one wavefront per chunk and pair of systems).  Chunk c's format-1 table
+ * (64 lanes, R rows per lane) sits at table + c * tab_elems elements; its pivots continue
+ * those of the chunk before.  A chunk's result is linear in its right-hand sides and in
+ * three carries: w of the row before it, z and y of the row after it.  The reduced data
+ * `red` (elements of the arithmetic type) holds, per chunk c and system s of the slice,
+ * array a at ((a * nchunks + c) * red_sys + s):
+ *   a = 0..2  F, Z, Y: the chunk's w at its last row and z, y at its first row, all
+ *             carries zero (REDUCE)
+ *   a = 3..6  the right-hand side of rows rb, rb+1, rb+C-2, rb+C-1 as read (REDUCE),
+ *             for the neighbours' banded matvec (the step runs in place)
+ *   a = 7..9  the carries w_in, z_in, y_in (the reduced solve)
+ * then, at 10 * nchunks * red_sys + 9 c + 3 j + o, the response of output o (w last,
+ * z first, y first) to a unit carry j (w_in, z_in, y_in) with zero right-hand side (COEF). */
+enum { JW_LONG_COEF = 0, JW_LONG_REDUCE = 1, JW_LONG_FINISH = 2 };
+#define JW_LONG_ARRAYS 10
+typedef struct jw_long_desc {
+    jw_run_desc run;          /* rhs (slice), nsys (slice), stride, offSol, keep ranges, N, recurrences, scale */
+    const void* table;
+    int64_t tab_elems;        /* elements of one chunk's table */
+    void* red;
+    int64_t red_sys;          /* systems per reduced array row (the slice capacity) */
+    int32_t nchunks, mode;
+} jw_long_desc;
+
 /* One matrix set for the batched prologue (one workgroup each). */
 typedef struct jw_prologue_job {
     jw_source_desc src;
@@ -194,10 +219,22 @@ int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int
 int jwdev_prologue_chain(const jw_chain_jobs* jobs, void* stream);
 int jwdev_chain64_occupancy(int R);
 int jwdev_chain64(const jw_run_desc* r, const void* tables, int64_t tab_elems, int K, int R, int grid, void* stream);
+/* long systems: rows per lane of a chunk (0: precision not built), the table of every
+ * chunk (one workgroup in order when the solve has pivots, else one per build of
+ * `nsub` chunks), the chunk kernel in one of the JW_LONG_* modes, the reduced solve */
+int jwdev_long_rows(int precision);
+int jwdev_long_nsub(int precision);
+int64_t jwdev_long_lds_bytes(int precision);
+int jwdev_long_occupancy(int precision);
+int jwdev_long_prologue(const jw_source_desc* src, const void* matrices, void* table, int64_t tab_elems,
+                        int precision, void* stream);
+int jwdev_long_chunks(const jw_long_desc* d, int precision, int grid_x, void* stream);
+int jwdev_long_reduced(const jw_long_desc* d, int precision, void* stream);
 int jwdev_block(const jw_block_desc* d, void* stream);
 int jwdev_gbmv(const jw_gbmv_desc* d, void* stream);
 int jwdev_device_info(int* cu_count, int* lds_per_block, int* arch_major, int* arch_minor);
 const char* jwdev_last_error(void);
+void jwdev_set_error(const char* msg);
 /* this GPU's scratch of the allocation-free entry point: table, error flag, table capacity */
 int jwdev_noalloc_scratch(void** table, int** err, size_t* table_bytes);
 

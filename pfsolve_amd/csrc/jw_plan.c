@@ -11,6 +11,12 @@
 #define JW_MAX_N_DD 512
 #define JW_MAX_N_FUSED_BLOCKS 512   /* fused block I/O runs on the compute / store split kernel */
 #define JW_MAX_N_CHAIN 1024         /* chained steps: R <= 16 rows per lane on one wavefront */
+#define JW_MAX_N_LONG (1 << 22)     /* long systems: chunks of 512 (FP64) or 256 (DD) rows */
+
+int jw_is_chunked(int64_t N, int precision)
+{
+    return precision == JW_PREC_F64 ? N > JW_MAX_N_F64 : N > JW_MAX_N_DD;
+}
 
 int jw_plan_kind(const PfSolveConfiguration* cfg)
 {
@@ -90,8 +96,10 @@ PfSolveResult jw_configure(const PfSolveConfiguration* in, PfSolveConfiguration*
             if (out->KL == 0 && out->KU == 0) { out->KL = 2; out->KU = 2; }
             if (out->KL < 0 || out->KU < 0 || out->KL > 2 || out->KU > 2) return PFSOLVE_ERROR_UNSUPPORTED_RADIX;
         }
-        if (!alt && prec == JW_PREC_F64 && N > JW_MAX_N_F64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
-        if (!alt && prec != JW_PREC_F64 && N > JW_MAX_N_DD) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        /* beyond one wavefront's rows the step is cut into chunks (the reference's
+         * multi-upload path for M_size > 3840, pfSolve_InitializeApp.h:1540-1580) */
+        if (!alt && N > JW_MAX_N_LONG) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        if (!alt && jw_is_chunked(N, prec) && out->kernelVariant == 2) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         /* block operators fused around the step: layout 4 on the input (complex in, no
          * accumulate), and on the output layout 7 (complex out, optional accumulate)
          * or none (the step's real systems) */

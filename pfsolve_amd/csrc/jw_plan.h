@@ -39,6 +39,13 @@ struct pfsolve_plan {
     /* ALT variant: two parity subsystems, one format-1 table of alt_tab bytes each */
     int alt;
     size_t alt_tab;
+    /* long systems cut into chunks of 64 long_R rows (long_R = 0: not available; long_only:
+     * the only path of this plan).  A slot's area holds, from long_tab_off, long_ntab
+     * format-1 tables of long_tab_elems elements, and from long_red_off the reduced data
+     * for slices of long_red_sys systems; long_occ chunk workgroups fit one CU */
+    int long_R, long_only, long_occ;
+    int64_t long_nc, long_ntab, long_tab_elems, long_red_sys;
+    size_t long_tab_off, long_red_off;
 };
 
 /* Applies the reference's configuration defaults (pfSolve_InitializeApp.h:981-1212)
@@ -47,6 +54,9 @@ PfSolveResult jw_configure(const PfSolveConfiguration* in, PfSolveConfiguration*
 /* kind of plan a configuration asks for */
 int jw_plan_kind(const PfSolveConfiguration* cfg);
 int jw_precision(const PfSolveConfiguration* cfg);
+/* 1 when a plain JW step of N rows is cut into chunks (FP64 N > 2048, double-double
+ * N > 512): longer than the one-wavefront kernels hold */
+int jw_is_chunked(int64_t N, int precision);
 /* geometry of the JW kernels for N rows */
 PfSolveResult jw_select_geometry(int64_t N, int precision, int* L, int* R);
 

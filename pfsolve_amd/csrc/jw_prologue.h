@@ -110,12 +110,32 @@ __device__ __forceinline__ void row_band(const jw_source_desc& d, const void* ma
     }
 }
 
+// storage position of row r of the system the descriptor reads
+__device__ __forceinline__ int64_t source_pos(const jw_source_desc& d, int64_t r, int64_t N)
+{
+    return d.alt ? alt_pos(2 * r + d.alt_parity, d.alt_N)
+                 : (d.permuted ? reorder_pos(r, N, d.Nalign, d.Msplit0) : r);
+}
+
+// One table build.  Local row r of the build is row rb + r of the system.  A build of a
+// long system's later rows (rb > 0, format 1) takes from `carry` the pivot of the row
+// before its first (carry[0] = g_{rb-1}) and that row's raw super-diagonal entry
+// (carry[1] = ud_{rb-1}), so that its pivots continue the exact recurrence; it writes
+// nsub format-1 tables of R * L rows each, one after the other (table_elems apart).
 template <typename T>
 __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const void* mat, T* table, int L, int R, int format,
-                              unsigned char* smem_raw)
+                              unsigned char* smem_raw, int64_t rb = 0, const T* carry = nullptr, int nsub = 1)
 {
     using A = Ar<T>;
-    const int RL = R * L;
+    const int RLs = R * L;                 // rows of one table
+    const int RL = RLs * nsub;             // rows of this build
+    const int64_t tab_elems = (int64_t)JW_NCOEF * RLs + 4 * (int64_t)ilog2_rt(L) * L;
+    // format-1 element (p, c) of local row r: table rr / RLs, row-in-lane and lane of rr % RLs
+    auto tab1 = [&](int r, int p, int c) -> int64_t {
+        const int k = r / RLs, rr = r - k * RLs;
+        return k * tab_elems + tab_coef(p, rr % R, rr / R, c, R, L);
+    };
+    const bool has_carry = rb > 0 && carry != nullptr;
     T* sl = reinterpret_cast<T*>(smem_raw);    // raw ld, then l, then (format 2) l'
     T* sh = sl + RL;                           // raw ud, then h, then (format 2) h'
     T* sD = sh + RL;
@@ -135,18 +155,18 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
 #pragma unroll
     for (int j = 0; j < kProRows; j++) {
         const int r = tid + j * nthr;
+        const int64_t gr = rb + r;
         T ldv = A::zero(), udv = A::zero(), Dv = A::zero(), Ev = A::zero();
 #pragma unroll
         for (int k = 0; k < 5; k++) vb[j][k] = A::zero();
-        if (r < N) {
-            const int64_t P = d.alt ? alt_pos(2 * (int64_t)r + d.alt_parity, d.alt_N)
-                                    : (d.permuted ? reorder_pos(r, N, d.Nalign) : (int64_t)r);
-            row_band<T>(d, mat, r, N, P, vb[j]);
-            if (has_f && r >= 1) ldv = mem_load<T>(mat, d.ldS + P, mf);
-            if (has_b1 && r + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);
+        if (r < RL && gr < N) {
+            const int64_t P = source_pos(d, gr, N);
+            row_band<T>(d, mat, gr, N, P, vb[j]);
+            if (has_f && gr >= 1) ldv = mem_load<T>(mat, d.ldS + P, mf);
+            if (has_b1 && gr + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);
             if (d.has_m) {
                 Dv = mem_load<T>(mat, d.mdM + P, mf);
-                if (r + 1 < N) Ev = mem_load<T>(mat, d.udM + P, mf);
+                if (gr + 1 < N) Ev = mem_load<T>(mat, d.udM + P, mf);
             } else {
                 Dv = A::one();
             }
@@ -157,12 +177,11 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
             sD[r] = Dv;
             sE[r] = Ev;
             if (format == 1) {
-                const int t = r / R, i = r % R;
-                table[tab_coef(0, i, t, 0, R, L)] = vb[j][0];
-                table[tab_coef(0, i, t, 1, R, L)] = vb[j][1];
-                table[tab_coef(1, i, t, 0, R, L)] = vb[j][2];
-                table[tab_coef(1, i, t, 1, R, L)] = vb[j][3];
-                table[tab_coef(2, i, t, 0, R, L)] = vb[j][4];
+                table[tab1(r, 0, 0)] = vb[j][0];
+                table[tab1(r, 0, 1)] = vb[j][1];
+                table[tab1(r, 1, 0)] = vb[j][2];
+                table[tab1(r, 1, 1)] = vb[j][3];
+                table[tab1(r, 2, 0)] = vb[j][4];
             }
         }
     }
@@ -208,7 +227,9 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         for (int i = 0; i < RT; i++) {
             const int r = r0 + i;
             if (r >= RL) break;
-            T m = (r >= 1 && r < N) ? A::mul(sl[r], sh[r - 1]) : A::zero();
+            const int64_t gr = rb + r;
+            T m = A::zero();
+            if (gr >= 1 && gr < N) m = A::mul(sl[r], r >= 1 ? sh[r - 1] : (has_carry ? carry[1] : A::zero()));
             T na = A::msub(a, m, c), nb = A::msub(b, m, e);
             c = a;
             e = b;
@@ -218,7 +239,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         }
         // ud of the row before this thread's first, read before any thread
         // overwrites its ud with h (after the barrier below)
-        T udprev = (r0 >= 1 && r0 < RL) ? sh[r0 - 1] : A::zero();
+        T udprev = (r0 >= 1 && r0 < RL) ? sh[r0 - 1] : ((r0 == 0 && has_carry) ? carry[1] : A::zero());
         // the wavefronts past the first NP threads own no rows: they skip the scan
         // (wave-uniform) and only keep the barriers
         const bool part = wv < NP / 64;
@@ -258,8 +279,15 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 wa = ta; wb = tb; wc = tc; we = te;
             }
             compose(xa, xb, xc, xe, wa, wb, wc, we);
-            // g_{r-1} = 1 / p_{r-1} of the row before this thread's first
-            if (r0 >= 1) gprev = A::div(A::add(xc, xe), A::add(xa, xb));
+            // g_{r-1} = 1 / p_{r-1} of the row before this thread's first: the prefix map
+            // applied to (1, 1) (the first row has no sub-diagonal, so any start works), or
+            // after a carry to (1, g_in), i.e. p = 1 / g_in
+            if (has_carry) {
+                const T gin = carry[0];
+                gprev = A::div(A::madd(xc, xe, gin), A::madd(xa, xb, gin));
+            } else if (r0 >= 1) {
+                gprev = A::div(A::add(xc, xe), A::add(xa, xb));
+            }
         }
         __syncthreads();   // the totals are read: g's array is free
         // first pass: the exact recurrence over this thread's rows from the scanned pivot;
@@ -277,9 +305,9 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 ldv[i] = sl[r];
                 udv[i] = sh[r];
                 T g = A::zero();
-                if (r < N) {
+                if (rb + r < N) {
                     // p_r = 1 - ld_r ud_{r-1} / p_{r-1},  g_r = 1/p_r
-                    const T l = (r >= 1) ? A::mul(ldv[i], gprev) : A::zero();
+                    const T l = (rb + r >= 1) ? A::mul(ldv[i], gprev) : A::zero();
                     pr[i] = A::sub(A::one(), A::mul(l, udprev));
                     g = A::div(A::one(), pr[i]);
                     gprev = g;
@@ -306,7 +334,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         // row.  The refinement runs only when some first-row residual exceeds a few ulps
         // (the flag is raised by any thread, read by all: the choice is workgroup-uniform
         // and a function of the matrix alone, so every kernel still builds the same table).
-        if (part && r0 >= 1 && r0 < N) {
+        if (part && r0 >= 1 && rb + r0 < N) {
             const double tol = sizeof(T) == 8 ? 0x1p-51 : 0x1p-103;
             const T rho0 = A::add(A::sub(pr[0], A::one()), A::mul(A::mul(ldv[0], udp0), sg[r0 - 1]));
             if (!(fabs(A::hi(rho0)) <= tol * fabs(A::hi(pr[0])))) sh[0] = A::one();
@@ -316,15 +344,15 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         T rho[kMaxRT], cc[kMaxRT];
         T sc = A::one(), ss = A::zero();      // this thread's map e -> sc e + ss
         if (refine && part) {
-            T gm = (r0 >= 1 && r0 < RL) ? sg[r0 - 1] : A::one();
+            T gm = (r0 >= 1 && r0 < RL) ? sg[r0 - 1] : ((r0 == 0 && has_carry) ? carry[0] : A::one());
             udprev = udp0;
 #pragma unroll
             for (int i = 0; i < kMaxRT; i++) {
                 rho[i] = A::zero();
                 cc[i] = A::zero();
                 const int r = r0 + i;
-                if (i >= RT || r >= N) continue;
-                const T m = (r >= 1) ? A::mul(ldv[i], udprev) : A::zero();
+                if (i >= RT || rb + r >= N) continue;
+                const T m = (rb + r >= 1) ? A::mul(ldv[i], udprev) : A::zero();
                 const T mg = A::mul(m, gm);
                 rho[i] = A::add(A::sub(pr[i], A::one()), mg);
                 cc[i] = A::mul(mg, gm);
@@ -363,7 +391,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
 #pragma unroll
             for (int i = 0; i < kMaxRT; i++) {
                 const int r = r0 + i;
-                if (i >= RT || r >= N) continue;
+                if (i >= RT || rb + r >= N) continue;
                 ein = A::madd(rho[i], cc[i], ein);
                 // a correction that is not finite (no dominance, products out of range)
                 // is not applied
@@ -382,8 +410,8 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 if (i >= RT || r >= RL) continue;
                 // l_r = ld_r g_{r-1},  h_r = ud_r g_r  (zero outside the matrix)
                 T l = A::zero(), h = A::zero();
-                if (r < N) {
-                    if (r >= 1) l = A::mul(ldv[i], sg[r - 1]);
+                if (rb + r < N) {
+                    if (rb + r >= 1) l = A::mul(ldv[i], r >= 1 ? sg[r - 1] : (has_carry ? carry[0] : A::zero()));
                     h = A::mul(udv[i], sg[r]);
                 }
                 sl[r] = l;
@@ -392,7 +420,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         }
     } else {
         for (int r = tid; r < RL; r += nthr) {
-            const bool in = r < N;
+            const bool in = rb + r < N;
             if (!(in && d.solve_kind == JW_SOLVE_LOWER)) sl[r] = A::zero();
             sg[r] = in ? A::one() : A::zero();
             if (!(in && d.solve_kind == JW_SOLVE_UPPER)) sh[r] = A::zero();
@@ -486,21 +514,24 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     // format 1: pairs (vm2,vm1) (v0,vp1) (vp2,l) (g,h) (D,E) + Hillis-Steele multipliers
     // (the band pairs were written in phase 1)
     for (int r = tid; r < RL; r += nthr) {
-        const int t = r / R, i = r % R;
-        table[tab_coef(2, i, t, 1, R, L)] = sl[r];
-        table[tab_coef(3, i, t, 0, R, L)] = sg[r];
-        table[tab_coef(3, i, t, 1, R, L)] = sh[r];
-        table[tab_coef(4, i, t, 0, R, L)] = sD[r];
-        table[tab_coef(4, i, t, 1, R, L)] = sE[r];
+        table[tab1(r, 2, 1)] = sl[r];
+        table[tab1(r, 3, 0)] = sg[r];
+        table[tab1(r, 3, 1)] = sh[r];
+        table[tab1(r, 4, 0)] = sD[r];
+        table[tab1(r, 4, 1)] = sE[r];
     }
-    if (tid < 64) {
-        const int t = tid;
+    // scan multipliers: wavefront k (wave-uniform) takes table k
+    if ((tid >> 6) < nsub) {
+        const int kt = tid >> 6;
+        const int t = tid & 63;
+        T* const tabk = table + kt * tab_elems;
+        const int rbase = kt * RLs;
         const bool active = t < L;
         const int log2L = ilog2_rt(L);
         T bf = A::one(), bb1 = A::one(), bb2 = A::one();
         if (active) {
             for (int i = 0; i < R; i++) {
-                const int r = t * R + i;
+                const int r = rbase + t * R + i;
                 bf = A::mul(bf, A::neg(sl[r]));
                 bb1 = A::mul(bb1, A::neg(sh[r]));
                 bb2 = A::mul(bb2, A::neg(sE[r]));
@@ -509,10 +540,10 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         for (int k = 0; k < log2L; k++) {
             const int s = 1 << k;
             if (active) {
-                table[tab_scan(k, t, 0, R, L)] = bf;
-                table[tab_scan(k, t, 1, R, L)] = bb1;
-                table[tab_scan(k, t, 2, R, L)] = bb2;
-                table[tab_scan(k, t, 3, R, L)] = A::zero();
+                tabk[tab_scan(k, t, 0, R, L)] = bf;
+                tabk[tab_scan(k, t, 1, R, L)] = bb1;
+                tabk[tab_scan(k, t, 2, R, L)] = bb2;
+                tabk[tab_scan(k, t, 3, R, L)] = A::zero();
             }
             const int su = (t >= s) ? t - s : t;
             const int sdn = (t + s < L) ? t + s : t;

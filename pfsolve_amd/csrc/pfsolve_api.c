@@ -43,6 +43,19 @@ static PfSolveResult fill_geometry(const PfSolveConfiguration* cfg, pfsolve_geom
     int L = 0, R = 0;
     /* the ALT variant solves two subsystems of ceil(N/2) and floor(N/2) rows */
     const int alt = (cfg->jw_type % 1000) / 100 == 1;
+    if (!alt && cfg->numConsecutiveJWIterations <= 1 && jw_is_chunked(cfg->M_size, g->precision)) {
+        /* long systems: chunks of 64 R rows on the chunk kernel, one table per chunk */
+        const int kp = g->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD;
+        const int64_t rows = 64 * (int64_t)jwdev_long_rows(kp);
+        const int64_t build = rows * jwdev_long_nsub(kp);
+        const int64_t ntab = (cfg->M_size + build - 1) / build * jwdev_long_nsub(kp);
+        g->lanes = 64;
+        g->rows_per_lane = jwdev_long_rows(kp);
+        g->waves = 8;
+        g->table_bytes = ntab * jwdev_table_elems(64, g->rows_per_lane) * (kp == JW_PREC_F64 ? 8 : 16);
+        g->lds_bytes = jwdev_long_lds_bytes(kp);
+        return PFSOLVE_SUCCESS;
+    }
     PfSolveResult r = jw_select_geometry(alt ? (cfg->M_size + 1) / 2 : cfg->M_size, g->precision, &L, &R);
     if (r != PFSOLVE_SUCCESS) return r;
     if (R > 16) {
@@ -71,9 +84,8 @@ PfSolveResult pfsolve_validate_configuration(const PfSolveConfiguration* cfg, pf
     if (r != PFSOLVE_SUCCESS) return r;
     if (jw_plan_kind(&c) == PF_KIND_JW) {
         const pfsolve_geometry* gg = g ? g : &tmp;
-        const int long_n = gg->lanes == 0;
-        if (long_n && c.kernelVariant == 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
-        if (!long_n &&
+        /* lanes == 0: 1024 < N <= 2048 in FP64, the wave-per-system kernel or chunks */
+        if (gg->lanes > 0 &&
             !jwdev_supported(gg->lanes, gg->rows_per_lane, gg->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD))
             return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
     }
@@ -97,6 +109,36 @@ void deletePfSolve(PfSolveApplication* app)
         free(c->stream_event);
         c->stream_event = NULL;
     }
+}
+
+/* The chunked path of long systems: chunk geometry, the tables of every chunk and the
+ * reduced data after whatever the slot's area already holds (p->table_bytes). */
+static PfSolveResult long_setup(PfSolveApplication* app, struct pfsolve_plan* p)
+{
+    const PfSolveConfiguration* cfg = &app->configuration;
+    const int kp = p->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD;
+    const size_t elt = kp == JW_PREC_F64 ? 8 : 16;
+    const int R = jwdev_long_rows(kp);
+    const int64_t rows = 64 * (int64_t)R;
+    const int64_t build = rows * jwdev_long_nsub(kp);
+    p->long_R = R;
+    p->long_nc = (p->N + rows - 1) / rows;
+    p->long_ntab = (p->N + build - 1) / build * jwdev_long_nsub(kp);
+    p->long_tab_elems = jwdev_table_elems(64, R);
+    p->long_occ = jwdev_long_occupancy(kp);
+    if (p->long_occ < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
+    /* systems per slice: every system of the plan, up to 64 MiB of reduced data */
+    int64_t nsys = (int64_t)(cfg->size[1] * cfg->size[2]);
+    int64_t cap = ((int64_t)64 << 20) / (JW_LONG_ARRAYS * p->long_nc * (int64_t)elt);
+    if (nsys < cap) cap = nsys;
+    if (cap < 2) cap = 2;
+    cap = (cap + 1) / 2 * 2;
+    p->long_red_sys = cap;
+    p->long_tab_off = (p->table_bytes + 255) / 256 * 256;
+    p->long_red_off = p->long_tab_off + ((size_t)p->long_ntab * p->long_tab_elems * elt + 255) / 256 * 256;
+    p->table_bytes = p->long_red_off + (size_t)(JW_LONG_ARRAYS * p->long_nc * cap + 9 * p->long_nc) * elt;
+    p->table_stride = (p->table_bytes + 255) / 256 * 256;
+    return PFSOLVE_SUCCESS;
 }
 
 /* Everything a plan needs that allocates nothing: kind, precision, device limits,
@@ -144,14 +186,21 @@ static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
         pfsolve_geometry g;
         res = fill_geometry(cfg, &g);
         if (res != PFSOLVE_SUCCESS) return res;
+        if ((cfg->jw_type % 1000) / 100 != 1 && jw_is_chunked(p->N, p->precision)) {
+            /* longer than one wavefront holds: chunks only */
+            p->long_only = 1;
+            p->waves = g.waves;
+            p->lds_bytes = g.lds_bytes;
+            p->table_bytes = 0;
+            return long_setup(app, p);
+        }
         p->L = g.lanes;
         p->R = g.rows_per_lane;
         p->waves = g.waves;
         p->lds_bytes = g.lds_bytes;
         p->table_bytes = (size_t)g.table_bytes;
         const int kp = p->precision == JW_PREC_F64 ? JW_PREC_F64 : JW_PREC_DD;
-        const int long_n = p->L == 0;   /* N > 1024: the wave-per-system long kernel only */
-        if (long_n && cfg->kernelVariant == 1) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        const int long_n = p->L == 0;   /* 1024 < N <= 2048: the wave-per-system kernel or chunks */
         if (!long_n && !jwdev_supported(p->L, p->R, kp)) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         if ((cfg->jw_type % 1000) / 100 == 1) {
             /* ALT: each parity subsystem on the general kernel (rows two elements apart), one
@@ -180,7 +229,7 @@ static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
                 if (b2 > p->table_bytes) p->table_bytes = b2;
             }
         }
-        if ((cfg->kernelVariant == 2 || long_n) && !p->R64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        if (cfg->kernelVariant == 2 && !p->R64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         /* compute / store split form of the wave-per-system kernel (R <= 8) */
         if (p->R64 && p->R64 <= 8) {
             int c = 0, st = 0;
@@ -193,6 +242,11 @@ static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
         if (!long_n) {
             p->occupancy = jwdev_main_occupancy(p->L, p->R, p->precision, p->waves);
             if (p->occupancy < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
+        } else if (cfg->kernelVariant != 2) {
+            /* systems the wave-per-system kernel does not take (unaligned, or the general
+             * kernel asked for) run in chunks */
+            res = long_setup(app, p);
+            if (res != PFSOLVE_SUCCESS) return res;
         }
     } else if (p->kind == PF_KIND_BLOCK) {
         snprintf(app->kernelName, sizeof(app->kernelName), "PfSolve_block_%llu_%llu_%d_%d_%lld",
@@ -456,6 +510,50 @@ static PfSolveResult append_alt(PfSolveApplication* app, const jw_source_desc* f
     return PFSOLVE_SUCCESS;
 }
 
+/* Long systems in chunks: the tables of every chunk, the chunks' responses to unit
+ * carries, then per slice of systems the reduction, the reduced solve and the finishing
+ * pass (jw_long.hip). */
+static PfSolveResult append_long(PfSolveApplication* app, const jw_source_desc* src, const jw_run_desc* run,
+                                 hipStream_t s, void* slot_area)
+{
+    struct pfsolve_plan* p = app->localFFTPlan;
+    PfSolveConfiguration* cfg = &app->configuration;
+    if (!p->long_R) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+    if (run->nsys <= 0) return PFSOLVE_SUCCESS;
+    jw_long_desc d;
+    memset(&d, 0, sizeof(d));
+    d.run = *run;
+    d.table = (char*)slot_area + p->long_tab_off;
+    d.tab_elems = p->long_tab_elems;
+    d.red = (char*)slot_area + p->long_red_off;
+    d.red_sys = p->long_red_sys;
+    d.nchunks = (int32_t)p->long_nc;
+    if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+    if (jwdev_long_prologue(src, cfg->buffer[0], (void*)d.table, d.tab_elems, p->precision, s) != 0)
+        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    d.mode = JW_LONG_COEF;
+    d.run.nsys = 3;
+    if (jwdev_long_chunks(&d, p->precision, 1, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    /* workgroups per chunk: enough for two rounds of the whole GPU, at most one per 8 pairs */
+    const int64_t target = 2 * (int64_t)p->cu_count * p->long_occ;
+    for (int64_t s0 = 0; s0 < run->nsys; s0 += p->long_red_sys) {
+        const int64_t n = run->nsys - s0 < p->long_red_sys ? run->nsys - s0 : p->long_red_sys;
+        d.run.nsys = n;
+        d.run.offSol = run->offSol + s0 * run->stride;
+        const int64_t need = ((n + 1) / 2 + 7) / 8;
+        int64_t gx = (target + p->long_nc - 1) / p->long_nc;
+        if (gx > need) gx = need;
+        if (gx < 1) gx = 1;
+        d.mode = JW_LONG_REDUCE;
+        if (jwdev_long_chunks(&d, p->precision, (int)gx, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+        if (jwdev_long_reduced(&d, p->precision, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+        d.mode = JW_LONG_FINISH;
+        if (jwdev_long_chunks(&d, p->precision, (int)gx, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    }
+    if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+    return PFSOLVE_SUCCESS;
+}
+
 static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchParams* lp, uint64_t slot)
 {
     struct pfsolve_plan* p = app->localFFTPlan;
@@ -501,9 +599,10 @@ static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchPar
                 return PFSOLVE_ERROR_EMPTY_outputBufferSize;
         }
         run.rhs = cfg->outputBuffer[0];
+        if (p->long_only) return append_long(app, &src, &run, s, table);
         const uintptr_t base = (uintptr_t)run.rhs;
         run.vec2 = p->precision == JW_PREC_F64 && (base % 16) == 0 && (run.stride % 2) == 0 && (run.offSol % 2) == 0;
-        if (p->R64 && run.vec2) {
+        if (p->R64 && run.vec2 && cfg->kernelVariant != 1) {
             /* wave-per-system kernel: table format 2, built by a prologue launch or,
              * in the compute / store split kernel, by every workgroup in LDS */
             if (nsys <= 0) return PFSOLVE_SUCCESS;
@@ -527,9 +626,9 @@ static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchPar
             if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
             return PFSOLVE_SUCCESS;
         }
-        /* long systems run only on the wave-per-system kernel, which needs 16-byte
-         * aligned systems (even stride and offset, aligned buffer) */
-        if (p->L == 0) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        /* 1024 < N <= 2048 without 16-byte aligned systems (or with the general kernel
+         * asked for): in chunks */
+        if (p->L == 0) return append_long(app, &src, &run, s, table);
         if (jwdev_prologue(&src, cfg->buffer[0], table, p->L, p->R, p->precision, 1, s) != 0)
             return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
         if (nsys <= 0) return PFSOLVE_SUCCESS;

@@ -81,9 +81,11 @@ def test_version_and_error_strings_match_reference():
 
 def test_reorder_i0_library_matches_oracle():
     lib = pf.lib()
-    for N in [1, 63, 64, 65, 192, 193, 512, 777, 1024, 3840, 4000]:
+    for N in [1, 63, 64, 65, 192, 193, 512, 777, 1024, 3840, 4000, 10000]:
         for i in range(0, N, max(1, N // 97)):
             assert lib.pfsolve_reorder_i0(i, N, N, 0) == O.lib().pfo_reorder_i0(i, N, N, 0)
+    for i in range(0, 7777, 13):
+        assert lib.pfsolve_reorder_i0(i, 7777, 7777, 333) == O.lib().pfo_reorder_i0(i, 7777, 7777, 333)
 
 
 def cfg(**kw):
@@ -111,7 +113,8 @@ def test_validation_error_codes():
     assert validate(cfg(jw_type=10, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
     assert validate(cfg(jw_type=13, upperBanded=3))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
     assert validate(cfg(jw_type=13, upperBanded=2, doublePrecision=0))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
-    assert validate(cfg(N=5000, jw_type=13, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(N=5000, jw_type=13, upperBanded=2))[0] == 0          # long system, in chunks
+    assert validate(cfg(N=(1 << 22) + 1, jw_type=13, upperBanded=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
     assert validate(cfg(jw_type=14, KL=3, KU=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
     assert validate(cfg(LDA=4, KL=2, KU=2))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
     assert validate(cfg(block=108))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX
@@ -153,19 +156,41 @@ def test_geometry_covers_every_row(N):
 
 
 @pytest.mark.parametrize("N", [1025, 1500, 2048])
-def test_long_systems_validate_for_the_wave_per_system_kernel_only(N):
-    """1024 < N <= 2048 (FP64): accepted with no general-kernel geometry; the general
-    kernel alone (kernelVariant 1) and N > 2048 are rejected"""
+def test_long_systems_validate_for_the_wave_per_system_kernel(N):
+    """1024 < N <= 2048 (FP64): accepted with no general-kernel geometry (the wave-per-system
+    kernel, or chunks for unaligned systems and kernelVariant 1)"""
     rc, g = validate(cfg(N=N, jw_type=14))
     assert rc == 0 and g.lanes == 0 and g.table_bytes > 0
-    assert validate(cfg(N=N, jw_type=14, kernelVariant=1))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
-    assert validate(cfg(N=2049, jw_type=14))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(N=N, jw_type=14, kernelVariant=1))[0] == 0
+
+
+@pytest.mark.parametrize("N,dd", [(2049, 0), (3841, 0), (100000, 0), (1 << 22, 0), (513, 1), (5000, 1)])
+def test_chunked_geometry(N, dd):
+    """beyond one wavefront (FP64 N > 2048, double-double N > 512): chunks of 64 R rows,
+    one table per chunk of every 4-chunk build; the wave-per-system kernel cannot be forced;
+    the ALT variant, chained steps and fused blocks keep their own limits"""
+    prec = dict(doublePrecision=0, quadDoubleDoublePrecision=1) if dd else {}
+    for jt, ub in [(10, 0), (11, 1), (12, 0), (13, 1), (13, 2), (14, 0), (1013, 2)]:
+        rc, g = validate(cfg(N=N, jw_type=jt, upperBanded=ub, **prec))
+        assert rc == 0, (jt, ub)
+        R = 4 if dd else 8
+        assert g.lanes == 64 and g.rows_per_lane == R and g.waves == 8
+        nbuild = -(-N // (4 * 64 * R))
+        assert g.table_bytes == nbuild * 4 * (10 * 64 * R + 24 * 64) * (16 if dd else 8)
+        assert g.lds_bytes == (10 * 64 * R + 24 * 64) * (16 if dd else 8)
+    assert validate(cfg(N=N, jw_type=14, kernelVariant=2, **prec))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(N=N, jw_type=14, inputBlock=4, **prec))[0] == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
+    assert validate(cfg(N=N, jw_type=10, upperBanded=1, numConsecutiveJWIterations=2, **prec))[0] != 0
+    assert validate(cfg(N=2 * N + 2100, jw_type=113, upperBanded=2, **prec))[0] == \
+        _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
 
 
 def test_dd_geometry_and_limits():
     rc, g = validate(cfg(N=256, jw_type=14, doublePrecision=0, quadDoubleDoublePrecision=1))
     assert rc == 0 and g.precision == 2 and g.lanes * g.rows_per_lane >= 256
-    rc, _ = validate(cfg(N=1024, jw_type=14, doublePrecision=0, quadDoubleDoublePrecision=1))
+    rc, g = validate(cfg(N=1024, jw_type=14, doublePrecision=0, quadDoubleDoublePrecision=1))
+    assert rc == 0 and g.lanes == 64 and g.rows_per_lane == 4   # chunks
+    rc, _ = validate(cfg(N=(1 << 22) + 1, jw_type=14, doublePrecision=0, quadDoubleDoublePrecision=1))
     assert rc == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
 
 

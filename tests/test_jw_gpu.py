@@ -438,17 +438,6 @@ def test_fused_step_long_systems(N, permuted):
     check_systems(got, want, x, B, S, 0, N, 1, N - 2, TOL64)
 
 
-def test_long_systems_need_aligned_layout():
-    """the long kernel moves 16-byte chunks: an odd stride is refused, not run"""
-    N, B = 1500, 3
-    raw, buf, (offV, offT, offM) = fused_buffers(N, 3, 1)
-    x = O.uniform(4, -1, 1, B * (N + 1))
-    cfg = dict(size=[N, B, 1], jw_type=14, doublePrecision=1, offsetV=offV, offsetT=offT, offsetM=offM,
-               jw_control_bitmask=_abi.RUNTIME_OUTPUTBUFFERSTRIDE)
-    code = run_plan(cfg, buf, x, dict(outputBufferStride=N + 1), expect=1)
-    assert code == _abi.PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH
-
-
 @pytest.mark.parametrize("jw_type,ub", MODES)
 @pytest.mark.parametrize("N", [2, 64, 130, 512, 1024, 1800])
 def test_reference_modes_wave_per_system(jw_type, ub, N):

@@ -37,9 +37,13 @@ def test_reorder_i0_matches_reference_build():
         pytest.skip("oracle/_ref not built (needs the reference sources)")
     ref = C.CDLL(O.REF_SO)
     ref.ref_reorder_i0.argtypes = [C.c_int] * 4
-    for N in list(range(1, 200)) + [255, 256, 257, 511, 512, 513, 777, 1024, 2048, 3840, 3841, 5000]:
+    for N in list(range(1, 200)) + [255, 256, 257, 511, 512, 513, 777, 1024, 2048, 3840, 3841, 5000, 7040, 10000]:
         for i in range(N):
             assert lib().pfo_reorder_i0(i, N, N, 0) == ref.ref_reorder_i0(i, N, N, 0), (N, i)
+    # blocks of a caller-chosen Msplit0 beyond 3840 rows (odd block rows, a short last block)
+    for N, ms in [(4000, 512), (5000, 1000), (7777, 333)]:
+        for i in range(N):
+            assert lib().pfo_reorder_i0(i, N, N, ms) == ref.ref_reorder_i0(i, N, N, ms), (N, ms, i)
 
 
 def test_reorder_i0_is_permutation_and_identity_for_short_systems():
@@ -48,6 +52,20 @@ def test_reorder_i0_is_permutation_and_identity_for_short_systems():
         assert sorted(pos) == list(range(N))
         if N <= 64:
             assert pos == list(range(N))
+
+
+def test_reorder_i0_long_systems_permute_within_blocks():
+    """beyond 3840 rows every Msplit0-row block (704 by default) is a permutation of its
+    own rows, laid out as a 704-row system (11 rows per lane, 64 lanes): the first lane's
+    rows of a block sit at its first positions one register apart"""
+    for N, ms in [(3841, 0), (7040, 0), (10000, 0), (5000, 1000)]:
+        blk = ms or 704
+        pos = [lib().pfo_reorder_i0(i, N, N, ms) for i in range(N)]
+        assert sorted(pos) == list(range(N))
+        for b0 in range(0, N, blk):
+            assert sorted(pos[b0:b0 + blk]) == list(range(b0, min(N, b0 + blk)))
+    pos = [lib().pfo_reorder_i0(i, 7040, 7040, 0) for i in range(704)]
+    assert pos[0] == 0 and pos[1] == 64 and pos[11] == 1
 
 
 def test_reorder_i0_row_major_chunks():
