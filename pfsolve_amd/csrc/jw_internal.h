@@ -117,7 +117,7 @@ typedef struct jw_run_desc {
  * those of the chunk before.  A chunk's result is linear in its right-hand sides and in
  * three carries: w of the row before it, z and y of the row after it.  The reduced data
  * `red` (elements of the arithmetic type) holds, per chunk c and system s of the slice,
- * array a at ((a * nchunks + c) * red_sys + s):
+ * array a at ((a * red_sys + s) * nchunks + c) (a system's chunks contiguous):
  *   a = 0..2  F, Z, Y: the chunk's w at its last row and z, y at its first row, all
  *             carries zero (REDUCE)
  *   a = 3..6  the right-hand side of rows rb, rb+1, rb+C-2, rb+C-1 as read (REDUCE),
@@ -219,15 +219,17 @@ int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int
 int jwdev_prologue_chain(const jw_chain_jobs* jobs, void* stream);
 int jwdev_chain64_occupancy(int R);
 int jwdev_chain64(const jw_run_desc* r, const void* tables, int64_t tab_elems, int K, int R, int grid, void* stream);
-/* long systems: rows per lane of a chunk (0: precision not built), the table of every
- * chunk (one workgroup in order when the solve has pivots, else one per build of
- * `nsub` chunks), the chunk kernel in one of the JW_LONG_* modes, the reduced solve */
+/* long systems: rows per lane of a chunk, chunks per table build, the table of every
+ * chunk (one workgroup per build of `nsub` chunks; pivots handed from build to build),
+ * the chunk kernel in one of the JW_LONG_* modes, the reduced solve */
 int jwdev_long_rows(int precision);
 int jwdev_long_nsub(int precision);
 int64_t jwdev_long_lds_bytes(int precision);
 int jwdev_long_occupancy(int precision);
+/* sync: 1 + builds ints (ticket, per-build flags), gval: one value per build, err: the
+ * device error flag (used by tridiagonal solves only) */
 int jwdev_long_prologue(const jw_source_desc* src, const void* matrices, void* table, int64_t tab_elems,
-                        int precision, void* stream);
+                        int precision, int* sync, void* gval, int* err, void* stream);
 int jwdev_long_chunks(const jw_long_desc* d, int precision, int grid_x, void* stream);
 int jwdev_long_reduced(const jw_long_desc* d, int precision, void* stream);
 int jwdev_block(const jw_block_desc* d, void* stream);

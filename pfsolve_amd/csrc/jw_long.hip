@@ -38,15 +38,21 @@ extern "C" void jwdev_set_error(const char* msg);
 
 namespace {
 
-constexpr int kLongThreads = 512;   // 8 wavefronts per chunk workgroup
+constexpr int kLongThreads = 256;   // 4 wavefronts per chunk workgroup
 constexpr int kLongL = 64;
 constexpr int kLongLog2L = 6;
 
 // rows per lane of a chunk: FP64 8 (512-row chunks, 52 KiB table in LDS), double-double 4
-// (256-row chunks, 64 KiB)
+// (256-row chunks, 64 KiB); NSUB chunks per table build; BOUND: the thread count the chunk
+// kernel's register budget is declared for (768: 168 VGPRs, three 256-thread workgroups per
+// CU, as many as the FP64 table's LDS allows; 512: two, as for the DD table)
 template <typename T> struct LongGeom;
-template <> struct LongGeom<double> { static constexpr int R = 8; static constexpr int NSUB = 4; };
-template <> struct LongGeom<dd> { static constexpr int R = 4; static constexpr int NSUB = 4; };
+template <> struct LongGeom<double> {
+    static constexpr int R = 8, NSUB = 4, BOUND = 768;
+};
+template <> struct LongGeom<dd> {
+    static constexpr int R = 4, NSUB = 4, BOUND = 512;
+};
 
 template <typename T>
 constexpr int64_t long_tab_elems()
@@ -57,47 +63,86 @@ constexpr int64_t long_tab_elems()
 }  // namespace
 
 // ------------------------------------------------------------------------
-// tables: NSUB chunks per build of nsub * 64 * R rows
+// tables: NSUB chunks per build of NSUB * 64 * R rows, one workgroup per build.
+// Tridiagonal solves hand the pivot on: build b (taken in dispatch order from a ticket, so
+// every build it waits for has started) runs its loads, its pivot-map scan and everything
+// else that needs no pivot, then waits for build b-1's last pivot, finishes its pivots by
+// the exact recurrence and publishes its own last pivot before it writes its tables.  The
+// wait is bounded: when it runs out the device error flag is raised (PfSolveSync reports
+// it) and the build goes on with a unit pivot, so the grid always drains.
 // ------------------------------------------------------------------------
+constexpr int kLongMaxPolls = 1 << 22;
+
+template <typename T>
+struct ChainHooks {
+    T* carry;        // LDS: carry[0] = g entering the build
+    int* flags;      // per build: 1 once its last pivot is in gval
+    T* gval;         // per build: g of its last row
+    int* err;
+    int64_t build;
+    int chained;
+    __device__ void wait_carry(int tid) const
+    {
+        if (!chained || build == 0 || tid != 0) return;
+        int polls = 0;
+        bool ok = true;
+        while (__hip_atomic_load(&flags[build - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            if (++polls > kLongMaxPolls) {
+                ok = false;
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        carry[0] = ok ? gval[build - 1] : Ar<T>::one();
+    }
+    __device__ void publish(int tid, const T* sg, int RL) const
+    {
+        if (!chained || tid != 0) return;
+        gval[build] = sg[RL - 1];
+        __hip_atomic_store(&flags[build], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
 template <typename T>
 __global__ __launch_bounds__(kPrologueThreads) void jw_long_prologue_kernel(jw_source_desc d, const void* mat,
-                                                                          T* table, int sequential)
+                                                                          T* table, int chained, int* sync, T* gval,
+                                                                          int* err)
 {
     extern __shared__ __align__(16) unsigned char pro_smem[];
+    __shared__ int s_build;
     constexpr int R = LongGeom<T>::R;
     constexpr int NSUB = LongGeom<T>::NSUB;
     constexpr int SR = NSUB * R * kLongL;
     constexpr int64_t TE = long_tab_elems<T>();
-    // the pivot carry follows the build's five row arrays; g's array is the fifth
+    // the pivot carry follows the build's five row arrays
     T* carry = reinterpret_cast<T*>(pro_smem + prologue_lds_bytes(SR, sizeof(T)));
-    const T* sg = reinterpret_cast<const T*>(pro_smem) + 4 * (int64_t)SR;
-    const int64_t nit = (d.N + SR - 1) / SR;
-    const int64_t it0 = sequential ? 0 : (int64_t)blockIdx.x;
-    const int64_t it1 = sequential ? nit : it0 + 1;
-    for (int64_t it = it0; it < it1; it++) {
-        const int64_t rb = it * SR;
-        prologue_body<T>(d, mat, table + it * NSUB * TE, kLongL, R, 1, pro_smem, rb, sequential ? carry : nullptr,
-                         NSUB);
-        if (it + 1 < it1) {
-            // every thread is done with this build's LDS and carry; the next build starts
-            // from the final pivot of the last row and that row's raw super-diagonal entry
-            // (the row has a successor: a later build exists)
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const int64_t r = rb + SR - 1;
-                carry[0] = sg[SR - 1];
-                carry[1] = mem_load<T>(mat, d.udS + source_pos(d, r, d.N), d.mem_f64);
-            }
-            __syncthreads();
-        }
+    int64_t build = blockIdx.x;
+    if (chained) {
+        if (threadIdx.x == 0) s_build = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        build = s_build;
     }
+    const int64_t rb = build * SR;
+    // the raw super-diagonal entry of the row before the build (the build's first barrier
+    // publishes it)
+    if (chained && threadIdx.x == 0)
+        carry[1] = rb > 0 ? mem_load<T>(mat, d.udS + source_pos(d, rb - 1, d.N), d.mem_f64) : Ar<T>::zero();
+    const ChainHooks<T> hooks{carry, sync + 1, gval, err, build, chained};
+    prologue_body<T, ChainHooks<T>>(d, mat, table + build * NSUB * TE, kLongL, R, 1, pro_smem, rb,
+                                    chained ? carry : nullptr, NSUB, hooks);
+}
+
+__global__ void jw_long_clear_kernel(int* p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // ------------------------------------------------------------------------
 // chunk kernel: grid (workgroups per chunk, chunks)
 // ------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kLongThreads) void jw_long_chunk_kernel(jw_long_desc d)
+template <typename T, int MODE>
+__global__ __launch_bounds__(LongGeom<T>::BOUND) void jw_long_chunk_kernel(jw_long_desc d)
 {
     using A = Ar<T>;
     constexpr int R = LongGeom<T>::R;
@@ -121,69 +166,99 @@ __global__ __launch_bounds__(kLongThreads) void jw_long_chunk_kernel(jw_long_des
     const int t = lane;
     const int64_t rb = c * C;
     const int64_t r0 = rb + (int64_t)t * R;
-    const int mode = d.mode;
+    constexpr int mode = MODE;   // one kernel per mode: no mode branches at run time
     const int mf = rd.mem_f64;
     const int64_t nc = d.nchunks;
     const int64_t nsys = mode == JW_LONG_COEF ? 3 : rd.nsys;
     const int64_t npass = (nsys + 1) / 2;
     T* red = reinterpret_cast<T*>(d.red);
     const int64_t rs = d.red_sys;
-    auto at = [&](int a, int64_t cc, int64_t s) -> int64_t { return ((int64_t)a * nc + cc) * rs + s; };
+    auto at = [&](int a, int64_t cc, int64_t s) -> int64_t { return ((int64_t)a * rs + s) * nc + cc; };
     T* coef = red + (int64_t)JW_LONG_ARRAYS * nc * rs;
 
     for (int64_t q = (int64_t)blockIdx.x * nwaves + wave; q < npass; q += (int64_t)gridDim.x * nwaves) {
         const int64_t sys0 = 2 * q;
         T x[2][R];
-        T hm2[2], hm1[2], hp1[2], hp2[2], cw[2], cz[2], cy[2];
+        // the rows next to the chunk (lane 0: rows rb-1, rb-2; lane 63: rows rb+C, rb+C+1)
+        // and the carries (lane 0: w entering; lane 63: z and y entering)
+        T h1[2], h2[2], ca[2], cb[2];
 
-        // ---- load: rows outside [in_lo, in_hi) (in_hi <= N) and absent systems read as 0 ----
+        // ---- load: rows outside [in_lo, in_hi) (in_hi <= N) and absent systems read as 0.
+        // A system whose chunk lies inside the keep-range (the common case; a wavefront-
+        // uniform test) is loaded straight, every load issued before the first use; the
+        // others row by row.  Guarded loads start from a zero already in the register, so
+        // no path overwrites a register a load is still filling (which would make the
+        // compiler wait for every load in flight). ----
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int64_t sys = sys0 + s;
+            const bool ok = sys < nsys && mode != JW_LONG_COEF;
+            const bool full = ok && rb >= rd.in_lo && rb + C <= rd.in_hi;
+            const int64_t base = sys * rd.stride + rd.offSol;
+            if (full) {
+                if constexpr (sizeof(T) == 8) {
+                    const double* pp = reinterpret_cast<const double*>(rd.rhs) + base + r0;
+                    if (rd.vec2) {
+#pragma unroll
+                        for (int i = 0; i < R; i += 2) {
+                            const double2 v = *reinterpret_cast<const double2*>(pp + i);
+                            x[s][i] = v.x;
+                            x[s][i + 1] = v.y;
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < R; i++) x[s][i] = pp[i];
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < R; i++) x[s][i] = mem_load<T>(rd.rhs, base + r0 + i, mf);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < R; i++) {
+                    const int64_t gr = r0 + i;
+                    x[s][i] = A::zero();
+                    if (ok && gr >= rd.in_lo && gr < rd.in_hi) x[s][i] = mem_load<T>(rd.rhs, base + gr, mf);
+                }
+            }
+        }
 #pragma unroll
         for (int s = 0; s < 2; s++) {
             const int64_t sys = sys0 + s;
             const bool ok = sys < nsys && mode != JW_LONG_COEF;
             const int64_t base = sys * rd.stride + rd.offSol;
-#pragma unroll
-            for (int i = 0; i < R; i++) {
-                const int64_t gr = r0 + i;
-                const bool k = ok && gr >= rd.in_lo && gr < rd.in_hi;
-                x[s][i] = k ? mem_load<T>(rd.rhs, base + gr, mf) : A::zero();
-            }
-            hm2[s] = hm1[s] = hp1[s] = hp2[s] = A::zero();
+            h1[s] = A::zero();
+            h2[s] = A::zero();
             if (rd.band && ok) {
                 if (mode == JW_LONG_REDUCE) {
                     // the neighbouring chunks' rows, before any of them is written (the
-                    // finishing launch takes them from the reduced data instead)
-                    auto rhs_at = [&](int64_t gr) -> T {
-                        return (gr >= rd.in_lo && gr < rd.in_hi) ? mem_load<T>(rd.rhs, base + gr, mf) : A::zero();
-                    };
-                    if (t == 0) {
-                        hm2[s] = rhs_at(rb - 2);
-                        hm1[s] = rhs_at(rb - 1);
-                    }
-                    if (t == L - 1) {
-                        hp1[s] = rhs_at(rb + C);
-                        hp2[s] = rhs_at(rb + C + 1);
-                    }
+                    // finishing launch takes them from the reduced data instead): lane 0
+                    // rows rb-1, rb-2, lane 63 rows rb+C, rb+C+1
+                    const int64_t g1 = t == 0 ? rb - 1 : rb + C;
+                    const int64_t g2 = t == 0 ? rb - 2 : rb + C + 1;
+                    const bool edge = t == 0 || t == L - 1;
+                    if (edge && g1 >= rd.in_lo && g1 < rd.in_hi) h1[s] = mem_load<T>(rd.rhs, base + g1, mf);
+                    if (edge && g2 >= rd.in_lo && g2 < rd.in_hi) h2[s] = mem_load<T>(rd.rhs, base + g2, mf);
                 } else {
-                    if (t == 0 && c > 0) {
-                        hm2[s] = red[at(5, c - 1, sys)];
-                        hm1[s] = red[at(6, c - 1, sys)];
-                    }
-                    if (t == L - 1 && c + 1 < nc) {
-                        hp1[s] = red[at(3, c + 1, sys)];
-                        hp2[s] = red[at(4, c + 1, sys)];
+                    const bool lo = t == 0 && c > 0, hi = t == L - 1 && c + 1 < nc;
+                    const int64_t i1 = lo ? at(6, c - 1, sys) : at(3, c + 1, sys);
+                    const int64_t i2 = lo ? at(5, c - 1, sys) : at(4, c + 1, sys);
+                    if (lo || hi) {
+                        h1[s] = red[i1];
+                        h2[s] = red[i2];
                     }
                 }
             }
-            cw[s] = cz[s] = cy[s] = A::zero();
+            ca[s] = A::zero();
+            cb[s] = A::zero();
             if (mode == JW_LONG_COEF) {
-                if (sys == 0) cw[s] = A::one();
-                if (sys == 1) cz[s] = A::one();
-                if (sys == 2) cy[s] = A::one();
+                if (t == 0 && sys == 0) ca[s] = A::one();
+                if (t == L - 1 && sys == 1) ca[s] = A::one();
+                if (t == L - 1 && sys == 2) cb[s] = A::one();
             } else if (mode == JW_LONG_FINISH && sys < nsys) {
-                cw[s] = red[at(7, c, sys)];
-                cz[s] = red[at(8, c, sys)];
-                cy[s] = red[at(9, c, sys)];
+                // lane 0: w entering; lane 63: z and y entering
+                if (t == 0 || t == L - 1) ca[s] = red[at(t == 0 ? 7 : 8, c, sys)];
+                if (t == L - 1) cb[s] = red[at(9, c, sys)];
             }
             if (mode == JW_LONG_REDUCE && sys < nsys) {
                 if (t == 0) {
@@ -202,10 +277,10 @@ __global__ __launch_bounds__(kLongThreads) void jw_long_chunk_kernel(jw_long_des
 #pragma unroll
             for (int s = 0; s < 2; s++) {
                 T xm2 = A::zero(), xm1 = A::zero(), xp1 = A::zero(), xp2 = A::zero();
-                if (rd.kl >= 1) xm1 = A::sel(t >= 1, lane_fetch(x[s][R - 1], t >= 1 ? lane - 1 : lane), hm1[s]);
-                if (rd.kl >= 2) xm2 = A::sel(t >= 1, lane_fetch(x[s][R - 2], t >= 1 ? lane - 1 : lane), hm2[s]);
-                if (rd.ku >= 1) xp1 = A::sel(t + 1 < L, lane_fetch(x[s][0], t + 1 < L ? lane + 1 : lane), hp1[s]);
-                if (rd.ku >= 2) xp2 = A::sel(t + 1 < L, lane_fetch(x[s][1], t + 1 < L ? lane + 1 : lane), hp2[s]);
+                if (rd.kl >= 1) xm1 = A::sel(t >= 1, lane_fetch(x[s][R - 1], t >= 1 ? lane - 1 : lane), h1[s]);
+                if (rd.kl >= 2) xm2 = A::sel(t >= 1, lane_fetch(x[s][R - 2], t >= 1 ? lane - 1 : lane), h2[s]);
+                if (rd.ku >= 1) xp1 = A::sel(t + 1 < L, lane_fetch(x[s][0], t + 1 < L ? lane + 1 : lane), h1[s]);
+                if (rd.ku >= 2) xp2 = A::sel(t + 1 < L, lane_fetch(x[s][1], t + 1 < L ? lane + 1 : lane), h2[s]);
                 T pm2 = xm2, pm1 = xm1;
 #pragma unroll
                 for (int i = 0; i < R; i++) {
@@ -235,7 +310,7 @@ __global__ __launch_bounds__(kLongThreads) void jw_long_chunk_kernel(jw_long_des
             T acc[2];
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                T a = A::sel(t == 0, cw[s], A::zero());
+                T a = ca[s];   // zero on every lane but lane 0
 #pragma unroll
                 for (int i = 0; i < R; i++) a = A::msub(x[s][i], lr[i], a);
                 acc[s] = a;
@@ -254,7 +329,7 @@ __global__ __launch_bounds__(kLongThreads) void jw_long_chunk_kernel(jw_long_des
 #pragma unroll
             for (int s = 0; s < 2; s++) {
                 const T cprev = lane_fetch(acc[s], t >= 1 ? lane - 1 : lane);
-                T w = A::sel(t >= 1, cprev, cw[s]);
+                T w = A::sel(t >= 1, cprev, ca[s]);
 #pragma unroll
                 for (int i = 0; i < R; i++) {
                     w = A::msub(x[s][i], lr[i], w);
@@ -278,13 +353,13 @@ __global__ __launch_bounds__(kLongThreads) void jw_long_chunk_kernel(jw_long_des
                     gg[i] = tab[tab_coef(p, i, t, 0, R, L)];
                     hh[i] = tab[tab_coef(p, i, t, 1, R, L)];
                 }
-                T cin[2];
+                T cin[2];   // zero on every lane but lane 63
 #pragma unroll
-                for (int s = 0; s < 2; s++) cin[s] = stage == 0 ? cz[s] : cy[s];
+                for (int s = 0; s < 2; s++) cin[s] = A::sel(t == L - 1, stage == 0 ? ca[s] : cb[s], A::zero());
                 T acc[2];
 #pragma unroll
                 for (int s = 0; s < 2; s++) {
-                    T a = A::sel(t == L - 1, cin[s], A::zero());
+                    T a = cin[s];
 #pragma unroll
                     for (int i = R - 1; i >= 0; i--) {
                         x[s][i] = A::mul(gg[i], x[s][i]);
@@ -327,16 +402,39 @@ __global__ __launch_bounds__(kLongThreads) void jw_long_chunk_kernel(jw_long_des
 #pragma unroll
                 for (int s = 0; s < 2; s++) x[s][0] = A::mul(x[s][0], sc);
             }
-            // ---- store rows in [out_lo, out_hi) (out_hi <= N) ----
+            // ---- store rows in [out_lo, out_hi) (out_hi <= N): straight when the chunk
+            // lies inside the range (wavefront-uniform), else row by row ----
 #pragma unroll
             for (int s = 0; s < 2; s++) {
                 const int64_t sys = sys0 + s;
                 if (sys >= nsys) continue;
                 const int64_t base = sys * rd.stride + rd.offSol;
+                const bool full = rb >= rd.out_lo && rb + C <= rd.out_hi;
+                if (full) {
+                    if constexpr (sizeof(T) == 8) {
+                        double* pp = reinterpret_cast<double*>(rd.rhs) + base + r0;
+                        if (rd.vec2) {
 #pragma unroll
-                for (int i = 0; i < R; i++) {
-                    const int64_t gr = r0 + i;
-                    if (gr >= rd.out_lo && gr < rd.out_hi) mem_store<T>(rd.rhs, base + gr, x[s][i], mf);
+                            for (int i = 0; i < R; i += 2) {
+                                double2 v;
+                                v.x = x[s][i];
+                                v.y = x[s][i + 1];
+                                *reinterpret_cast<double2*>(pp + i) = v;
+                            }
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < R; i++) pp[i] = x[s][i];
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < R; i++) mem_store<T>(rd.rhs, base + r0 + i, x[s][i], mf);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < R; i++) {
+                        const int64_t gr = r0 + i;
+                        if (gr >= rd.out_lo && gr < rd.out_hi) mem_store<T>(rd.rhs, base + gr, x[s][i], mf);
+                    }
                 }
             }
         } else {
@@ -364,74 +462,95 @@ __global__ __launch_bounds__(kLongThreads) void jw_long_chunk_kernel(jw_long_des
 }
 
 // ------------------------------------------------------------------------
-// reduced solve: one thread per system, the chunks in blocks of B (loads of a block issued
-// together, then the dependent updates)
+// reduced solve: one wavefront per system, lanes over 64 chunks at a time.  Forward: the
+// maps w -> K_w(c) w + F(c) of the chunks, composed by an inclusive scan across the lanes;
+// backward: the maps of (z, y), [[psi, 0], [zeta, omega]] (z, y) + (Z + chi w_in, Y + xi w_in),
+// composed from the last chunk down.  The data of one system are contiguous per array.
 // ------------------------------------------------------------------------
-template <typename T, int B>
+template <typename T>
 __global__ __launch_bounds__(256) void jw_long_reduced_kernel(jw_long_desc d)
 {
     using A = Ar<T>;
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= d.run.nsys) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (s >= d.run.nsys) return;   // wavefront-uniform
     const int64_t nc = d.nchunks, rs = d.red_sys;
-    const T* __restrict__ in = reinterpret_cast<const T*>(d.red);                        // F, Z, Y
-    T* __restrict__ car = reinterpret_cast<T*>(d.red) + 7 * nc * rs;                     // w_in, z_in, y_in
-    const T* __restrict__ K = reinterpret_cast<const T*>(d.red) + JW_LONG_ARRAYS * nc * rs;
+    T* red = reinterpret_cast<T*>(d.red);
+    const T* F = red + (0 * rs + s) * nc;
+    const T* Z = red + (1 * rs + s) * nc;
+    const T* Y = red + (2 * rs + s) * nc;
+    T* Win = red + (7 * rs + s) * nc;
+    T* Zin = red + (8 * rs + s) * nc;
+    T* Yin = red + (9 * rs + s) * nc;
+    const T* K = red + (int64_t)JW_LONG_ARRAYS * nc * rs;
 
-    // w_in of chunk c is w at the last row of chunk c - 1: F_{c-1} + K_w(c-1) w_in(c-1)
+    // forward: w entering chunk c
     T w = A::zero();
-    for (int64_t c0 = 0; c0 < nc; c0 += B) {
-        T f[B], kw[B];
+    for (int64_t c0 = 0; c0 < nc; c0 += 64) {
+        const int64_t c = c0 + lane;
+        const bool on = c < nc;
+        T a = on ? K[9 * c] : A::one();
+        T b = on ? F[c] : A::zero();
+        // inclusive: lane j holds the map of chunks c0..c0+j (later applied after earlier)
 #pragma unroll
-        for (int j = 0; j < B; j++) {
-            const int64_t c = c0 + j;
-            f[j] = A::zero();
-            kw[j] = A::zero();
-            if (c < nc) {
-                f[j] = in[c * rs + s];
-                kw[j] = K[9 * c];
+        for (int k = 0; k < 6; k++) {
+            const int sft = 1 << k;
+            const int src = lane >= sft ? lane - sft : lane;
+            const T pa = lane_fetch(a, src), pb = lane_fetch(b, src);
+            if (lane >= sft) {
+                b = A::madd(b, a, pb);
+                a = A::mul(a, pa);
             }
         }
-#pragma unroll
-        for (int j = 0; j < B; j++) {
-            const int64_t c = c0 + j;
-            if (c < nc) {
-                car[c * rs + s] = w;
-                w = A::madd(f[j], kw[j], w);
-            }
-        }
+        // w leaving chunk c = a w + b; entering chunk c = leaving chunk c - 1
+        const T wout = A::madd(b, a, w);
+        const T prev = lane_fetch(wout, lane >= 1 ? lane - 1 : lane);
+        if (on) Win[c] = lane >= 1 ? prev : w;
+        w = lane_fetch(wout, 63);
     }
-    // z_in, y_in of chunk c are z, y at the first row of chunk c + 1
+    // backward: (z, y) entering chunk c from chunk c + 1
     T z = A::zero(), y = A::zero();
-    for (int64_t c1 = nc - 1; c1 >= 0; c1 -= B) {
-        T zc[B], yc[B], wi[B], k1[B], k2[B], k4[B], k5[B], k8[B];
+    for (int64_t c1 = nc - 1; c1 >= 0; c1 -= 64) {
+        const int64_t c = c1 - lane;
+        const bool on = c >= 0;
+        // map of chunk c: (z, y) -> (p z + bz, q z + r y + by)
+        T p = A::one(), q = A::zero(), r = A::one(), bz = A::zero(), by = A::zero();
+        if (on) {
+            const T wi = Win[c];
+            p = K[9 * c + 4];
+            q = K[9 * c + 5];
+            r = K[9 * c + 8];
+            bz = A::madd(Z[c], K[9 * c + 1], wi);
+            by = A::madd(Y[c], K[9 * c + 2], wi);
+        }
+        // inclusive over chunks c1 .. c (applied in that order)
 #pragma unroll
-        for (int j = 0; j < B; j++) {
-            const int64_t c = c1 - j;
-            zc[j] = yc[j] = wi[j] = k1[j] = k2[j] = k4[j] = k5[j] = k8[j] = A::zero();
-            if (c >= 0) {
-                zc[j] = in[(nc + c) * rs + s];
-                yc[j] = in[(2 * nc + c) * rs + s];
-                wi[j] = car[c * rs + s];
-                k1[j] = K[9 * c + 1];
-                k2[j] = K[9 * c + 2];
-                k4[j] = K[9 * c + 4];
-                k5[j] = K[9 * c + 5];
-                k8[j] = K[9 * c + 8];
+        for (int k = 0; k < 6; k++) {
+            const int sft = 1 << k;
+            const int src = lane >= sft ? lane - sft : lane;
+            const T pp = lane_fetch(p, src), pq = lane_fetch(q, src), pr = lane_fetch(r, src);
+            const T pbz = lane_fetch(bz, src), pby = lane_fetch(by, src);
+            if (lane >= sft) {
+                // this map after the earlier one: z'' = p (pp z + pbz) + bz,
+                // y'' = q (pp z + pbz) + r (pq z + pr y + pby) + by
+                const T nq = A::madd(A::mul(q, pp), r, pq);
+                by = A::madd(A::madd(by, q, pbz), r, pby);
+                bz = A::madd(bz, p, pbz);
+                q = nq;
+                p = A::mul(p, pp);
+                r = A::mul(r, pr);
             }
         }
-#pragma unroll
-        for (int j = 0; j < B; j++) {
-            const int64_t c = c1 - j;
-            if (c >= 0) {
-                car[(nc + c) * rs + s] = z;
-                car[(2 * nc + c) * rs + s] = y;
-                const T zn = A::madd(A::madd(zc[j], k1[j], wi[j]), k4[j], z);
-                const T yn = A::madd(A::madd(A::madd(yc[j], k2[j], wi[j]), k5[j], z), k8[j], y);
-                z = zn;
-                y = yn;
-            }
+        const T zout = A::madd(bz, p, z);
+        const T yout = A::madd(A::madd(by, q, z), r, y);
+        const T pz = lane_fetch(zout, lane >= 1 ? lane - 1 : lane);
+        const T py = lane_fetch(yout, lane >= 1 ? lane - 1 : lane);
+        if (on) {
+            Zin[c] = lane >= 1 ? pz : z;
+            Yin[c] = lane >= 1 ? py : y;
         }
+        z = lane_fetch(zout, 63);
+        y = lane_fetch(yout, 63);
     }
 }
 
@@ -457,6 +576,16 @@ template <typename T> size_t prologue_lds()
     constexpr int SR = LongGeom<T>::NSUB * LongGeom<T>::R * kLongL;
     return prologue_lds_bytes(SR, sizeof(T)) + 2 * sizeof(T);
 }
+template <typename T>
+const void* chunk_fn(int mode)
+{
+    switch (mode) {
+    case JW_LONG_COEF: return reinterpret_cast<const void*>(&jw_long_chunk_kernel<T, JW_LONG_COEF>);
+    case JW_LONG_REDUCE: return reinterpret_cast<const void*>(&jw_long_chunk_kernel<T, JW_LONG_REDUCE>);
+    default: return reinterpret_cast<const void*>(&jw_long_chunk_kernel<T, JW_LONG_FINISH>);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -471,8 +600,7 @@ int64_t jwdev_long_lds_bytes(int precision)
 
 int jwdev_long_occupancy(int precision)
 {
-    const void* f = precision == JW_PREC_F64 ? reinterpret_cast<const void*>(&jw_long_chunk_kernel<double>)
-                                             : reinterpret_cast<const void*>(&jw_long_chunk_kernel<dd>);
+    const void* f = precision == JW_PREC_F64 ? chunk_fn<double>(JW_LONG_FINISH) : chunk_fn<dd>(JW_LONG_FINISH);
     const size_t lds = (size_t)jwdev_long_lds_bytes(precision);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) { fail("hipFuncSetAttribute(long chunks)", e); return 0; }
@@ -483,7 +611,7 @@ int jwdev_long_occupancy(int precision)
 }
 
 int jwdev_long_prologue(const jw_source_desc* src, const void* matrices, void* table, int64_t tab_elems,
-                        int precision, void* stream)
+                        int precision, int* sync, void* gval, int* err, void* stream)
 {
     if (!matrices || !table) return fail("long prologue: no matrices or table");
     const bool f64 = precision == JW_PREC_F64;
@@ -493,21 +621,29 @@ int jwdev_long_prologue(const jw_source_desc* src, const void* matrices, void* t
                        : LongGeom<dd>::NSUB * LongGeom<dd>::R * kLongL;
     const int64_t nit = (src->N + SR - 1) / SR;
     if (nit < 1) return 0;
-    // pivots continue from one build to the next: one workgroup, in order
-    const int sequential = src->solve_kind == JW_SOLVE_TRI;
+    // pivots continue from one build to the next (tridiagonal solves): the ticket and the
+    // builds' flags start at zero (a fill kernel, so that the launch can be captured)
+    const int chained = src->solve_kind == JW_SOLVE_TRI;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (chained) {
+        if (!sync || !gval || !err) return fail("long prologue: no hand-over scratch");
+        const int n = (int)nit + 1;
+        hipLaunchKernelGGL(jw_long_clear_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sync, n);
+        hipError_t ec = hipGetLastError();
+        if (ec != hipSuccess) return fail("long clear launch", ec);
+    }
     const size_t lds = f64 ? prologue_lds<double>() : prologue_lds<dd>();
     const void* fn = f64 ? reinterpret_cast<const void*>(&jw_long_prologue_kernel<double>)
                          : reinterpret_cast<const void*>(&jw_long_prologue_kernel<dd>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return fail("hipFuncSetAttribute(long prologue)", e);
-    const dim3 grid((unsigned)(sequential ? 1 : nit));
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const dim3 grid((unsigned)nit);
     if (f64)
         hipLaunchKernelGGL(jw_long_prologue_kernel<double>, grid, dim3(kPrologueThreads), lds, s, *src, matrices,
-                           reinterpret_cast<double*>(table), sequential);
+                           reinterpret_cast<double*>(table), chained, sync, reinterpret_cast<double*>(gval), err);
     else
         hipLaunchKernelGGL(jw_long_prologue_kernel<dd>, grid, dim3(kPrologueThreads), lds, s, *src, matrices,
-                           reinterpret_cast<dd*>(table), sequential);
+                           reinterpret_cast<dd*>(table), chained, sync, reinterpret_cast<dd*>(gval), err);
     e = hipGetLastError();
     if (e != hipSuccess) return fail("long prologue launch", e);
     return 0;
@@ -522,8 +658,8 @@ int jwdev_long_chunks(const jw_long_desc* d, int precision, int grid_x, void* st
     if (d->nchunks < 1 || (int64_t)d->nchunks * C < d->run.N || d->nchunks > 65535 || grid_x < 1)
         return fail("long chunks: bad chunk count or grid");
     if (d->mode != JW_LONG_COEF && d->run.nsys > d->red_sys) return fail("long chunks: slice exceeds reduced data");
-    const void* f = f64 ? reinterpret_cast<const void*>(&jw_long_chunk_kernel<double>)
-                        : reinterpret_cast<const void*>(&jw_long_chunk_kernel<dd>);
+    if (d->mode < JW_LONG_COEF || d->mode > JW_LONG_FINISH) return fail("long chunks: bad mode");
+    const void* f = f64 ? chunk_fn<double>(d->mode) : chunk_fn<dd>(d->mode);
     const size_t lds = (size_t)jwdev_long_lds_bytes(precision);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return fail("hipFuncSetAttribute(long chunks)", e);
@@ -539,12 +675,13 @@ int jwdev_long_reduced(const jw_long_desc* d, int precision, void* stream)
     if (!d->red) return fail("long reduced: no reduced data");
     if (d->run.nsys > d->red_sys) return fail("long reduced: slice exceeds reduced data");
     if (d->run.nsys < 1) return 0;
-    const unsigned grid = (unsigned)((d->run.nsys + 255) / 256);
+    // one wavefront per system, four per workgroup
+    const unsigned grid = (unsigned)((d->run.nsys + 3) / 4);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (precision == JW_PREC_F64)
-        hipLaunchKernelGGL((jw_long_reduced_kernel<double, 8>), dim3(grid), dim3(256), 0, s, *d);
+        hipLaunchKernelGGL(jw_long_reduced_kernel<double>, dim3(grid), dim3(256), 0, s, *d);
     else
-        hipLaunchKernelGGL((jw_long_reduced_kernel<dd, 4>), dim3(grid), dim3(256), 0, s, *d);
+        hipLaunchKernelGGL(jw_long_reduced_kernel<dd>, dim3(grid), dim3(256), 0, s, *d);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail("long reduced launch", e);
     return 0;

@@ -46,6 +46,9 @@ struct pfsolve_plan {
     int long_R, long_only, long_occ;
     int64_t long_nc, long_ntab, long_tab_elems, long_red_sys;
     size_t long_tab_off, long_red_off;
+    /* pivot hand-over between table builds: ticket and per-build flags (ints), then the
+     * per-build last pivots from long_gval_off */
+    size_t long_sync_off, long_gval_off;
 };
 
 /* Applies the reference's configuration defaults (pfSolve_InitializeApp.h:981-1212)

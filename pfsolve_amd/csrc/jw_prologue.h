@@ -117,14 +117,24 @@ __device__ __forceinline__ int64_t source_pos(const jw_source_desc& d, int64_t r
                  : (d.permuted ? reorder_pos(r, N, d.Nalign, d.Msplit0) : r);
 }
 
+// No pivot hand-over between builds.
+struct NoPivotHooks {
+    __device__ void wait_carry(int) const {}
+    template <typename T> __device__ void publish(int, const T*, int) const {}
+};
+
 // One table build.  Local row r of the build is row rb + r of the system.  A build of a
 // long system's later rows (rb > 0, format 1) takes from `carry` the pivot of the row
 // before its first (carry[0] = g_{rb-1}) and that row's raw super-diagonal entry
 // (carry[1] = ud_{rb-1}), so that its pivots continue the exact recurrence; it writes
 // nsub format-1 tables of R * L rows each, one after the other (table_elems apart).
-template <typename T>
+// Hooks of a build that hands its pivots on (tridiagonal solves only): wait_carry(tid),
+// called by every thread just before the barrier after which carry[0] is first read, and
+// publish(tid, sg, RL), called by every thread once the g of every row is final in sg.
+template <typename T, typename H = NoPivotHooks>
 __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const void* mat, T* table, int L, int R, int format,
-                              unsigned char* smem_raw, int64_t rb = 0, const T* carry = nullptr, int nsub = 1)
+                              unsigned char* smem_raw, int64_t rb = 0, const T* carry = nullptr, int nsub = 1,
+                              const H& hooks = H())
 {
     using A = Ar<T>;
     const int RLs = R * L;                 // rows of one table
@@ -268,6 +278,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 xa = A::one(); xb = A::zero(); xc = A::zero(); xe = A::one();
             }
         }
+        hooks.wait_carry(tid);
         __syncthreads();
         T gprev = A::one();
         if (part) {
@@ -403,6 +414,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         // refined: the corrected g of every row (and of the row before) is in sg; not
         // refined: every thread has read the flag before thread 0 writes h_0 over it
         __syncthreads();
+        hooks.publish(tid, sg, RL);
         if (part) {
 #pragma unroll
             for (int i = 0; i < kMaxRT; i++) {

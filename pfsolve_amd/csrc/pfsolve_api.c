@@ -51,7 +51,7 @@ static PfSolveResult fill_geometry(const PfSolveConfiguration* cfg, pfsolve_geom
         const int64_t ntab = (cfg->M_size + build - 1) / build * jwdev_long_nsub(kp);
         g->lanes = 64;
         g->rows_per_lane = jwdev_long_rows(kp);
-        g->waves = 8;
+        g->waves = 4;
         g->table_bytes = ntab * jwdev_table_elems(64, g->rows_per_lane) * (kp == JW_PREC_F64 ? 8 : 16);
         g->lds_bytes = jwdev_long_lds_bytes(kp);
         return PFSOLVE_SUCCESS;
@@ -136,7 +136,11 @@ static PfSolveResult long_setup(PfSolveApplication* app, struct pfsolve_plan* p)
     p->long_red_sys = cap;
     p->long_tab_off = (p->table_bytes + 255) / 256 * 256;
     p->long_red_off = p->long_tab_off + ((size_t)p->long_ntab * p->long_tab_elems * elt + 255) / 256 * 256;
-    p->table_bytes = p->long_red_off + (size_t)(JW_LONG_ARRAYS * p->long_nc * cap + 9 * p->long_nc) * elt;
+    const int64_t nbuild = p->long_ntab / jwdev_long_nsub(kp);
+    p->long_sync_off = p->long_red_off +
+                       ((size_t)(JW_LONG_ARRAYS * p->long_nc * cap + 9 * p->long_nc) * elt + 255) / 256 * 256;
+    p->long_gval_off = p->long_sync_off + ((size_t)(nbuild + 1) * sizeof(int) + 255) / 256 * 256;
+    p->table_bytes = p->long_gval_off + (size_t)nbuild * elt;
     p->table_stride = (p->table_bytes + 255) / 256 * 256;
     return PFSOLVE_SUCCESS;
 }
@@ -528,19 +532,25 @@ static PfSolveResult append_long(PfSolveApplication* app, const jw_source_desc* 
     d.red = (char*)slot_area + p->long_red_off;
     d.red_sys = p->long_red_sys;
     d.nchunks = (int32_t)p->long_nc;
+    /* FP64 systems 16 bytes aligned (every slice then too): row pairs move as 16-byte accesses */
+    d.run.vec2 = p->precision == JW_PREC_F64 && ((uintptr_t)run->rhs % 16) == 0 && (run->stride % 2) == 0 &&
+                 (run->offSol % 2) == 0;
     if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
-    if (jwdev_long_prologue(src, cfg->buffer[0], (void*)d.table, d.tab_elems, p->precision, s) != 0)
+    if (jwdev_long_prologue(src, cfg->buffer[0], (void*)d.table, d.tab_elems, p->precision,
+                            (int*)((char*)slot_area + p->long_sync_off), (char*)slot_area + p->long_gval_off, p->d_err,
+                            s) != 0)
         return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     d.mode = JW_LONG_COEF;
     d.run.nsys = 3;
     if (jwdev_long_chunks(&d, p->precision, 1, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
-    /* workgroups per chunk: enough for two rounds of the whole GPU, at most one per 8 pairs */
+    /* workgroups per chunk: enough for two rounds of the whole GPU, at most one per 4 pairs
+     * (a workgroup of 4 wavefronts, one pair of systems per wavefront pass) */
     const int64_t target = 2 * (int64_t)p->cu_count * p->long_occ;
     for (int64_t s0 = 0; s0 < run->nsys; s0 += p->long_red_sys) {
         const int64_t n = run->nsys - s0 < p->long_red_sys ? run->nsys - s0 : p->long_red_sys;
         d.run.nsys = n;
         d.run.offSol = run->offSol + s0 * run->stride;
-        const int64_t need = ((n + 1) / 2 + 7) / 8;
+        const int64_t need = ((n + 1) / 2 + 3) / 4;
         int64_t gx = (target + p->long_nc - 1) / p->long_nc;
         if (gx > need) gx = need;
         if (gx < 1) gx = 1;

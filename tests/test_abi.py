@@ -174,7 +174,7 @@ def test_chunked_geometry(N, dd):
         rc, g = validate(cfg(N=N, jw_type=jt, upperBanded=ub, **prec))
         assert rc == 0, (jt, ub)
         R = 4 if dd else 8
-        assert g.lanes == 64 and g.rows_per_lane == R and g.waves == 8
+        assert g.lanes == 64 and g.rows_per_lane == R and g.waves == 4
         nbuild = -(-N // (4 * 64 * R))
         assert g.table_bytes == nbuild * 4 * (10 * 64 * R + 24 * 64) * (16 if dd else 8)
         assert g.lds_bytes == (10 * 64 * R + 24 * 64) * (16 if dd else 8)
