@@ -483,6 +483,37 @@ def bench_chain(torch, pf, steps: int, warmup: int, N: int = 512, ncx: int = 655
     }
 
 
+def bench_long(torch, pf, steps: int, warmup: int, N: int = 100000, ncx: int = 256) -> dict:
+    """Long systems: the fused step y = M^-1 T^-1 V x at N = 100000 rows (matrices in the
+    reference's Msplit0 = 704 block storage) on 256 complex systems (XXII), FP64.  The step is
+    the chunk path: per-chunk tables with the pivots handed from build to build, the chunks'
+    responses to unit carries, the reduction, the reduced solve and the finishing pass."""
+    lib = pf.lib()
+    nreal = 2 * ncx
+    buf, (offV, offT, offM) = gen_fused_matrices(N, 3, lib.pfsolve_reorder_i0)
+    dmat = torch.from_numpy(buf).cuda()
+    g = torch.Generator(device="cuda").manual_seed(9)
+    x = torch.rand(nreal * N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    stream = torch.cuda.Stream()
+    plan = pf.Plan(dmat.data_ptr(), x.data_ptr(), stream=stream.cuda_stream, size=[N, nreal, 1], jw_type=14,
+                   doublePrecision=1, scaleC=1.0, KL=2, KU=2, offsetV=offV, offsetT=offT, offsetM=offM)
+    geo = plan.geometry()
+    ms, per, _, kms = time_plan_steps(torch, lib, plan, stream, warmup, steps, plan.append)
+    plan.close()
+    assert torch.isfinite(x).all().item(), "non-finite long-system output"
+    # minimal traffic: every row of every system read and written once, the matrices once
+    alg = algorithmic_bytes(N, nreal, N, N)
+    return {
+        "workload": f"fused JW step on long systems, N = {N} rows (Msplit0 = 704 storage blocks), {ncx} complex "
+                    f"systems (XXII), KL=KU=2, FP64",
+        "value": ncx / (ms * 1e-3), "unit": "complex systems/s", "ms_per_step": ms,
+        "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+        "chunk_rows": 64 * geo.rows_per_lane, "kernels": "jw_long_prologue_kernel, jw_long_chunk_kernel x3, "
+                                                         "jw_long_reduced_kernel",
+        "kernels_ms": kms, "algorithmic_bytes": alg, "hbm_frac": alg / (kms * 1e-3) / HBM_PEAK,
+    }
+
+
 def plan_rows(N: int) -> int:
     r = 2
     while r * 64 < N:
@@ -636,6 +667,7 @@ def main() -> int:
     ap.add_argument("--no-dd", action="store_true", help="skip the double-double measurement")
     ap.add_argument("--no-fused-blocks", action="store_true", help="skip the fused block I/O measurement")
     ap.add_argument("--no-chain", action="store_true", help="skip the chained-iterations measurement")
+    ap.add_argument("--no-long", action="store_true", help="skip the long-system (N = 100000) measurement")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the serving stream (60 = the full stream; 0 skips it)")
     ap.add_argument("--no-stream-reference", action="store_true",
@@ -782,6 +814,8 @@ def main() -> int:
         result["fused_blocks"] = bench_fused_blocks(torch, pf, max(10, K // 2), 3)
     if not args.no_chain:
         result["chain"] = bench_chain(torch, pf, max(10, K // 2), 3)
+    if not args.no_long:
+        result["long"] = bench_long(torch, pf, max(10, K // 2), 3)
     if args.serve_seconds > 0:
         result["serve"] = bench_serve(torch, pf, args.serve_seconds)
     if args.calibrate or not args.no_stream_reference:

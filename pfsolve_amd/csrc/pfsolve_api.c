@@ -985,6 +985,13 @@ PfSolveResult pfsolve_plan_geometry(const PfSolveApplication* app, pfsolve_geome
     g->precision = p->precision;
     g->table_bytes = (int64_t)p->table_bytes;
     g->lds_bytes = p->lds_bytes;
+    if (p->long_only) {
+        /* long systems: the chunk kernel's geometry, every chunk's table */
+        g->lanes = 64;
+        g->rows_per_lane = p->long_R;
+        g->table_bytes = p->long_ntab * p->long_tab_elems * (p->precision == JW_PREC_F64 ? 8 : 16);
+        return PFSOLVE_SUCCESS;
+    }
     g->fast_rows_per_lane = p->R64;
     g->fast_waves = p->split64 ? p->split64 : p->waves64;
     if (p->split64) {
