@@ -49,6 +49,10 @@ struct pfsolve_plan {
     /* pivot hand-over between table builds: ticket and per-build flags (ints), then the
      * per-build last pivots from long_gval_off */
     size_t long_sync_off, long_gval_off;
+    /* bytes of a slot's area the paths other than chunks use (the chunk fallback of FP64
+     * 1025-2048 rows adds its tables after them), and the area actually provided (0: the
+     * full table_stride, as initializePfSolve allocates it) */
+    size_t main_bytes, area_cap;
 };
 
 /* Applies the reference's configuration defaults (pfSolve_InitializeApp.h:981-1212)

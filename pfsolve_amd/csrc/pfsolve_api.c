@@ -249,6 +249,7 @@ static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
         } else if (cfg->kernelVariant != 2) {
             /* systems the wave-per-system kernel does not take (unaligned, or the general
              * kernel asked for) run in chunks */
+            p->main_bytes = p->table_stride;
             res = long_setup(app, p);
             if (res != PFSOLVE_SUCCESS) return res;
         }
@@ -523,6 +524,7 @@ static PfSolveResult append_long(PfSolveApplication* app, const jw_source_desc* 
     struct pfsolve_plan* p = app->localFFTPlan;
     PfSolveConfiguration* cfg = &app->configuration;
     if (!p->long_R) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+    if (p->area_cap && p->table_bytes > p->area_cap) return PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER;
     if (run->nsys <= 0) return PFSOLVE_SUCCESS;
     jw_long_desc d;
     memset(&d, 0, sizeof(d));
@@ -745,7 +747,10 @@ PfSolveResult pfsolve_append_noalloc(const PfSolveConfiguration* configuration,
         int* err = NULL;
         size_t cap = 0;
         if (jwdev_noalloc_scratch(&table, &err, &cap) != 0) return PFSOLVE_ERROR_INVALID_DEVICE;
-        if (plan.table_stride > cap) return PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER;
+        /* a chunk fallback that does not fit is refused when a launch needs it */
+        const size_t need = plan.main_bytes ? plan.main_bytes : plan.table_stride;
+        if (need > cap) return PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER;
+        plan.area_cap = cap;
         plan.table = table;
         plan.d_err = err;
         /* the flag starts at zero for this call (a fill kernel: callers may capture it) */

@@ -100,6 +100,8 @@ def test_noalloc_refusals():
     cfg.num_streams = 2
     assert lib.pfsolve_append_noalloc(C.byref(cfg), None) == _abi.PFSOLVE_ERROR_INVALID_QUEUE
     cfg.num_streams = 1
+    cfg.size[0] = 1501   # odd stride: the chunk path, whose tables do not fit the scratch
+    assert lib.pfsolve_append_noalloc(C.byref(cfg), None) == _abi.PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER
     cfg.size[0] = 5000   # a long system: its chunk tables and reduced data exceed the scratch
     assert lib.pfsolve_append_noalloc(C.byref(cfg), None) == _abi.PFSOLVE_ERROR_INSUFFICIENT_TEMP_BUFFER
     cfg.size[0] = (1 << 22) + 1
