@@ -189,25 +189,37 @@ void pfsolve_batch_destroy(pfsolve_batch* b);
  * pfsolve_serve runs a request loop over `reqs` (sorted by arrival time): each
  * request is a bundle of `nsys` real systems of size N (host input rhs_in and
  * output rhs_out, systems contiguous) whose fused-step matrices (the jw_type
- * 14 / 1014 layout, host memory) are identified by matrix_key.  Requests are
- * admitted in arrival order while device memory (one arena of hbm_budget
- * bytes for bundles in flight and resident matrices) and max_inflight allow;
- * matrices and their coefficient table stay resident per key for the whole
- * run; every admitted request waiting for launch joins the next launch on a
- * free stream slot (up to max_batch requests: copies in, one persistent
- * queue kernel per row class, copies out), so new requests are launched while
- * earlier launches are still on the device.  realtime = 1 admits a request no
- * earlier than its arrival_s (seconds from the start, wall clock); realtime =
- * 0 treats every request as arrived at the start.  Host buffers should be
- * pinned for the copies to overlap.  A request that cannot fit the budget even
- * alone is rejected (status PFSOLVE_ERROR_FAILED_TO_ALLOCATE), as is a key
- * reused with another N.  No counterpart in the reference. */
+ * 14 / 1014 layout, host memory, unchanged during the run) are identified by
+ * matrix_key.  Requests are admitted in arrival order while device memory (one
+ * arena of hbm_budget bytes for bundles in flight and resident matrices) and
+ * max_inflight allow; matrices and their coefficient table stay resident per key
+ * (any number of keys) until the arena needs their room: keys with no request in
+ * flight are then evicted least recently used first.  Every admitted request
+ * waiting for launch joins the next launch on a free stream slot (up to max_batch
+ * requests: a new key's matrices and table, copies in, one persistent queue kernel
+ * per row class, copies out), so new requests are launched while earlier launches
+ * are still on the device; finished launches are found from their events.
+ * realtime = 1 admits a request no earlier than its arrival_s (seconds from the
+ * start, wall clock); realtime = 0 treats every request as arrived at the start.
+ * Host buffers should be pinned for the copies to overlap.  A request that cannot
+ * fit the budget even alone is rejected (status PFSOLVE_ERROR_FAILED_TO_ALLOCATE),
+ * as is a key reused with another N.  A launch that fails ends its requests with
+ * the failure's status and the loop goes on with the others (fail_launch = k > 0
+ * makes the k-th launch fail before its kernels, to test that path).  dry_run = n
+ * > 0 runs the same policy without a device: host memory for the arena, copies
+ * instead of transfers, no kernels (every result equals its input), each launch
+ * complete after n polls, and every completion checks that its keys' matrices were
+ * not overwritten while it was in flight (a mismatch: status
+ * PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE, counted in integrity_failures).  No
+ * counterpart in the reference. */
 typedef struct pfsolve_serve_config {
     int64_t hbm_budget;       /* bytes of device memory the loop may hold */
     int32_t max_inflight;     /* requests admitted and not finished, at most */
     int32_t streams;          /* stream slots (launches in flight), 1..16 */
     int32_t max_batch;        /* requests per launch */
     int32_t realtime;
+    int32_t fail_launch;      /* 0, or the launch (1-based) made to fail */
+    int32_t dry_run;          /* 0: the GPU; n > 0: host dry run, launches done after n polls */
 } pfsolve_serve_config;
 
 typedef struct pfsolve_serve_request {
@@ -230,7 +242,11 @@ typedef struct pfsolve_serve_stats {
     double p50_ms, p99_ms, max_ms;    /* completion - arrival (realtime) or - start */
     double mean_batch;                /* requests per launch */
     int64_t peak_bytes;               /* arena bytes in use at the peak */
-    int64_t resident_keys, resident_bytes;
+    int64_t resident_keys, resident_bytes;   /* at the end of the run */
+    int64_t evictions;                /* keys evicted to make room */
+    int64_t failed;                   /* requests ended by a failed launch */
+    int64_t integrity_failures;       /* dry run: completions whose matrices were overwritten */
+    int64_t peak_keys;                /* resident keys at the peak */
 } pfsolve_serve_stats;
 
 PfSolveResult pfsolve_serve(const pfsolve_serve_config* cfg, pfsolve_serve_request* reqs, int64_t nreq,

@@ -238,9 +238,11 @@ class Batch:
 
 
 def serve(requests, hbm_budget: int, max_inflight: int = 256, streams: int = 4, max_batch: int = 64,
-          realtime: bool = False):
+          realtime: bool = False, fail_launch: int = 0, dry_run: int = 0):
     """Runs the serving loop (pfsolve_serve) over `requests`, dicts with the
     pfsolve_serve_request fields (host pointers as ints), sorted by arrival_s.
+    fail_launch = k > 0 makes the k-th launch fail; dry_run = n > 0 runs the policy
+    on the host (no device, launches done after n polls).
     Returns (stats, per-request (admit_s, done_s, status) list)."""
     arr = (_abi.PfSolveServeRequest * max(1, len(requests)))()
     for i, q in enumerate(requests):
@@ -248,7 +250,8 @@ def serve(requests, hbm_budget: int, max_inflight: int = 256, streams: int = 4, 
         d.update(q)
         for k, v in d.items():
             setattr(arr[i], k, v)
-    cfg = _abi.PfSolveServeConfig(hbm_budget, max_inflight, streams, max_batch, int(realtime))
+    cfg = _abi.PfSolveServeConfig(hbm_budget, max_inflight, streams, max_batch, int(realtime), int(fail_launch),
+                                  int(dry_run))
     st = _abi.PfSolveServeStats()
     check(lib().pfsolve_serve(C.byref(cfg), arr, len(requests), C.byref(st)), "pfsolve_serve")
     return st, [(arr[i].admit_s, arr[i].done_s, arr[i].status) for i in range(len(requests))]

@@ -331,7 +331,8 @@ class PfSolveBatchInfo(C.Structure):
 
 class PfSolveServeConfig(C.Structure):
     _fields_ = [("hbm_budget", i64), ("max_inflight", C.c_int32), ("streams", C.c_int32),
-                ("max_batch", C.c_int32), ("realtime", C.c_int32)]
+                ("max_batch", C.c_int32), ("realtime", C.c_int32), ("fail_launch", C.c_int32),
+                ("dry_run", C.c_int32)]
 
 
 class PfSolveServeRequest(C.Structure):
@@ -350,7 +351,8 @@ class PfSolveServeStats(C.Structure):
         ("requests", i64), ("rejected", i64), ("systems", i64), ("launches", i64),
         ("wall_s", C.c_double), ("systems_per_s", C.c_double),
         ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double), ("mean_batch", C.c_double),
-        ("peak_bytes", i64), ("resident_keys", i64), ("resident_bytes", i64),
+        ("peak_bytes", i64), ("resident_keys", i64), ("resident_bytes", i64), ("evictions", i64),
+        ("failed", i64), ("integrity_failures", i64), ("peak_keys", i64),
     ]
 
 

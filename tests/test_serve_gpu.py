@@ -127,3 +127,149 @@ def test_serve_real_time_arrivals():
         assert s == 0 and adm >= w.reqs[i]["arrival_s"] - 1e-6 and done >= adm
     for i in (0, n // 2, n - 1):
         w.check(i)
+
+
+def test_serve_injected_launch_failure_on_the_device():
+    """the second launch fails before its kernels (forced from the host): its two requests
+    end with PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL, the key whose table it would have written
+    is written by a later launch, and every other request matches the oracle"""
+    specs = [(100, 3, 1, 1, 1.0), (513, 2, 2, 1, 1.0), (300, 4, 3, 1, 1.0), (513, 1, 2, 1, 1.0),
+             (300, 2, 3, 1, 0.5), (64, 5, 4, 1, 1.0), (100, 2, 1, 1, 1.0)]
+    w = Workload(specs, seed=6)
+    st, out = pf.serve(w.reqs, hbm_budget=64 << 20, streams=1, max_batch=2, max_inflight=2, fail_launch=2)
+    codes = [c for _, _, c in out]
+    assert [i for i, c in enumerate(codes) if c != 0] == [2, 3]
+    assert codes[2] == codes[3] == _abi.PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL
+    assert st.failed == 2 and st.requests == len(specs) - 2
+    for i in (0, 1, 4, 5, 6):
+        w.check(i)
+
+
+def test_serve_evicts_under_a_budget_with_more_keys_than_the_old_table():
+    """4600 matrix sets through a budget that holds a few hundred: keys with no request in
+    flight are evicted least recently used first, keys come back after eviction (uploaded
+    and their table written again), and the results match the oracle"""
+    nkeys = 4600
+    specs = [(48 + (k % 3) * 8, 2, k, 1, 1.0) for k in range(nkeys)]
+    # early keys come back after they were evicted
+    specs += [(48 + (k % 3) * 8, 2, k, 1, 1.0) for k in range(0, 400, 7)]
+    w = Workload(specs, seed=7)
+    budget = 8 << 20
+    st, out = pf.serve(w.reqs, hbm_budget=budget, streams=4, max_batch=8, max_inflight=64)
+    assert st.requests == len(specs) and st.rejected == 0 and st.failed == 0
+    assert st.evictions > nkeys - 400 and st.peak_bytes <= budget
+    for i in list(range(0, nkeys, 211)) + list(range(nkeys, len(specs), 5)):
+        w.check(i)
+
+
+def ring_offsets(nbytes, ring):
+    """sequential regions of a ring (a region that would cross the end starts again at 0):
+    offsets, cycle numbers"""
+    offs, cyc, pos, c = [], [], 0, 0
+    for b in nbytes:
+        if pos + b > ring:
+            pos, c = 0, c + 1
+        offs.append(pos)
+        cyc.append(c)
+        pos += b
+    return np.array(offs), np.array(cyc)
+
+
+def ring_overlaps(offs, cyc, nbytes):
+    """for each request, the requests of the previous cycle whose regions it reuses"""
+    out = [[] for _ in offs]
+    byc = {}
+    for i, c in enumerate(cyc):
+        byc.setdefault(int(c), []).append(i)
+    for c, js in byc.items():
+        prev = byc.get(c - 1, [])
+        k = 0
+        for j in js:
+            lo, hi = offs[j], offs[j] + nbytes[j]
+            while k < len(prev) and offs[prev[k]] + nbytes[prev[k]] <= lo:
+                k += 1
+            m = k
+            while m < len(prev) and offs[prev[m]] < hi:
+                out[j].append(prev[m])
+                m += 1
+    return out
+
+
+def test_serve_poisson_stream_of_complex_bundles():
+    """A Poisson stream at 2000 requests/s for 2 s: each request a bundle of log-uniform
+    [64, 4096] complex systems (XXII: the re/im halves of a system adjacent, 2 real systems
+    each) of N log-uniform in [32, 2048], matrices resident per N.  Inputs are windows of one
+    pinned pool; every request writes its own region of a pinned output ring (a region is
+    reused only by a request admitted after the earlier one finished: checked from the
+    admission and completion times).  Every request completes; sampled requests whose
+    outputs are never reused match the oracle and give the same bits alone as in the stream."""
+    import torch
+    rng = np.random.default_rng(21)
+    rate, seconds = 2000.0, 2.0
+    n = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    Ns = np.exp(rng.uniform(np.log(32), np.log(2048), n)).astype(np.int64)
+    ncx = np.exp(rng.uniform(np.log(64), np.log(4096), n)).astype(np.int64)
+    mats = {}
+    for N in np.unique(Ns):
+        N = int(N)
+        offV, offT, offM = 0, 5 * N + 2, 8 * N + 6
+        raw = O.raw_matrices(5000 + N, N)
+        mats[N] = (O.pack_fused(N, raw, 1, offV, offT, offM, offM + 2 * N + 2), (offV, offT, offM))
+    pool_t = torch.empty(2 * 4096 * 2048 + 4096, dtype=torch.float64, pin_memory=True)
+    pool = pool_t.numpy()
+    pool[:] = O.uniform(22, -1, 1, pool.size)
+    nbytes = [2 * int(k) * int(N) * 8 for k, N in zip(ncx, Ns)]
+    ring = 6 << 30
+    offs, cyc = ring_offsets(nbytes, ring)
+    ring_t = torch.empty(ring // 8, dtype=torch.float64, pin_memory=True)
+    out_np = ring_t.numpy()
+    reqs, starts = [], []
+    for i in range(n):
+        N, nsys = int(Ns[i]), 2 * int(ncx[i])
+        buf, (offV, offT, offM) = mats[N]
+        start = (i * 7919) % (pool.size - nsys * N)
+        starts.append(start)
+        reqs.append(dict(N=N, nsys=nsys, matrix_key=N, matrices=buf.ctypes.data, offsetV=offV, offsetT=offT,
+                         offsetM=offM, naturalOrder=0, rhs_in=pool.ctypes.data + 8 * start,
+                         rhs_out=ring_t.data_ptr() + int(offs[i]), arrival_s=float(arrivals[i])))
+    print(f"poisson stream: {n} requests, {sum(b // 8 for b in nbytes)} values, {cyc[-1] + 1} ring cycles",
+          flush=True)
+    st, out = pf.serve(reqs, hbm_budget=64 << 30, streams=8, max_batch=64, max_inflight=512, realtime=True)
+    print(f"done: p50 {st.p50_ms:.2f} ms p99 {st.p99_ms:.2f} ms, {st.systems_per_s:.3g} systems/s, "
+          f"{st.launches} launches, mean batch {st.mean_batch:.1f}", flush=True)
+    assert st.requests == n and st.rejected == 0 and st.failed == 0
+    assert all(c == 0 for _, _, c in out)
+    for i, (adm, done, _) in enumerate(out):
+        assert adm >= reqs[i]["arrival_s"] - 1e-6 and done >= adm
+    # no output region was shared by requests in flight together
+    for j, prev in enumerate(ring_overlaps(offs, cyc, nbytes)):
+        for i in prev:
+            assert out[i][1] <= out[j][0], (i, j)
+    # samples among the requests of the last cycle (their outputs are still in the ring)
+    last = [i for i in range(n) if cyc[i] == cyc[-1]]
+    sample = [int(i) for i in rng.choice(last, 12, replace=False)]
+
+    def got_of(i):
+        o = int(offs[i]) // 8
+        return out_np[o:o + reqs[i]["nsys"] * reqs[i]["N"]]
+
+    for i in sample:
+        N, nsys = reqs[i]["N"], reqs[i]["nsys"]
+        buf, (offV, offT, offM) = mats[N]
+        # one complex system (two real systems) at the start and at the end of the bundle
+        for b0 in (0, nsys - 2):
+            x = pool[starts[i] + b0 * N: starts[i] + (b0 + 2) * N].copy()
+            p = O.problem(jw_type=14, N=N, B=2, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                          cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+            want = O.jw_f128(p, buf, x)
+            for b in range(2):
+                got = got_of(i)[(b0 + b) * N:(b0 + b + 1) * N]
+                assert O.relerr_norm(got, want[b * N:(b + 1) * N]) <= TOL64, (i, N, b0 + b)
+    # alone against the stream, bit for bit
+    for i in sample[:4]:
+        mixed = got_of(i).copy()
+        got_of(i)[:] = np.nan
+        st1, out1 = pf.serve([dict(reqs[i], arrival_s=0.0)], hbm_budget=8 << 30, streams=1, max_batch=1)
+        assert st1.requests == 1 and out1[0][2] == 0
+        np.testing.assert_array_equal(got_of(i), mixed, err_msg=f"request {i} alone vs in the stream")
