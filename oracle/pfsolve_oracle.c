@@ -553,6 +553,27 @@ int pfo_jw_step_f64_range(const pfo_jw_problem* p, const double* buffer, double*
 #endif
 }
 
+int pfo_jw_step_dd_range(const pfo_jw_problem* p, const pfo_dd* buffer, pfo_dd* out, int64_t b0, int64_t b1,
+                         int threads)
+{
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+    int err = 0;
+    int64_t chunk = (b1 - b0 + nt - 1) / nt;
+#pragma omp parallel num_threads(nt) reduction(| : err)
+    {
+        int tid = omp_get_thread_num();
+        int64_t s = b0 + (int64_t)tid * chunk;
+        int64_t e = s + chunk < b1 ? s + chunk : b1;
+        if (s < e) err |= jw_step_dd(p, buffer, out, s, e);
+    }
+    return err;
+#else
+    (void)threads;
+    return jw_step_dd(p, buffer, out, b0, b1);
+#endif
+}
+
 int pfo_num_threads(void)
 {
 #ifdef _OPENMP

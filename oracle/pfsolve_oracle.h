@@ -173,6 +173,8 @@ void pfo_dense_solve_f128(int64_t N, const double* A /*row-major NxN*/, const do
 /* Runs pfo_jw_step_f64 on systems [b0,b1) only; threads = OpenMP threads (0 = all). */
 int pfo_jw_step_f64_range(const pfo_jw_problem* p, const double* buffer, double* out,
                           int64_t b0, int64_t b1, int threads);
+int pfo_jw_step_dd_range(const pfo_jw_problem* p, const pfo_dd* buffer, pfo_dd* out, int64_t b0, int64_t b1,
+                         int threads);
 int pfo_num_threads(void);
 
 #ifdef __cplusplus

@@ -166,30 +166,33 @@ constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v / 2); }
 // ceil(n/R) rows for j < n % R and floor(n/R) after.
 __device__ __forceinline__ int64_t reorder_pos(int64_t i, int64_t N, int64_t Nalign = 0, int64_t Msplit0 = 704)
 {
-    // rows per thread of the register layout: from Nalign (the largest system of a
-    // chained step) or the block size, the split of the stored rows over them: from N
-    int64_t blk = N;
-    int64_t NA = Nalign > 0 ? Nalign : N;
-    if (N > 3840) {
-        blk = Msplit0 > 0 ? Msplit0 : 704;
+    // 32-bit arithmetic (systems have fewer than 2^31 rows): the divisions are the cost of
+    // this function in the table builds
+    const uint32_t ui = (uint32_t)i, uN = (uint32_t)N;
+    uint32_t blk = uN;
+    uint32_t NA = Nalign > 0 ? (uint32_t)Nalign : uN;
+    if (uN > 3840u) {
+        blk = Msplit0 > 0 ? (uint32_t)Msplit0 : 704u;
         NA = blk;
     }
-    int64_t ws = ((NA + 1023) / 1024) * 64;
-    if (NA < 512) ws = 64;
-    int64_t R = (NA + ws - 1) / ws;
-    if ((R & 1) == 0) R++;
-    const int64_t base = (i / blk) * blk;
-    const int64_t n = base + blk > N ? N - base : blk;
-    const int64_t li = i - base;
-    const int64_t ceilq = (n + R - 1) / R;
-    const int64_t k = li % R;
-    const int64_t rem = n % R;
-    int64_t ret = li / R;
+    // rows per thread of the register layout: from Nalign (the largest system of a
+    // chained step) or the block size, the split of the stored rows over them: from N
+    uint32_t ws = ((NA + 1023u) / 1024u) * 64u;
+    if (NA < 512u) ws = 64u;
+    uint32_t R = (NA + ws - 1u) / ws;
+    if ((R & 1u) == 0u) R++;
+    const uint32_t base = blk == uN ? 0u : (ui / blk) * blk;
+    const uint32_t n = base + blk > uN ? uN - base : blk;
+    const uint32_t li = ui - base;
+    const uint32_t ceilq = (n + R - 1u) / R;
+    const uint32_t q = li / R, k = li - q * R;
+    const uint32_t nq = n / R, rem = n - nq * R;
+    uint32_t ret = q;
     if (k < rem)
         ret += k * ceilq;
     else
-        ret += rem * ceilq + (k - rem) * (n / R);
-    return base + ret;
+        ret += rem * ceilq + (k - rem) * nq;
+    return (int64_t)(base + ret);
 }
 
 

@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
         L.pfo_jw_step_dd.argtypes = [C.POINTER(JwProblem), ddptr, ddptr]
         L.pfo_jw_step_f128_dd.argtypes = [C.POINTER(JwProblem), ddptr, ddptr]
         L.pfo_jw_step_f64_range.argtypes = [C.POINTER(JwProblem), dptr, dptr, i64, i64, C.c_int]
+        L.pfo_jw_step_dd_range.argtypes = [C.POINTER(JwProblem), ddptr, ddptr, i64, i64, C.c_int]
         L.pfo_num_threads.restype = C.c_int
         L.pfo_dgbmv_f64.argtypes = [C.POINTER(JwProblem), C.c_int, C.c_int, C.c_int, dptr, dptr]
         L.pfo_dgbmv_f128.argtypes = [C.POINTER(JwProblem), C.c_int, C.c_int, C.c_int, dptr, dptr]
