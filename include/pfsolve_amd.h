@@ -131,7 +131,8 @@ typedef struct pfsolve_batch_tuning {
     int32_t kernel[PFSOLVE_BATCH_CLASSES];      /* 0: default, 1: every wavefront loads, computes and
                                                    stores its pairs, 2: compute / store split
                                                    (R <= 16; `waves` counts the computing
-                                                   wavefronts) */
+                                                   wavefronts), 3: direct I/O (R <= 16: the
+                                                   table alone in LDS, up to 8 wavefronts) */
     int32_t order;            /* 0: largest N first, 1: sets as given */
     int32_t pad;
 } pfsolve_batch_tuning;

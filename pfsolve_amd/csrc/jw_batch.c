@@ -211,13 +211,18 @@ static int group_waves_default(int g)
     return jwdev_main64_waves(2 << g);
 }
 
-/* kernel of launch group g: 1 = load, compute and store in every wavefront,
- * 2 = compute / store split (R <= 16).  Default: the split where it is
+/* kernel of launch group g: 1 = load, compute and store in every wavefront
+ * (input through an LDS staging), 2 = compute / store split (R <= 16), 3 = direct
+ * I/O (R <= 16: LDS holds the table only, loads one pass ahead, stores from
+ * registers, up to 8 wavefronts on one table).  Default: the split where it is
  * measured faster (R <= 8: 5.35 against 4.94 TB/s at N = 512); at R = 16 the
- * table leaves LDS for 4-5 computing wavefronts and both reach the same rate. */
+ * staged kernels reach the same rate with the 4-5 wavefronts LDS leaves them, and
+ * direct I/O measured slower (harmonic batch, solve only: 256-295 us against
+ * 213-227 us: a lane's 16-byte accesses 128 bytes apart reach memory less well than
+ * the staged, contiguous ones), so it is a tuner candidate only. */
 static int group_kernel_default(int g) { return g <= 2 ? 2 : 1; }
 
-static int group_kernel_ok(int g, int k) { return k == 1 || (k == 2 && g <= 3); }
+static int group_kernel_ok(int g, int k) { return k == 1 || ((k == 2 || k == 3) && g <= 3); }
 
 typedef struct n_index {
     int64_t N, idx;
@@ -297,7 +302,10 @@ static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
         }
         if (!group_kernel_ok(g, k)) k = group_kernel_default(g);
         int occ = 0;
-        if (k == 2) {
+        if (k == 3) {
+            if (w <= 0 || w > 8) w = 8;
+            occ = jwdev_batch64d_occupancy(2 << g, w);
+        } else if (k == 2) {
             int nc = 0, ns = 0;
             if (!jwdev_batch64s_shape(2 << g, w, &nc, &ns)) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
             w = nc;
@@ -482,12 +490,14 @@ static PfSolveResult launch_step(pfsolve_batch* b, const jw_batch_item* d_items,
         return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     for (int g = 0; g < NCLS; g++) {
         if (!b->unit_count[g] || (only_group >= 0 && g != only_group)) continue;
-        const int rc = b->group_kernel[g] == 2
-                           ? jwdev_batch64s(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
-                                            b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g],
-                                            b->d_counters + NCLS, b->wait_polls, stream)
-                           : jwdev_batch64(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
-                                           b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g], stream);
+        const int k = b->group_kernel[g];
+        const int rc = k == 2 ? jwdev_batch64s(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
+                                               b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g],
+                                               b->d_counters + NCLS, b->wait_polls, stream)
+                       : k == 3 ? jwdev_batch64d(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
+                                                 b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g], stream)
+                                : jwdev_batch64(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
+                                                b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g], stream);
         if (rc != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     }
     return PFSOLVE_SUCCESS;
@@ -707,7 +717,7 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
             const int shifts = c < NSHIFT ? 2 : 1;
             for (int sh = 0; sh < shifts && res == PFSOLVE_SUCCESS; sh++) {
                 const int g = c + sh;
-                for (int k = 1; k <= 2 && res == PFSOLVE_SUCCESS; k++) {
+                for (int k = 1; k <= 3 && res == PFSOLVE_SUCCESS; k++) {
                     if (!group_kernel_ok(g, k)) continue;
                     /* a few wave counts: for the one-wavefront kernel the largest, its half and
                      * quarter; for the split kernel (computing wavefronts) the default, its half
@@ -718,6 +728,10 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
                         wl[0] = wmax;
                         wl[1] = (wmax + 1) / 2;
                         wl[2] = (wmax + 3) / 4;
+                    } else if (k == 3) {
+                        wl[0] = 8;
+                        wl[1] = 4;
+                        wl[2] = 2;
                     } else {
                         int nc = 0, ns = 0;
                         if (jwdev_batch64s_shape(2 << g, 0, &nc, &ns)) wl[0] = nc;
@@ -727,8 +741,9 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
                     for (int wi = 0; wi < 3 && res == PFSOLVE_SUCCESS; wi++) {
                         const int w = wl[wi];
                         if (w < 1 || (wi > 0 && w == wl[0]) || (wi == 2 && w == wl[1])) continue;
-                        const int occ = k == 2 ? jwdev_batch64s_occupancy(2 << g, w)
-                                               : jwdev_batch64_occupancy(2 << g, w);
+                        const int occ = k == 2   ? jwdev_batch64s_occupancy(2 << g, w)
+                                        : k == 3 ? jwdev_batch64d_occupancy(2 << g, w)
+                                                 : jwdev_batch64_occupancy(2 << g, w);
                         for (int gpc = 1; gpc <= occ && gpc <= 2; gpc++) {
                             for (size_t ui = 0; ui < sizeof(units_try) / sizeof(units_try[0]); ui++) {
                                 pfsolve_batch_tuning t = cur;

@@ -205,6 +205,11 @@ int jwdev_batch64_occupancy(int R, int waves);
 int jwdev_zero_counters(int* d_counters, int n, void* stream);
 int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
                   int waves, int grid, void* stream);
+/* the same queue with direct I/O (R <= 16): LDS holds the table only, every wavefront
+ * loads one pass ahead and stores from registers; up to 8 wavefronts per workgroup */
+int jwdev_batch64d_occupancy(int R, int waves);
+int jwdev_batch64d(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
+                   int waves, int grid, void* stream);
 /* the same queue with the compute / store split (R <= 16): `ncomp` compute
  * wavefronts (0: default); *d_err is raised if a bounded wait ran out (after
  * `wait_polls` polls: 0 the default, negative none) */

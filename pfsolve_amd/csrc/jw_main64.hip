@@ -1081,6 +1081,96 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_batch64_kernel(const jw_b
     }
 }
 
+// 16-byte stores of a pair straight from the registers (lane t: chunks t H .. t H + H-1
+// of both systems, the output keep-range's full chunks), the odd range ends by their
+// owning lane
+template <int R>
+__device__ __forceinline__ void store_pair_direct(int lane, const jw_pass64& g, int64_t sys0, const double (&x0)[R],
+                                                  const double (&x1)[R])
+{
+    constexpr int H = R / 2;
+    const bool ok1 = sys0 + 1 < g.nsys;
+    double2* dst0 = reinterpret_cast<double2*>(g.rhs) + ((sys0 * g.stride + g.offSol) >> 1);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (s == 1 && !ok1) continue;
+        double2* dst = dst0 + s * (g.stride >> 1);
+        const double(&x)[R] = s == 0 ? x0 : x1;
+#pragma unroll
+        for (int m = 0; m < H; m++) {
+            const int c = lane * H + m;
+            // cached stores: the H stores of a lane each fill part of 64 lines that the
+            // sibling stores complete, and L2 merges them before the write-back
+            if (c >= g.ocl && c < g.och) dst[c] = make_double2(x[2 * m], x[2 * m + 1]);
+        }
+    }
+    if (g.out_edge_lo || g.out_edge_hi) {
+        double* base = g.rhs + sys0 * g.stride + g.offSol;
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const bool on = e == 0 ? g.out_edge_lo : g.out_edge_hi;
+            const int r = e == 0 ? g.out_lo : g.out_hi - 1;
+            if (on && lane == r / R) {
+#pragma unroll
+                for (int k = 0; k < R; k++)
+                    if (k == r % R) {
+                        base[r] = x0[k];
+                        if (ok1) base[g.stride + r] = x1[k];
+                    }
+            }
+        }
+    }
+}
+
+// The batch queue with direct I/O: LDS holds the set's table only (no staging), so more
+// wavefronts share one table; every wavefront loads its pairs straight into registers
+// one pass ahead, computes and stores straight from registers (nontemporal 16-byte
+// stores).  Units and the table hand-over as in jw_batch64_kernel.
+constexpr int kBatchDirectThreads = 512;
+template <int R>
+__global__ __launch_bounds__(kBatchDirectThreads) void jw_batch64d_kernel(const jw_batch_item* items,
+                                                                          const jw_batch_unit* units, int nunits,
+                                                                          int* counter)
+{
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_unit;
+    const double* tab = reinterpret_cast<const double*>(smem);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    int cur = -1;
+    for (;;) {
+        __syncthreads();   // the previous unit is finished: s_unit and the table may change
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (u >= nunits) break;
+        const jw_batch_unit un = units[u];
+        const jw_batch_item& it = items[un.item];
+        const jw_pass64 g = make_pass64(reinterpret_cast<double*>(it.rhs), it.nsys, it.stride, it.offSol, 0, it.N, 0,
+                                        it.N, it.kl, it.ku, true, true, true, it.apply_scale != 0, it.scale, lane);
+        int64_t q = un.pair0 + wave;
+        // the first pair's loads go out before the table copy
+        double2 pf[2][R / 2];
+        if (q < un.pair1) load_pair_direct<R>(pf, lane, g, 2 * q);
+        if (un.item != cur) {
+            copy_table_lds<NTAB>(smem, it.table, wave, nwaves, lane);
+            __syncthreads();
+            cur = un.item;
+        }
+        while (q < un.pair1) {
+            const int64_t qn = q + nwaves;
+            double x0[R], x1[R];
+            unpack_pair<R>(pf, lane, g, 2 * q, x0, x1);
+            if (qn < un.pair1) load_pair_direct<R>(pf, lane, g, 2 * qn);
+            compute_pair<R, true>(tab, lane, g, x0, x1);
+            store_pair_direct<R>(lane, g, 2 * q, x0, x1);
+            q = qn;
+        }
+    }
+}
+
 // The same queue for long systems (R = 32): one system per wavefront pass, table
 // read through a buffer resource; the unit ranges count systems, not pairs.
 __global__ __launch_bounds__(256) void jw_batch64g_kernel(const jw_batch_item* items, const jw_batch_unit* units,
@@ -1476,6 +1566,46 @@ int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, in
     if (waves < 1 || waves > max_waves64(R) || grid < 1) { snprintf(g_err64, sizeof(g_err64), "bad batch launch"); return -1; }
     const size_t lds = (size_t)jwdev_main64_lds_bytes(R, waves);
     if (lds > (size_t)kLdsBytesMax) { snprintf(g_err64, sizeof(g_err64), "batch LDS %zu too large", lds); return -1; }
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return -2;
+    void* args[] = {const_cast<jw_batch_item**>(&d_items), const_cast<jw_batch_unit**>(&d_units), &nunits,
+                    &d_counter};
+    e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? 0 : -3;
+}
+
+static const void* pick_batch64d(int R)
+{
+    switch (R) {
+    case 2: return reinterpret_cast<const void*>(&jw_batch64d_kernel<2>);
+    case 4: return reinterpret_cast<const void*>(&jw_batch64d_kernel<4>);
+    case 8: return reinterpret_cast<const void*>(&jw_batch64d_kernel<8>);
+    case 16: return reinterpret_cast<const void*>(&jw_batch64d_kernel<16>);
+    default: return nullptr;
+    }
+}
+
+int jwdev_batch64d_occupancy(int R, int waves)
+{
+    const void* f = pick_batch64d(R);
+    if (!f || waves < 1 || 64 * waves > kBatchDirectThreads) return 0;
+    const int lds = JW_TAB64_ELEMS(R) * 8;
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * waves, lds) != hipSuccess) return 0;
+    return nb;
+}
+
+int jwdev_batch64d(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
+                   int waves, int grid, void* stream)
+{
+    const void* f = pick_batch64d(R);
+    if (!f) { snprintf(g_err64, sizeof(g_err64), "no direct batch kernel for R=%d", R); return -1; }
+    if (waves < 1 || 64 * waves > kBatchDirectThreads || grid < 1) {
+        snprintf(g_err64, sizeof(g_err64), "bad direct batch launch");
+        return -1;
+    }
+    const size_t lds = (size_t)JW_TAB64_ELEMS(R) * 8;
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -2;
     void* args[] = {const_cast<jw_batch_item**>(&d_items), const_cast<jw_batch_unit**>(&d_units), &nunits,

@@ -163,6 +163,10 @@ def tuning(**kw):
     dict(kernel=2, waves=64, order=1),
     dict(kernel=[1, 2, 1, 2, 1], rows_per_lane_shift=[1, 1, 0, 0, 0]),
     dict(kernel=[2, 2, 2, 2, 1], waves=5, unit_pairs=2),
+    dict(kernel=3),
+    dict(kernel=[3, 3, 3, 3, 1], waves=4, unit_pairs=2),
+    dict(kernel=3, waves=2, order=1, unit_pairs=1),
+    dict(kernel=[3, 1, 3, 2, 1], rows_per_lane_shift=[1, 0, 1, 0, 0]),
 ])
 def test_batch_tunings(kw):
     """every tuning gives the oracle's results; without a rows-per-lane change the
@@ -234,7 +238,7 @@ def test_batch_tuner_keeps_caller_data_and_results():
     d.check(got)
     for c in range(_abi.BATCH_CLASSES):
         assert best.rows_per_lane_shift[c] in (0, 1)
-        assert best.kernel[c] in (0, 1, 2)
+        assert best.kernel[c] in (0, 1, 2, 3)
         assert best.waves[c] >= 0
 
 
