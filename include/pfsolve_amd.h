@@ -204,7 +204,8 @@ void pfsolve_batch_destroy(pfsolve_batch* b);
  * start, wall clock); realtime = 0 treats every request as arrived at the start.
  * Host buffers should be pinned for the copies to overlap.  A request that cannot
  * fit the budget even alone is rejected (status PFSOLVE_ERROR_FAILED_TO_ALLOCATE),
- * as is a key reused with another N.  A launch that fails ends its requests with
+ * as is a key reused with another N (PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH) or with
+ * another band (KL, KU), matrix offsets or storage order (PFSOLVE_ERROR_UNSUPPORTED_RADIX).  A launch that fails ends its requests with
  * the failure's status and the loop goes on with the others (fail_launch = k > 0
  * makes the k-th launch fail before its kernels, to test that path).  dry_run = n
  * > 0 runs the same policy without a device: host memory for the arena, copies

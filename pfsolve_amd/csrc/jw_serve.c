@@ -625,8 +625,15 @@ PfSolveResult pfsolve_serve(const pfsolve_serve_config* cfg, pfsolve_serve_reque
             const int64_t mbytes = (matrix_extent(q) * 8 + ALIGN - 1) / ALIGN * ALIGN;
             const int64_t kbytes = mbytes + jwdev_table64_elems(R) * 8;
             int status = PFSOLVE_SUCCESS;
+            jw_source_desc qs;
+            fused_source(q, &qs);
             if (v >= 0 && S.K.pool[v].N != q->N) {
                 status = PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;   /* one key, one size */
+            } else if (v >= 0 && (S.K.pool[v].src.KL != qs.KL || S.K.pool[v].src.KU != qs.KU ||
+                                  S.K.pool[v].src.offV != qs.offV || S.K.pool[v].src.ldS != qs.ldS ||
+                                  S.K.pool[v].src.mdM != qs.mdM || S.K.pool[v].src.permuted != qs.permuted)) {
+                /* the key's table was built for another band, layout or storage order */
+                status = PFSOLVE_ERROR_UNSUPPORTED_RADIX;
             } else if (r->bytes + (v >= 0 ? 0 : kbytes) > S.A.size) {
                 status = PFSOLVE_ERROR_FAILED_TO_ALLOCATE;      /* can never fit: rejected, not waited for */
             }

@@ -153,3 +153,18 @@ def test_budget_exhaustion_queues_instead_of_failing():
     w = DryWorkload([(2048, 20, i % 2) for i in range(12)], seed=5)   # 330 KB each
     st, out = pf.serve(w.reqs, hbm_budget=1 << 20, streams=2, max_batch=1, max_inflight=16, dry_run=10)
     assert st.requests == 12 and st.rejected == 0 and st.peak_bytes <= 1 << 20
+
+
+def test_key_reused_with_another_layout_is_refused():
+    """a key's table is built for one band, set of matrix offsets and storage order: a later
+    request of the key with another KL/KU, offsets or naturalOrder is refused, the others run"""
+    w = DryWorkload([(100, 2, 7)] * 6, seed=6)
+    w.reqs[1]["KL"] = 1
+    w.reqs[2]["offsetT"] = w.reqs[2]["offsetT"] + 1
+    w.reqs[3]["naturalOrder"] = 1
+    w.reqs[4]["KU"] = 1
+    st, out = pf.serve(w.reqs, hbm_budget=8 << 20, streams=2, max_batch=2, dry_run=2)
+    codes = [c for _, _, c in out]
+    assert codes[0] == 0 and codes[5] == 0
+    assert codes[1:5] == [_abi.PFSOLVE_ERROR_UNSUPPORTED_RADIX] * 4
+    assert st.rejected == 4
