@@ -478,8 +478,9 @@ def gen_fused_matrices_dd(N: int, seed: int, reorder):
     return buf, (offV, offT, offM)
 
 
-def bench_dd(torch, pf, steps: int, warmup: int, N: int = 256, B: int = 16384) -> dict:
-    """Double-double fused step, N = 256, 16384 real systems, DD storage (16 B per value)."""
+def bench_dd(torch, pf, steps: int, warmup: int, N: int = 256, B: int = 16384, variant: int = 0) -> dict:
+    """Double-double fused step, N = 256, 16384 real systems, DD storage (16 B per value);
+    variant 0 the wave-per-system double-double kernel, 1 the general kernel."""
     lib = pf.lib()
     buf, (offV, offT, offM) = gen_fused_matrices_dd(N, 7, lib.pfsolve_reorder_i0)
     dmat = torch.from_numpy(buf.ravel()).cuda()
@@ -489,7 +490,8 @@ def bench_dd(torch, pf, steps: int, warmup: int, N: int = 256, B: int = 16384) -
     x = torch.stack([xh, xl], -1).contiguous()
     stream = torch.cuda.Stream()
     plan = pf.Plan(dmat.data_ptr(), x.data_ptr(), stream=stream.cuda_stream, size=[N, B, 1], jw_type=14,
-                   quadDoubleDoublePrecision=1, scaleC=1.0, offsetV=offV, offsetT=offT, offsetM=offM)
+                   quadDoubleDoublePrecision=1, scaleC=1.0, offsetV=offV, offsetT=offT, offsetM=offM,
+                   kernelVariant=variant)
     geo = plan.geometry()
     for _ in range(warmup):
         plan.append()
@@ -519,7 +521,8 @@ def bench_dd(torch, pf, steps: int, warmup: int, N: int = 256, B: int = 16384) -
         "workload": f"double-double fused step, N={N}, {B} real systems, DD storage, matrices FP64 draws "
                     "with low words U(-1,1) 2^-60 |hi|",
         "value": B / (ms * 1e-3), "unit": "systems/s", "ms_per_step": ms,
-        "kernel": f"jw_main_kernel<dd,{geo.rows_per_lane},{geo.lanes}>", "kernel_ms": kms,
+        "kernel": (f"jw_dd64_kernel<{geo.fast_rows_per_lane}>" if geo.fast_rows_per_lane
+                   else f"jw_main_kernel<dd,{geo.rows_per_lane},{geo.lanes}>"), "kernel_ms": kms,
         "algorithmic_bytes": alg, "hbm_frac": alg / (kms * 1e-3) / HBM_PEAK,
     }
 
@@ -1011,6 +1014,7 @@ def main() -> int:
         result["batch"] = bench_batch(torch, pf, max(10, K), max(3, args.warmup))
     if not args.no_dd:
         result["dd"] = bench_dd(torch, pf, max(5, K // 2), 2)
+        result["dd_general_kernel"] = bench_dd(torch, pf, max(5, K // 2), 2, variant=1)
     if not args.no_fused_blocks:
         result["fused_blocks"] = bench_fused_blocks(torch, pf, max(10, K // 2), 3)
     if not args.no_chain:

@@ -43,7 +43,7 @@ typedef struct pfsolve_geometry {
     int32_t precision;        /* 1 FP64, 2 double-double, 3 double-double with FP64 memory */
     int64_t table_bytes;      /* coefficient table per matrix set */
     int64_t lds_bytes;        /* LDS per workgroup */
-    int32_t fast_rows_per_lane;   /* wave-per-system FP64 kernel: R (0 = not used) */
+    int32_t fast_rows_per_lane;   /* wave-per-system kernel (FP64, or double-double N <= 512): R (0 = not used) */
     int32_t fast_waves;           /* ... and its computing waves per workgroup */
     int32_t fast_store_waves;     /* ... storing waves per workgroup (0: each wave stores its own) */
     int32_t pad;

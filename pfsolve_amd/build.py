@@ -22,7 +22,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 C_SOURCES = ["jw_plan.c", "pfsolve_api.c", "jw_batch.c", "jw_serve.c"]
-HIP_SOURCES = ["jw_device.hip", "jw_main64.hip", "jw_long.hip"]
+HIP_SOURCES = ["jw_device.hip", "jw_main64.hip", "jw_long.hip", "jw_dd64.hip"]
 
 C_FLAGS = [
     "-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter",

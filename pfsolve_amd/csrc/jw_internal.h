@@ -219,6 +219,12 @@ int jwdev_batch64s_occupancy(int R, int ncomp);
 int jwdev_batch64s(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
                    int ncomp, int grid, int* d_err, int wait_polls, void* stream);
 int jwdev_main(const jw_run_desc* r, int L, int R, int precision, int waves, int grid, void* stream);
+/* wave-per-system double-double kernel (table format 2 in double-double, 64 lanes, R in
+ * {2,4,8}: N <= 512); DD or FP64 storage (mem_f64), any alignment; `waves` per workgroup
+ * (at most 8) */
+int64_t jwdev_dd64_table_elems(int R);
+int jwdev_dd64_occupancy(int R, int waves);
+int jwdev_dd64(const jw_run_desc* r, int R, int waves, int grid, void* stream);
 /* chained steps: the K tables (one launch), then the kernel that keeps each pair of systems
  * in registers through the K iterations (FP64, 16-byte aligned pairs, R in {2,4,8,16}) */
 int jwdev_prologue_chain(const jw_chain_jobs* jobs, void* stream);

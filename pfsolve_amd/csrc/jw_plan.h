@@ -32,6 +32,9 @@ struct pfsolve_plan {
     int* d_err;
     int wait_polls;           /* jw_run_desc.wait_polls of the split kernel's launches */
     hipEvent_t ev_start, ev_stop;   /* optional timing events around the main kernel */
+    /* wave-per-system double-double kernel (table format 2 in double-double): rows per
+     * lane (0 = not used), waves per workgroup, workgroups per CU */
+    int dd64_R, dd64_waves, dd64_occ;
     /* chained steps: iterations (1 = none), rows per lane of the chained kernel, its
      * workgroups per CU; a slot's table area holds K format-2 tables of chain_tab bytes */
     int chain_K, chain_R, chain_occ;

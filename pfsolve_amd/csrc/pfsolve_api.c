@@ -233,7 +233,21 @@ static PfSolveResult plan_setup(PfSolveApplication* app, struct pfsolve_plan* p)
                 if (b2 > p->table_bytes) p->table_bytes = b2;
             }
         }
-        if (cfg->kernelVariant == 2 && !p->R64) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
+        /* wave-per-system double-double path: 64 lanes, R a power of two in [2, 8] */
+        if (p->precision != JW_PREC_F64 && cfg->kernelVariant != 1 && !long_n) {
+            int r = 2;
+            while (r * 64 < p->N) r *= 2;
+            if (r <= 8) {
+                p->dd64_waves = 8;
+                p->dd64_occ = jwdev_dd64_occupancy(r, p->dd64_waves);
+                if (p->dd64_occ >= 1) {
+                    p->dd64_R = r;
+                    size_t b2 = (size_t)jwdev_dd64_table_elems(r) * 16;
+                    if (b2 > p->table_bytes) p->table_bytes = b2;
+                }
+            }
+        }
+        if (cfg->kernelVariant == 2 && !p->R64 && !p->dd64_R) return PFSOLVE_ERROR_UNSUPPORTED_FFT_LENGTH;
         /* compute / store split form of the wave-per-system kernel (R <= 8) */
         if (p->R64 && p->R64 <= 8) {
             int c = 0, st = 0;
@@ -638,6 +652,21 @@ static PfSolveResult append_impl(PfSolveApplication* app, const PfSolveLaunchPar
             if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
             return PFSOLVE_SUCCESS;
         }
+        if (p->dd64_R) {
+            /* double-double wave-per-system kernel: a format-2 table in double-double */
+            if (nsys <= 0) return PFSOLVE_SUCCESS;
+            if (jwdev_prologue(&src, cfg->buffer[0], table, 64, p->dd64_R, p->precision, 2, s) != 0)
+                return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            const int64_t npass = (nsys + 1) / 2;
+            int64_t need = (npass + p->dd64_waves - 1) / p->dd64_waves;
+            int64_t grid = (int64_t)p->cu_count * p->dd64_occ;
+            if (grid > need) grid = need;
+            if (grid < 1) grid = 1;
+            if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+            if (jwdev_dd64(&run, p->dd64_R, p->dd64_waves, (int)grid, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+            return PFSOLVE_SUCCESS;
+        }
         /* 1024 < N <= 2048 without 16-byte aligned systems (or with the general kernel
          * asked for): in chunks */
         if (p->L == 0) return append_long(app, &src, &run, s, table);
@@ -999,6 +1028,10 @@ PfSolveResult pfsolve_plan_geometry(const PfSolveApplication* app, pfsolve_geome
     }
     g->fast_rows_per_lane = p->R64;
     g->fast_waves = p->split64 ? p->split64 : p->waves64;
+    if (p->dd64_R) {
+        g->fast_rows_per_lane = p->dd64_R;
+        g->fast_waves = p->dd64_waves;
+    }
     if (p->split64) {
         int c = 0, st = 0;
         if (jwdev_main64s_shape(p->R64, &c, &st)) g->fast_store_waves = st;

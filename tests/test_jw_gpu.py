@@ -251,12 +251,14 @@ def dd_inputs(N, B, seed):
     return buf, x, (offV, offT, offM)
 
 
-@pytest.mark.parametrize("N", [1, 7, 64, 100, 256, 512])
-def test_double_double_fused(N):
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("N", [1, 7, 64, 100, 128, 129, 256, 257, 512])
+def test_double_double_fused(N, variant):
+    """variant 0: the wave-per-system double-double kernel (N <= 512); 1: the general kernel"""
     B = 64 if N == 256 else 6
     buf, x, (offV, offT, offM) = dd_inputs(N, B, 31 + N)
     cfg = dict(size=[N, B, 1], jw_type=14, quadDoubleDoublePrecision=1, scaleC=1.0,
-               offsetV=offV, offsetT=offT, offsetM=offM)
+               offsetV=offV, offsetT=offT, offsetM=offM, kernelVariant=variant)
     got = run_plan(cfg, buf.ravel(), x.ravel()).reshape(-1, 2)
     p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
                   cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
@@ -266,30 +268,111 @@ def test_double_double_fused(N):
         assert err <= 1e-30, (b, err)
 
 
-def test_double_double_reference_mode_tridiagonal():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_double_double_reference_mode_tridiagonal(variant):
     N, B = 300, 4
     buf, x, _ = dd_inputs(N, B, 5)
     # jw 13 / ub 2 reads [ld | md | ud] at [0, N, 2N): reuse the T block of the fused packing
     tri = np.ascontiguousarray(buf[5 * N:8 * N])
-    cfg = dict(size=[N, B, 1], jw_type=13, upperBanded=2, quadDoubleDoublePrecision=1, scaleC=1.0)
+    cfg = dict(size=[N, B, 1], jw_type=13, upperBanded=2, quadDoubleDoublePrecision=1, scaleC=1.0,
+               kernelVariant=variant)
     got = run_plan(cfg, tri.ravel(), x.ravel()).reshape(-1, 2)
     p = O.problem(jw_type=13, upperBanded=2, N=N, B=B, cfg_outStride=N, cfg_scaleC=1.0)
     want = O.jw_f128_dd(p, tri, x)
     assert O.dd_relerr_norm(got, want) <= 1e-30
 
 
-def test_double_double_compute_fp64_memory():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_double_double_compute_fp64_memory(variant):
     N, B = 200, 8
     raw, buf, (offV, offT, offM) = fused_buffers(N, 6, 1)
     x = O.uniform(6, -1, 1, B * N)
     cfg = dict(size=[N, B, 1], jw_type=14, quadDoubleDoublePrecisionDoubleMemory=1, scaleC=1.0,
-               offsetV=offV, offsetT=offT, offsetM=offM)
+               offsetV=offV, offsetT=offT, offsetM=offM, kernelVariant=variant)
     got = run_plan(cfg, buf, x)
     p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
                   cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
     want = O.jw_f128(p, buf, x)
     # DD arithmetic, result rounded to FP64: within one rounding of the exact result
     assert O.relerr_norm(got, want) <= 2.3e-16
+
+
+def promote_dd(a, seed):
+    """FP64 values as double-double with a low word of U(-1,1) 2^-60 |hi|"""
+    lo = O.uniform(seed, -1, 1, a.size) * 2.0 ** -60 * np.abs(a)
+    return np.ascontiguousarray(np.stack([a, lo], -1))
+
+
+def check_systems_dd(got, want, x, nsys, stride, off, N, out_lo, out_hi, tol):
+    """DD arrays (n, 2): rows of the output keep-range within tol of the oracle (relative
+    norm per system), every other element untouched"""
+    keep = np.zeros(got.shape[0], bool)
+    for b in range(nsys):
+        base = b * stride + off
+        if out_hi > out_lo:
+            err = O.dd_relerr_norm(got[base + out_lo:base + out_hi], want[base + out_lo:base + out_hi])
+            assert err <= tol, (b, err)
+        keep[base + out_lo:base + out_hi] = True
+    np.testing.assert_array_equal(got[~keep], x[~keep])
+
+
+@pytest.mark.parametrize("jw_type,ub", MODES)
+@pytest.mark.parametrize("N", [1, 2, 3, 17, 64, 65, 127, 200, 300, 512])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_double_double_reference_modes(jw_type, ub, N, variant):
+    """every reference mode in double-double with runtime offsets, keep ranges, output
+    stride and scaleC, on the wave-per-system double-double kernel (0) and the general
+    kernel (1)"""
+    jw = jw_type % 10
+    B = 5
+    offM, offV = 13, 3 * N + 29
+    total = max(3 * N, offV + N, offM + 2 * N) + 8
+    buf = promote_dd(mode_buffer(jw, ub, N, 1, offM, total, 1000 + N + jw), 3000 + N + jw)
+    S, offS = N + 7, 5
+    izp, ozp = (N // 4, N), (1, N - N // 8)
+    x = promote_dd(O.uniform(77 + N, -1, 1, B * S + offS + 9), 78 + N)
+    cfg = dict(size=[N, B, 1], jw_type=jw_type, upperBanded=ub, quadDoubleDoublePrecision=1, scaleC=1.0,
+               outputBufferStride=[N, 0, 0], jw_control_bitmask=RT_ALL, kernelVariant=variant)
+    launch = dict(offsetM=offM, offsetV=offV, offsetSolution=offS, inputZeropad=izp, outputZeropad=ozp,
+                  scaleC=0.75, outputBufferStride=S)
+    got = run_plan(cfg, buf.ravel(), x.ravel(), launch).reshape(-1, 2)
+    p = O.problem(jw_type=jw_type, upperBanded=ub, N=N, B=B, mask=RT_ALL, cfg_outStride=N,
+                  cfg_scaleC=1.0, run_offsetM=offM, run_offsetV=offV, run_offsetSolution=offS,
+                  run_izp=list(izp), run_ozp=list(ozp), run_scaleC=0.75, run_outStride=S)
+    want = O.jw_f128_dd(p, buf, x)
+    lo, hi = ozp[0], min(ozp[1], N)
+    check_systems_dd(got, want, x, B, S, offS, N, lo, max(lo, hi), 1e-30)
+
+
+@pytest.mark.parametrize("N", [7, 200])
+def test_double_double_large_odd_batch(N):
+    """more pairs of systems than one sweep of the grid holds (the wavefronts loop), odd
+    batch: the last pass has one system"""
+    B = 20001 if N == 7 else 4097
+    buf, x, (offV, offT, offM) = dd_inputs(N, B, 900 + N)
+    cfg = dict(size=[N, B, 1], jw_type=14, quadDoubleDoublePrecision=1, scaleC=1.0,
+               offsetV=offV, offsetT=offT, offsetM=offM)
+    got = run_plan(cfg, buf.ravel(), x.ravel()).reshape(-1, 2)
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=1.0,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    want = O.jw_f128_dd(p, buf, x)
+    errs = [O.dd_relerr_norm(got[b * N:(b + 1) * N], want[b * N:(b + 1) * N]) for b in range(B)]
+    assert max(errs) <= 1e-30
+
+
+def test_double_double_geometry_reports_the_wave_per_system_kernel():
+    import torch
+    for N, R in [(64, 2), (200, 4), (512, 8)]:
+        buf, _, (offV, offT, offM) = dd_inputs(N, 1, 3)
+        dmat = torch.from_numpy(buf.ravel()).cuda()
+        x = torch.zeros(2 * N, dtype=torch.float64, device="cuda")
+        with pf.Plan(dmat.data_ptr(), x.data_ptr(), size=[N, 1, 1], jw_type=14, quadDoubleDoublePrecision=1,
+                     offsetV=offV, offsetT=offT, offsetM=offM) as plan:
+            g = plan.geometry()
+            assert g.fast_rows_per_lane == R and g.fast_waves == 8
+        with pf.Plan(dmat.data_ptr(), x.data_ptr(), size=[N, 1, 1], jw_type=14, quadDoubleDoublePrecision=1,
+                     offsetV=offV, offsetT=offT, offsetM=offM, kernelVariant=1) as plan:
+            assert plan.geometry().fast_rows_per_lane == 0
 
 
 # ---------------------------------------------------------------- block plan
