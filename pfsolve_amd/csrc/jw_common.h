@@ -77,6 +77,37 @@ __device__ __forceinline__ dd dd_div(dd x, dd y)
 }
 
 // ------------------------------------------------------------------------
+// double-double as the reference's generated code forms it
+// (PfSolve/pfSolve_CodeGen/pfSolve_MathUtils/pfSolve_MathUtils.h): a sum is TwoSum of the
+// high words, the low words added to its error term, one quick renormalisation
+// (PfQuadSum / PfQuadQuickSum, :1093-1152, double-double branch :1188-1196); a product is
+// TwoProd of the high words, the two cross terms added by FMA, one quick renormalisation
+// (:2398-2412).  dd_madd_r fuses a + b*c: the product's two words go into the sum without
+// the product's own renormalisation (15 FP64 operations; the accurate forms above take 29).
+// ------------------------------------------------------------------------
+__device__ __forceinline__ dd dd_mul_r(dd x, dd y)
+{
+    const double p = x.hi * y.hi;
+    double e = __builtin_fma(x.hi, y.hi, -p);
+    e = __builtin_fma(x.hi, y.lo, e);
+    e = __builtin_fma(x.lo, y.hi, e);
+    return fast_two_sum(p, e);
+}
+
+__device__ __forceinline__ dd dd_madd_r(dd a, dd b, dd c)
+{
+    const double p = b.hi * c.hi;
+    double e = __builtin_fma(b.hi, c.hi, -p);
+    e = __builtin_fma(b.hi, c.lo, e);
+    e = __builtin_fma(b.lo, c.hi, e);
+    const dd s = two_sum(a.hi, p);
+    return fast_two_sum(s.hi, s.lo + (a.lo + e));
+}
+
+// a - b*c
+__device__ __forceinline__ dd dd_msub_r(dd a, dd b, dd c) { return dd_madd_r(a, dd_neg(b), c); }
+
+// ------------------------------------------------------------------------
 // cross-lane moves (ds_bpermute: any lane of the wavefront, 32 bits per op)
 // ------------------------------------------------------------------------
 __device__ __forceinline__ double lane_fetch(double v, int src_lane)

@@ -9,15 +9,15 @@
 // affine scan across the wavefront (DPP row shifts 1, 2, 4, 8 inside 16-lane rows, then
 // across rows row_bcast15/row_bcast31 forward or readlane of the row starts backward)
 // and a lane-local redo; row-0 scaleC.  Every DPP move of a double-double moves its two
-// doubles (four 32-bit moves); the arithmetic is the double-double of jw_common.h
-// (PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel0/pfSolve_MathUtils.h:1093-1196,2398-2412
-// semantics).  DD storage (16 bytes per value) or FP64 storage (mem_f64).
+// doubles (four 32-bit moves).  Arithmetic: the reference's double-double forms
+// (dd_mul_r / dd_madd_r of jw_common.h, after
+// PfSolve/pfSolve_CodeGen/pfSolve_MathUtils/pfSolve_MathUtils.h:1093-1196,2398-2412).
+// Storage: DD (16 bytes per value) or FP64 promoted with a zero low word (mem_f64); rows
+// sys*stride + offSol + row, one 16-byte (or 8-byte) access per row and lane.
 //
 // Reference semantics restated (paths relative to the reference root):
 //   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel2/pfSolve_JonesWorland.h:38-408
 //   PfSolve/pfSolve_CodeGen/pfSolve_KernelsLevel1/pfSolve_TridiagonalSolve.h:1578-1929
-// Storage: DD (16 bytes per value) or FP64 promoted with a zero low word (mem_f64); rows
-// r*stride + offSol + row, one 16-byte (or 8-byte) access per row and lane.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -92,15 +92,15 @@ __device__ __forceinline__ void compute_pair_dd(const dd* tab, int lane, int kl,
             const dd n1b = (i + 1 < R) ? x1[(i + 1) % R] : p1b;
             const dd n2a = (i + 2 < R) ? x0[(i + 2) % R] : ((i + 2 == R) ? p1a : p2a);
             const dd n2b = (i + 2 < R) ? x1[(i + 2) % R] : ((i + 2 == R) ? p1b : p2b);
-            dd ua = A::mul(v0, ia), ub = A::mul(v0, ib);
-            ua = A::madd(ua, vm1, m1a);
-            ub = A::madd(ub, vm1, m1b);
-            ua = A::madd(ua, vm2, m2a);
-            ub = A::madd(ub, vm2, m2b);
-            ua = A::madd(ua, vp1, n1a);
-            ub = A::madd(ub, vp1, n1b);
-            ua = A::madd(ua, vp2, n2a);
-            ub = A::madd(ub, vp2, n2b);
+            dd ua = dd_mul_r(v0, ia), ub = dd_mul_r(v0, ib);
+            ua = dd_madd_r(ua, vm1, m1a);
+            ub = dd_madd_r(ub, vm1, m1b);
+            ua = dd_madd_r(ua, vm2, m2a);
+            ub = dd_madd_r(ub, vm2, m2b);
+            ua = dd_madd_r(ua, vp1, n1a);
+            ub = dd_madd_r(ub, vp1, n1b);
+            ua = dd_madd_r(ua, vp2, n2a);
+            ub = dd_madd_r(ub, vp2, n2b);
             m2a = m1a;
             m2b = m1b;
             m1a = ia;
@@ -118,8 +118,8 @@ __device__ __forceinline__ void compute_pair_dd(const dd* tab, int lane, int kl,
         dd a = x0[0], b = x1[0];
 #pragma unroll
         for (int i = 1; i < R; i++) {
-            a = A::msub(x0[i], lr[i], a);
-            b = A::msub(x1[i], lr[i], b);
+            a = dd_msub_r(x0[i], lr[i], a);
+            b = dd_msub_r(x1[i], lr[i], b);
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -131,24 +131,24 @@ __device__ __forceinline__ void compute_pair_dd(const dd* tab, int lane, int kl,
             case 2: sa = dpp_dd<SHR(4)>(a); sb = dpp_dd<SHR(4)>(b); break;
             default: sa = dpp_dd<SHR(8)>(a); sb = dpp_dd<SHR(8)>(b); break;
             }
-            a = A::madd(a, f, sa);
-            b = A::madd(b, f, sb);
+            a = dd_madd_r(a, f, sa);
+            b = dd_madd_r(b, f, sb);
         }
         {
             // rows 1 and 3 take the ends of rows 0 and 2; rows 2 and 3 then take lane 31
             const dd fp = tab[SC + 4 * 64 + lane], fp2 = tab[SC + 5 * 64 + lane];
             const dd sa = dpp_rm_dd<DPP_BCAST15, 0xa>(a), sb = dpp_rm_dd<DPP_BCAST15, 0xa>(b);
-            a = A::madd(a, fp, sa);
-            b = A::madd(b, fp, sb);
+            a = dd_madd_r(a, fp, sa);
+            b = dd_madd_r(b, fp, sb);
             const dd ta = dpp_rm_dd<DPP_BCAST31, 0xc>(a), tb = dpp_rm_dd<DPP_BCAST31, 0xc>(b);
-            a = A::madd(a, fp2, ta);
-            b = A::madd(b, fp2, tb);
+            a = dd_madd_r(a, fp2, ta);
+            b = dd_madd_r(b, fp2, tb);
         }
         dd wa = dpp_dd<WAVE_SHR1>(a), wb = dpp_dd<WAVE_SHR1>(b);
 #pragma unroll
         for (int i = 0; i < R; i++) {
-            wa = A::msub(x0[i], lr[i], wa);
-            wb = A::msub(x1[i], lr[i], wb);
+            wa = dd_msub_r(x0[i], lr[i], wa);
+            wb = dd_msub_r(x1[i], lr[i], wb);
             x0[i] = wa;
             x1[i] = wb;
         }
@@ -166,8 +166,8 @@ __device__ __forceinline__ void compute_pair_dd(const dd* tab, int lane, int kl,
         dd a = x0[R - 1], b = x1[R - 1];
 #pragma unroll
         for (int i = R - 2; i >= 0; i--) {
-            a = A::msub(x0[i], hh[i], a);
-            b = A::msub(x1[i], hh[i], b);
+            a = dd_msub_r(x0[i], hh[i], a);
+            b = dd_msub_r(x1[i], hh[i], b);
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -179,8 +179,8 @@ __device__ __forceinline__ void compute_pair_dd(const dd* tab, int lane, int kl,
             case 2: sa = dpp_dd<SHL(4)>(a); sb = dpp_dd<SHL(4)>(b); break;
             default: sa = dpp_dd<SHL(8)>(a); sb = dpp_dd<SHL(8)>(b); break;
             }
-            a = A::madd(a, f, sa);
-            b = A::madd(b, f, sb);
+            a = dd_madd_r(a, f, sa);
+            b = dd_madd_r(b, f, sb);
         }
         {
             // every row adds the in-row results of the later row starts (lanes 16, 32, 48),
@@ -190,26 +190,26 @@ __device__ __forceinline__ void compute_pair_dd(const dd* tab, int lane, int kl,
             const dd c3 = tab[SC + (j * 8 + 6) * 64 + lane];
             const dd a1 = readlane_dd(a, 16), a2 = readlane_dd(a, 32), a3 = readlane_dd(a, 48);
             const dd q1 = readlane_dd(b, 16), q2 = readlane_dd(b, 32), q3 = readlane_dd(b, 48);
-            a = A::madd(a, c1, a1);
-            b = A::madd(b, c1, q1);
-            a = A::madd(a, c2, a2);
-            b = A::madd(b, c2, q2);
-            a = A::madd(a, c3, a3);
-            b = A::madd(b, c3, q3);
+            a = dd_madd_r(a, c1, a1);
+            b = dd_madd_r(b, c1, q1);
+            a = dd_madd_r(a, c2, a2);
+            b = dd_madd_r(b, c2, q2);
+            a = dd_madd_r(a, c3, a3);
+            b = dd_madd_r(b, c3, q3);
         }
         dd za = dpp_dd<WAVE_SHL1>(a), zb = dpp_dd<WAVE_SHL1>(b);
 #pragma unroll
         for (int i = R - 1; i >= 0; i--) {
-            za = A::msub(x0[i], hh[i], za);
-            zb = A::msub(x1[i], hh[i], zb);
+            za = dd_msub_r(x0[i], hh[i], za);
+            zb = dd_msub_r(x1[i], hh[i], zb);
             x0[i] = za;
             x1[i] = zb;
         }
     }
     // ---- row-0 scaleC (pfSolve_TridiagonalSolve.h:1918-1927) ----
     if (apply_scale && lane == 0) {
-        x0[0] = A::mul(x0[0], scale);
-        x1[0] = A::mul(x1[0], scale);
+        x0[0] = dd_mul_r(x0[0], scale);
+        x1[0] = dd_mul_r(x1[0], scale);
     }
 }
 
