@@ -132,7 +132,11 @@ typedef struct pfsolve_batch_tuning {
                                                    stores its pairs, 2: compute / store split
                                                    (R <= 16; `waves` counts the computing
                                                    wavefronts), 3: direct I/O (R <= 16: the
-                                                   table alone in LDS, up to 8 wavefronts) */
+                                                   table alone in LDS, up to 8 wavefronts),
+                                                   4: as 1, with each set's table built in the
+                                                   workgroup's LDS from its matrices (R <= 16,
+                                                   >= 4 wavefronts; no table build launch and
+                                                   no table through HBM) */
     int32_t order;            /* 0: largest N first, 1: sets as given */
     int32_t pad;
 } pfsolve_batch_tuning;
@@ -155,15 +159,17 @@ PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set*
                                    const pfsolve_batch_tuning* tuning);
 PfSolveResult pfsolve_batch_run(pfsolve_batch* b, hipStream_t stream);
 /* run = prepare + solve.  prepare writes every set's coefficient table from its
- * matrices; solve applies the resident tables to the right-hand sides.  A
+ * matrices (except for row classes on queue kernel 4, whose solve builds them in
+ * LDS); solve applies the resident tables to the right-hand sides.  A
  * caller whose matrices do not change between steps prepares once and then
  * only solves. */
 PfSolveResult pfsolve_batch_prepare(pfsolve_batch* b, hipStream_t stream);
 PfSolveResult pfsolve_batch_solve(pfsolve_batch* b, hipStream_t stream);
 PfSolveResult pfsolve_batch_run_plain(pfsolve_batch* b, hipStream_t stream);
 /* Records pfsolve_batch_run on `stream` (not the null stream) into a hipGraph,
- * then replays it with pfsolve_batch_launch_graph.  The queue counters are
- * reset by a kernel inside the graph, so every replay is a full step. */
+ * then replays it with pfsolve_batch_launch_graph.  Every queue kernel leaves its
+ * work queue empty when it ends (no clearing node in the graph), so every replay
+ * is a full step. */
 PfSolveResult pfsolve_batch_capture_graph(pfsolve_batch* b, hipStream_t stream);
 /* solve_only = 1: the graph holds pfsolve_batch_solve (tables prepared beforehand) */
 PfSolveResult pfsolve_batch_capture_graph_mode(pfsolve_batch* b, hipStream_t stream, int solve_only);

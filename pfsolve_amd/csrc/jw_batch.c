@@ -7,9 +7,10 @@
  * per N, PfSolve/pfSolve_AppManagement/pfSolve_RunApp.h:80-159 per plan).
  * Here one step is:
  *   1. jw_prologue_batch_kernel: one workgroup per set writes the set's
- *      coefficient table (format 2) and zeroes the queue counters;
- *   2. per row class (R rows per lane, N <= 64 R): one persistent launch of
- *      jw_batch64_kernel over that class's work units.
+ *      coefficient table (format 2) and zeroes the queue counters (only the
+ *      counters when every row class builds its tables in LDS);
+ *   2. per row class (R rows per lane, N <= 64 R): one persistent launch of a
+ *      queue kernel over that class's work units.
  * Work units are (set, range of system pairs), listed largest N first, so
  * the long units start early and the short ones fill the tail.
  */
@@ -21,6 +22,7 @@
 #include "jw_plan.h"
 
 #define NCLS PFSOLVE_BATCH_CLASSES
+#define NKERNELS 4                    /* queue kernels 1..NKERNELS (0: the default) */
 
 struct pfsolve_batch {
     int64_t nsets;
@@ -31,10 +33,13 @@ struct pfsolve_batch {
     void* tables;                     /* one table slot per set, sized for 2R rows per lane */
     int64_t* table_off;               /* host: byte offset of each set's slot */
     int64_t table_bytes;
-    jw_prologue_job* d_jobs;
+    jw_prologue_job* d_jobs;          /* every set's table build, indexed by set */
+    jw_prologue_job* d_gjobs;         /* the builds of the sets whose tables go to global memory */
+    int64_t n_gjobs;
     jw_batch_item* d_items;
     jw_batch_unit* d_units;           /* units of all launch groups, back to back */
-    int* d_counters;                  /* one queue counter per launch group, then the error flag */
+    int* d_counters;                  /* per launch group a queue (next unit, workgroups left; the
+                                         queue kernels leave it at 0), then the error flag */
     /* launch groups: sets run with R_eff = 2 << g rows per lane */
     int64_t unit_first[NCLS], unit_count[NCLS];
     int group_waves[NCLS], group_grid[NCLS], group_kernel[NCLS];
@@ -130,7 +135,7 @@ PfSolveResult pfsolve_tuning_load(const char* path)
         int c, r, b, w, u, g, s, o, k = 0;
         const int nf = sscanf(line, "%d %d %d %d %d %d %d %d %d", &c, &r, &b, &w, &u, &g, &s, &o, &k);
         if ((nf != 8 && nf != 9) || c < 0 || c >= NCLS || r != (2 << c) || w < 0 || u < 0 || g < 0 || s < 0 ||
-            s > 1 || o < 0 || o > 1 || k < 0 || k > 2) {
+            s > 1 || o < 0 || o > 1 || k < 0 || k > NKERNELS) {
             bad = 1;
             continue;
         }
@@ -214,7 +219,9 @@ static int group_waves_default(int g)
 /* kernel of launch group g: 1 = load, compute and store in every wavefront
  * (input through an LDS staging), 2 = compute / store split (R <= 16), 3 = direct
  * I/O (R <= 16: LDS holds the table only, loads one pass ahead, stores from
- * registers, up to 8 wavefronts on one table).  Default: the split where it is
+ * registers, up to 8 wavefronts on one table), 4 = kernel 1 with each set's table
+ * built in the workgroup's LDS when its set changes (R <= 16, >= 4 wavefronts; no
+ * table build launch, no table through HBM).  Default: the split where it is
  * measured faster (R <= 8: 5.35 against 4.94 TB/s at N = 512); at R = 16 the
  * staged kernels reach the same rate with the 4-5 wavefronts LDS leaves them, and
  * direct I/O measured slower (harmonic batch, solve only: 256-295 us against
@@ -222,7 +229,7 @@ static int group_waves_default(int g)
  * the staged, contiguous ones), so it is a tuner candidate only. */
 static int group_kernel_default(int g) { return g <= 2 ? 2 : 1; }
 
-static int group_kernel_ok(int g, int k) { return k == 1 || ((k == 2 || k == 3) && g <= 3); }
+static int group_kernel_ok(int g, int k) { return k == 1 || ((k == 2 || k == 3 || k == 4) && g <= 3); }
 
 typedef struct n_index {
     int64_t N, idx;
@@ -302,7 +309,13 @@ static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
         }
         if (!group_kernel_ok(g, k)) k = group_kernel_default(g);
         int occ = 0;
-        if (k == 3) {
+        if (k == 4) {
+            /* the table build needs at least 4 wavefronts */
+            const int wmax = group_waves_default(g);
+            if (w <= 0 || w > wmax) w = wmax;
+            if (w < 4) w = 4;
+            occ = jwdev_batch64b_occupancy(2 << g, w);
+        } else if (k == 3) {
             if (w <= 0 || w > 8) w = 8;
             occ = jwdev_batch64d_occupancy(2 << g, w);
         } else if (k == 2) {
@@ -338,10 +351,14 @@ static PfSolveResult upload_queue(pfsolve_batch* b)
     hipError_t e = hipMemcpy(b->d_units, u, bytes, hipMemcpyHostToDevice);
     free(u);
     if (e != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_COPY;
-    /* the prologue jobs carry the effective rows per lane of each set */
-    jw_prologue_job* jobs = (jw_prologue_job*)calloc((size_t)(b->nsets > 0 ? b->nsets : 1), sizeof(jw_prologue_job));
-    if (!jobs) return PFSOLVE_ERROR_MALLOC_FAILED;
+    /* the prologue jobs carry the effective rows per lane of each set; the sets of groups
+     * that build their tables in LDS are left out of the table build launch */
+    const size_t njb = (size_t)(b->nsets > 0 ? b->nsets : 1);
+    jw_prologue_job* jobs = (jw_prologue_job*)calloc(njb, sizeof(jw_prologue_job));
+    jw_prologue_job* gjobs = (jw_prologue_job*)calloc(njb, sizeof(jw_prologue_job));
+    if (!jobs || !gjobs) { free(jobs); free(gjobs); return PFSOLVE_ERROR_MALLOC_FAILED; }
     b->max_RL = 0;
+    b->n_gjobs = 0;
     for (int64_t s = 0; s < b->nsets; s++) {
         source_of(&b->sets[s], &jobs[s].src);
         jobs[s].matrices = b->sets[s].matrices;
@@ -349,10 +366,15 @@ static PfSolveResult upload_queue(pfsolve_batch* b)
         jobs[s].L = 64;
         jobs[s].R = 2 << eff_group(b, s);
         jobs[s].format = 2;
+        if (b->group_kernel[eff_group(b, s)] == 4) continue;
+        gjobs[b->n_gjobs++] = jobs[s];
         if (64 * jobs[s].R > b->max_RL) b->max_RL = 64 * jobs[s].R;
     }
     e = hipMemcpy(b->d_jobs, jobs, sizeof(jw_prologue_job) * (size_t)b->nsets, hipMemcpyHostToDevice);
+    if (e == hipSuccess && b->n_gjobs > 0)
+        e = hipMemcpy(b->d_gjobs, gjobs, sizeof(jw_prologue_job) * (size_t)b->n_gjobs, hipMemcpyHostToDevice);
     free(jobs);
+    free(gjobs);
     if (e != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_COPY;
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = NULL; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = NULL; }
@@ -378,6 +400,7 @@ void pfsolve_batch_destroy(pfsolve_batch* b)
     if (b->graph) hipGraphDestroy(b->graph);
     if (b->tables) hipFree(b->tables);
     if (b->d_jobs) hipFree(b->d_jobs);
+    if (b->d_gjobs) hipFree(b->d_gjobs);
     if (b->d_items) hipFree(b->d_items);
     if (b->d_units) hipFree(b->d_units);
     if (b->d_counters) hipFree(b->d_counters);
@@ -436,12 +459,13 @@ PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set*
     b->table_bytes = off;
     if (hipMalloc(&b->tables, (size_t)off) != hipSuccess ||
         hipMalloc((void**)&b->d_jobs, sizeof(jw_prologue_job) * (size_t)nsets) != hipSuccess ||
+        hipMalloc((void**)&b->d_gjobs, sizeof(jw_prologue_job) * (size_t)nsets) != hipSuccess ||
         hipMalloc((void**)&b->d_items, sizeof(jw_batch_item) * (size_t)nsets) != hipSuccess ||
-        hipMalloc((void**)&b->d_counters, sizeof(int) * (NCLS + 1)) != hipSuccess) {
+        hipMalloc((void**)&b->d_counters, sizeof(int) * (2 * NCLS + 1)) != hipSuccess) {
         pfsolve_batch_destroy(b);
         return PFSOLVE_ERROR_FAILED_TO_ALLOCATE;
     }
-    if (hipMemset(b->d_counters, 0, sizeof(int) * (NCLS + 1)) != hipSuccess) {
+    if (hipMemset(b->d_counters, 0, sizeof(int) * (2 * NCLS + 1)) != hipSuccess) {
         pfsolve_batch_destroy(b);
         return PFSOLVE_ERROR_FAILED_TO_COPY;
     }
@@ -474,7 +498,7 @@ PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tun
     if (!b || !t) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     for (int c = 0; c < NCLS; c++)
         if (t->waves[c] < 0 || t->unit_pairs[c] < 0 || t->grid_per_cu[c] < 0 || t->rows_per_lane_shift[c] < 0 ||
-            t->rows_per_lane_shift[c] > 1 || t->kernel[c] < 0 || t->kernel[c] > 2)
+            t->rows_per_lane_shift[c] > 1 || t->kernel[c] < 0 || t->kernel[c] > NKERNELS)
             return PFSOLVE_ERROR_EMPTY_size;
     if (t->order < 0 || t->order > 1) return PFSOLVE_ERROR_EMPTY_size;
     b->tuning = *t;
@@ -482,22 +506,34 @@ PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tun
     return upload_queue(b);
 }
 
+/* the tables that live in global memory (that launch also clears the queues).  With none,
+ * nothing is launched: every queue kernel leaves its queue empty when it ends, so a step
+ * whose row classes all build their tables in LDS is one launch per class. */
+static int run_prologue(pfsolve_batch* b, hipStream_t stream)
+{
+    if (b->n_gjobs > 0)
+        return jwdev_prologue_batch(b->d_gjobs, (int)b->n_gjobs, b->max_RL, JW_PREC_F64, b->d_counters, 2 * NCLS,
+                                    stream);
+    return 0;
+}
+
 static PfSolveResult launch_step(pfsolve_batch* b, const jw_batch_item* d_items, hipStream_t stream, int only_group,
                                  int with_prologue)
 {
-    if (with_prologue &&
-        jwdev_prologue_batch(b->d_jobs, (int)b->nsets, b->max_RL, JW_PREC_F64, b->d_counters, NCLS, stream) != 0)
-        return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    if (with_prologue && run_prologue(b, stream) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     for (int g = 0; g < NCLS; g++) {
         if (!b->unit_count[g] || (only_group >= 0 && g != only_group)) continue;
         const int k = b->group_kernel[g];
-        const int rc = k == 2 ? jwdev_batch64s(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
-                                               b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g],
-                                               b->d_counters + NCLS, b->wait_polls, stream)
+        const int rc = k == 4 ? jwdev_batch64b(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
+                                               b->d_counters + 2 * g, 2 << g, b->group_waves[g], b->group_grid[g],
+                                               b->d_jobs, stream)
+                       : k == 2 ? jwdev_batch64s(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
+                                               b->d_counters + 2 * g, 2 << g, b->group_waves[g], b->group_grid[g],
+                                               b->d_counters + 2 * NCLS, b->wait_polls, stream)
                        : k == 3 ? jwdev_batch64d(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
-                                                 b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g], stream)
+                                                 b->d_counters + 2 * g, 2 << g, b->group_waves[g], b->group_grid[g], stream)
                                 : jwdev_batch64(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
-                                                b->d_counters + g, 2 << g, b->group_waves[g], b->group_grid[g], stream);
+                                                b->d_counters + 2 * g, 2 << g, b->group_waves[g], b->group_grid[g], stream);
         if (rc != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     }
     return PFSOLVE_SUCCESS;
@@ -512,15 +548,12 @@ PfSolveResult pfsolve_batch_run(pfsolve_batch* b, hipStream_t stream)
 PfSolveResult pfsolve_batch_prepare(pfsolve_batch* b, hipStream_t stream)
 {
     if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
-    return jwdev_prologue_batch(b->d_jobs, (int)b->nsets, b->max_RL, JW_PREC_F64, b->d_counters, NCLS, stream) == 0
-               ? PFSOLVE_SUCCESS
-               : PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+    return run_prologue(b, stream) == 0 ? PFSOLVE_SUCCESS : PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
 }
 
 PfSolveResult pfsolve_batch_solve(pfsolve_batch* b, hipStream_t stream)
 {
     if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
-    if (jwdev_zero_counters(b->d_counters, NCLS, stream) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     return launch_step(b, b->d_items, stream, -1, 0);
 }
 
@@ -612,10 +645,10 @@ PfSolveResult pfsolve_batch_sync(pfsolve_batch* b, hipStream_t stream)
     if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     if (hipStreamSynchronize(stream) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
     int e = 0;
-    if (hipMemcpy(&e, b->d_counters + NCLS, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(&e, b->d_counters + 2 * NCLS, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
         return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
     if (e) {
-        (void)hipMemset(b->d_counters + NCLS, 0, sizeof(int));
+        (void)hipMemset(b->d_counters + 2 * NCLS, 0, sizeof(int));
         return PFSOLVE_ERROR_FAILED_TO_SYNCHRONIZE;
     }
     return PFSOLVE_SUCCESS;
@@ -652,13 +685,11 @@ static int time_group(pfsolve_batch* b, const jw_batch_item* d_items, hipStream_
     /* warm-up (also writes the tables) */
     if (launch_step(b, d_items, stream, g, 1) != PFSOLVE_SUCCESS) rc = -1;
     for (int k = 0; rc == 0 && k < repeats; k++) {
-        /* the prologue resets the queue counters; only the solve is timed */
-        if (jwdev_prologue_batch(b->d_jobs, (int)b->nsets, b->max_RL, JW_PREC_F64, b->d_counters, NCLS, stream) != 0) {
-            rc = -1;
-            break;
-        }
+        /* the whole step: the tables built in global memory (and the counters reset) or
+         * only the counters reset when every class builds its tables in LDS, then the
+         * queue launches */
         hipEventRecord(e0, stream);
-        if (launch_step(b, d_items, stream, g, 0) != PFSOLVE_SUCCESS) { rc = -1; break; }
+        if (launch_step(b, d_items, stream, g, 1) != PFSOLVE_SUCCESS) { rc = -1; break; }
         hipEventRecord(e1, stream);
         if (hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
         float ms = 0.f;
@@ -717,7 +748,7 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
             const int shifts = c < NSHIFT ? 2 : 1;
             for (int sh = 0; sh < shifts && res == PFSOLVE_SUCCESS; sh++) {
                 const int g = c + sh;
-                for (int k = 1; k <= 3 && res == PFSOLVE_SUCCESS; k++) {
+                for (int k = 1; k <= NKERNELS && res == PFSOLVE_SUCCESS; k++) {
                     if (!group_kernel_ok(g, k)) continue;
                     /* a few wave counts: for the one-wavefront kernel the largest, its half and
                      * quarter; for the split kernel (computing wavefronts) the default, its half
@@ -732,6 +763,12 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
                         wl[0] = 8;
                         wl[1] = 4;
                         wl[2] = 2;
+                    } else if (k == 4) {
+                        /* the table build needs >= 4 wavefronts */
+                        const int wmax = group_waves_default(g);
+                        wl[0] = wmax;
+                        wl[1] = (wmax + 1) / 2 >= 4 ? (wmax + 1) / 2 : 0;
+                        wl[2] = 4;
                     } else {
                         int nc = 0, ns = 0;
                         if (jwdev_batch64s_shape(2 << g, 0, &nc, &ns)) wl[0] = nc;
@@ -743,6 +780,7 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
                         if (w < 1 || (wi > 0 && w == wl[0]) || (wi == 2 && w == wl[1])) continue;
                         const int occ = k == 2   ? jwdev_batch64s_occupancy(2 << g, w)
                                         : k == 3 ? jwdev_batch64d_occupancy(2 << g, w)
+                                        : k == 4 ? jwdev_batch64b_occupancy(2 << g, w)
                                                  : jwdev_batch64_occupancy(2 << g, w);
                         for (int gpc = 1; gpc <= occ && gpc <= 2; gpc++) {
                             for (size_t ui = 0; ui < sizeof(units_try) / sizeof(units_try[0]); ui++) {

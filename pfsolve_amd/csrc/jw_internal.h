@@ -210,6 +210,12 @@ int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, in
 int jwdev_batch64d_occupancy(int R, int waves);
 int jwdev_batch64d(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
                    int waves, int grid, void* stream);
+/* the same queue with each set's table built in the workgroup's LDS from the matrices
+ * of d_jobs[item] when the workgroup's set changes (R <= 16, waves >= 4): no table build
+ * launch and no table in global memory */
+int jwdev_batch64b_occupancy(int R, int waves);
+int jwdev_batch64b(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
+                   int waves, int grid, const jw_prologue_job* d_jobs, void* stream);
 /* the same queue with the compute / store split (R <= 16): `ncomp` compute
  * wavefronts (0: default); *d_err is raised if a bounded wait ran out (after
  * `wait_polls` polls: 0 the default, negative none) */

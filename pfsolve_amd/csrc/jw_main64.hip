@@ -143,6 +143,22 @@ __device__ __forceinline__ void zero_staging(double2* stg, int lane)
     for (int k = 0; k < R; k++) stg[64 * k + lane] = make_double2(0.0, 0.0);
 }
 
+// Leaving a work queue.  A queue is a counter pair {next unit, workgroups that have left};
+// the last workgroup to leave puts both back to 0, so the next launch on the stream finds
+// an empty queue with no clearing kernel or memset before it.  Every thread calls this
+// after its workgroup's last claim; thread 0 counts.  The acquire-release increment orders
+// every other workgroup's claims before the reset.
+__device__ __forceinline__ void queue_leave(int* counter)
+{
+    if (threadIdx.x == 0) {
+        const int n = __hip_atomic_fetch_add(counter + 1, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (n == (int)gridDim.x - 1) {
+            __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(counter + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // The pieces of one pass over systems sys0 and sys0+1 (if it exists).  stg: a
 // wavefront's staging for 2 systems (64 R double2); the input slots outside the
 // keep-range hold zeros whenever the input DMA is issued.
@@ -1079,6 +1095,73 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_batch64_kernel(const jw_b
             jw_pair64<R, true>(tab, stg, lane, g, 2 * (int64_t)q);
         }
     }
+    queue_leave(counter);
+}
+
+// LDS of the build-in-LDS queue kernel: the table, then the wavefronts' stagings, which
+// also hold the table build's scratch while a set's table is built
+__host__ __device__ constexpr int64_t batch64b_lds_bytes(int R, int waves)
+{
+    return (int64_t)JW_TAB64_ELEMS(R) * 8 +
+           ((int64_t)2 * 64 * R * 8 * waves > (int64_t)(5 * 64 * R) * 8 ? (int64_t)2 * 64 * R * 8 * waves
+                                                                         : (int64_t)(5 * 64 * R) * 8);
+}
+
+// one set's format-2 table built into LDS (the stagings after it as scratch); kept out of
+// line so that the build's registers do not add to the solve loop's
+// (the dynamic LDS is named here, not passed in, so the build addresses it as LDS)
+template <int R>
+__device__ __noinline__ void build_table_lds(const jw_prologue_job* job)
+{
+    extern __shared__ __align__(16) unsigned char smem_build[];
+    prologue_body<double>(job->src, job->matrices, reinterpret_cast<double*>(smem_build), 64, R, 2,
+                          smem_build + JW_TAB64_ELEMS(R) * 8);
+}
+
+// The batch queue with the table built in LDS: as jw_batch64_kernel, but a workgroup
+// whose set changes builds the set's format-2 table from the caller's matrices in its
+// own LDS (prologue_body, the staging as scratch) instead of copying a table that a
+// separate launch wrote to global memory: no table build launch before the queue, no
+// table written and read back through HBM; the build's latency is paid per set change
+// by the workgroup, while the other workgroups keep HBM busy.  Needs at least 256
+// threads and 4 threads per 16 rows (prologue_body's partition).
+template <int R>
+__global__ __launch_bounds__(max_threads<R>()) void jw_batch64b_kernel(const jw_batch_item* items,
+                                                                       const jw_batch_unit* units, int nunits,
+                                                                       int* counter, const jw_prologue_job* jobs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
+    __shared__ int s_unit;
+    const double* tab = reinterpret_cast<const double*>(smem);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* stg = reinterpret_cast<double2*>(smem + NTAB * 8) + (size_t)wave * 64 * R;
+    int cur = -1;
+    for (;;) {
+        __syncthreads();   // the previous unit is finished: s_unit, the table and the staging may change
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (u >= nunits) break;
+        const jw_batch_unit un = units[u];
+        const jw_batch_item& it = items[un.item];
+        if (un.item != cur) {
+            build_table_lds<R>(jobs + un.item);
+            __syncthreads();   // the build's scratch (the stagings) is free again
+            zero_staging<R>(stg, lane);
+            __syncthreads();
+            cur = un.item;
+        }
+        for (int q = un.pair0 + wave; q < un.pair1; q += nwaves) {
+            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(it.rhs), it.nsys, it.stride, it.offSol, 0,
+                                            it.N, 0, it.N, it.kl, it.ku, true, true, true, it.apply_scale != 0,
+                                            it.scale, lane);
+            jw_pair64<R, true>(tab, stg, lane, g, 2 * (int64_t)q);
+        }
+    }
+    queue_leave(counter);
 }
 
 // 16-byte stores of a pair straight from the registers (lane t: chunks t H .. t H + H-1
@@ -1169,6 +1252,7 @@ __global__ __launch_bounds__(kBatchDirectThreads) void jw_batch64d_kernel(const 
             q = qn;
         }
     }
+    queue_leave(counter);
 }
 
 // The same queue for long systems (R = 32): one system per wavefront pass, table
@@ -1206,6 +1290,7 @@ __global__ __launch_bounds__(256) void jw_batch64g_kernel(const jw_batch_item* i
             jw_single64<R, true>(tab, stg, lane, g, q);
         }
     }
+    queue_leave(counter);
 }
 
 // The batch queue with the compute / store split (R <= 16): the first `ncomp`
@@ -1333,6 +1418,7 @@ __global__ __launch_bounds__(bsplit_threads<R>()) void jw_batch64s_kernel(const 
             }
         }
     }
+    queue_leave(counter);
 }
 
 template <int R>
@@ -1570,6 +1656,52 @@ int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, in
     if (e != hipSuccess) return -2;
     void* args[] = {const_cast<jw_batch_item**>(&d_items), const_cast<jw_batch_unit**>(&d_units), &nunits,
                     &d_counter};
+    e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? 0 : -3;
+}
+
+static const void* pick_batch64b(int R)
+{
+    switch (R) {
+    case 2: return reinterpret_cast<const void*>(&jw_batch64b_kernel<2>);
+    case 4: return reinterpret_cast<const void*>(&jw_batch64b_kernel<4>);
+    case 8: return reinterpret_cast<const void*>(&jw_batch64b_kernel<8>);
+    case 16: return reinterpret_cast<const void*>(&jw_batch64b_kernel<16>);
+    default: return nullptr;
+    }
+}
+
+/* the table build needs >= 256 threads and 4 threads per 16 rows */
+static bool batch64b_shape_ok(int R, int waves)
+{
+    return pick_batch64b(R) && waves >= 4 && waves <= max_waves64(R) && 4 * 64 * waves >= 64 * R &&
+           batch64b_lds_bytes(R, waves) <= kLdsBytesMax;
+}
+
+int jwdev_batch64b_occupancy(int R, int waves)
+{
+    if (!batch64b_shape_ok(R, waves)) return 0;
+    const void* f = pick_batch64b(R);
+    const int lds = (int)batch64b_lds_bytes(R, waves);
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * waves, lds) != hipSuccess) return 0;
+    return nb;
+}
+
+int jwdev_batch64b(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
+                   int waves, int grid, const jw_prologue_job* d_jobs, void* stream)
+{
+    if (!batch64b_shape_ok(R, waves) || grid < 1 || !d_jobs) {
+        snprintf(g_err64, sizeof(g_err64), "bad build-in-LDS batch launch (R=%d, waves=%d)", R, waves);
+        return -1;
+    }
+    const void* f = pick_batch64b(R);
+    const size_t lds = (size_t)batch64b_lds_bytes(R, waves);
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return -2;
+    void* args[] = {const_cast<jw_batch_item**>(&d_items), const_cast<jw_batch_unit**>(&d_units), &nunits,
+                    &d_counter, const_cast<jw_prologue_job**>(&d_jobs)};
     e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : -3;
 }

@@ -580,8 +580,13 @@ PfSolveResult pfsolve_serve(const pfsolve_serve_config* cfg, pfsolve_serve_reque
             dev_host_alloc(dev, (void**)&x->h_units, sizeof(jw_batch_unit) * (size_t)max_units) != 0 ||
             dev_alloc(dev, (void**)&x->d_items, sizeof(jw_batch_item) * (size_t)maxb) != 0 ||
             dev_alloc(dev, (void**)&x->d_units, sizeof(jw_batch_unit) * (size_t)max_units) != 0 ||
-            dev_alloc(dev, (void**)&x->d_counters, sizeof(int) * NCLS) != 0) {
+            dev_alloc(dev, (void**)&x->d_counters, sizeof(int) * 2 * NCLS) != 0) {
             res = PFSOLVE_ERROR_FAILED_TO_ALLOCATE;
+            goto out;
+        }
+        /* the slot's queues start empty (each queue kernel leaves them so) */
+        if (!dev->dry && jwdev_zero_counters(x->d_counters, 2 * NCLS, x->stream) != 0) {
+            res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             goto out;
         }
         x->reqs = (int64_t*)malloc(sizeof(int64_t) * (size_t)maxb);
@@ -794,15 +799,15 @@ PfSolveResult pfsolve_serve(const pfsolve_serve_config* cfg, pfsolve_serve_reque
                 /* a launch failure injected from the host: the kernels of this launch */
                 if (!err && cfg->fail_launch == launches) err = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
                 if (!err && !dev->dry) {
+                    /* the queues are empty: zeroed at set-up, left empty by every queue kernel */
                     if (dev_copy(dev, x->d_items, x->h_items, sizeof(jw_batch_item) * (size_t)nb, x->stream) != 0 ||
                         (nunits > 0 &&
-                         dev_copy(dev, x->d_units, x->h_units, sizeof(jw_batch_unit) * (size_t)nunits, x->stream) != 0) ||
-                        jwdev_zero_counters(x->d_counters, NCLS, x->stream) != 0)
+                         dev_copy(dev, x->d_units, x->h_units, sizeof(jw_batch_unit) * (size_t)nunits, x->stream) != 0))
                         err = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
                     for (int c = 0; c < NCLS && !err; c++) {
                         if (!cls_n[c]) continue;
                         int64_t grid = cls_n[c] < grid_max[c] ? cls_n[c] : grid_max[c];
-                        if (jwdev_batch64(x->d_items, x->d_units + cls_first[c], (int)cls_n[c], x->d_counters + c,
+                        if (jwdev_batch64(x->d_items, x->d_units + cls_first[c], (int)cls_n[c], x->d_counters + 2 * c,
                                           2 << c, waves[c], (int)grid, x->stream) != 0)
                             err = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
                     }

@@ -167,6 +167,11 @@ def tuning(**kw):
     dict(kernel=[3, 3, 3, 3, 1], waves=4, unit_pairs=2),
     dict(kernel=3, waves=2, order=1, unit_pairs=1),
     dict(kernel=[3, 1, 3, 2, 1], rows_per_lane_shift=[1, 0, 1, 0, 0]),
+    dict(kernel=4),
+    dict(kernel=[4, 4, 4, 4, 1], waves=4, unit_pairs=2),
+    dict(kernel=4, waves=2, order=1, unit_pairs=1),
+    dict(kernel=[4, 1, 2, 4, 1], unit_pairs=3),
+    dict(kernel=[4, 1, 4, 3, 1], rows_per_lane_shift=[1, 0, 1, 0, 0]),
 ])
 def test_batch_tunings(kw):
     """every tuning gives the oracle's results; without a rows-per-lane change the
@@ -203,14 +208,15 @@ def test_batch_split_kernel_time_out_is_reported_and_recovers():
     np.testing.assert_array_equal(got, ref)
 
 
-def test_batch_graph_replays():
+@pytest.mark.parametrize("kern", [0, 4])
+def test_batch_graph_replays(kern):
     """the captured step replays as a full step every time (queue counters are reset
     inside the graph), with eager work and synchronisation between replays"""
     t = torch()
     d = BatchData(SMALL_SETS, seed=3)
     eager = run_batch(d)
     s = t.cuda.Stream()
-    with pf.Batch(d.sets) as b:
+    with pf.Batch(d.sets, tuning(kernel=kern) if kern else None) as b:
         b.capture_graph(s.cuda_stream)
         for rep in range(3):
             d.reset()
@@ -238,7 +244,7 @@ def test_batch_tuner_keeps_caller_data_and_results():
     d.check(got)
     for c in range(_abi.BATCH_CLASSES):
         assert best.rows_per_lane_shift[c] in (0, 1)
-        assert best.kernel[c] in (0, 1, 2, 3)
+        assert best.kernel[c] in (0, 1, 2, 3, 4)
         assert best.waves[c] >= 0
 
 
@@ -291,13 +297,15 @@ def test_batch_harmonic_degrees_full_size():
     assert O.relerr_norm(y12, a + 2.0 * y2) <= 1e-13
 
 
-def test_batch_prepare_then_solve_matches_run():
-    """tables prepared once and kept resident: every solve equals a full run"""
+@pytest.mark.parametrize("kern", [0, 4, [4, 1, 4, 2, 1]])
+def test_batch_prepare_then_solve_matches_run(kern):
+    """tables prepared once and kept resident: every solve equals a full run (row classes
+    on queue kernel 4 build their tables inside every solve)"""
     t = torch()
     d = BatchData(SMALL_SETS, seed=7)
     ref = run_batch(d)
     stream = t.cuda.current_stream().cuda_stream
-    with pf.Batch(d.sets) as b:
+    with pf.Batch(d.sets, tuning(kernel=kern) if kern else None) as b:
         b.prepare(stream)
         for _ in range(2):
             d.reset()
