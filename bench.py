@@ -351,7 +351,10 @@ def bench_batch(torch, pf, steps: int, warmup: int) -> dict:
     with pf.Batch(sets) as b:
         tuned = b.tune(sp, repeats=2)
         info = b.info()
-        b.capture_graph(sp)
+        # GSTEPS steps per graph: a replay pays one graph launch (about 7 us more than a
+        # stream launch, by kernel trace) for all of them
+        GSTEPS = 10
+        b.capture_graph(sp, steps=GSTEPS)
 
         def timed(fn, k):
             evs = [torch.cuda.Event(enable_timing=True) for _ in range(k + 1)]
@@ -364,16 +367,24 @@ def bench_batch(torch, pf, steps: int, warmup: int) -> dict:
             per = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(k)])
             return float(evs[0].elapsed_time(evs[k]) / k), per
 
+        reps = max(3, steps // GSTEPS)
+        for _ in range(max(1, warmup // GSTEPS)):
+            b.launch_graph(sp)
+        ms_b, per_b = timed(lambda: b.launch_graph(sp), reps)
+        ms_b, per_b = ms_b / GSTEPS, per_b / GSTEPS
+        ms_e, _ = timed(lambda: b.run(sp), max(3, steps // 4))
+        b.capture_graph(sp)
         for _ in range(warmup):
             b.launch_graph(sp)
-        ms_b, per_b = timed(lambda: b.launch_graph(sp), steps)
-        ms_e, _ = timed(lambda: b.run(sp), max(3, steps // 4))
+        ms_b1, _ = timed(lambda: b.launch_graph(sp), steps)
         b.sync(sp)
         # matrices unchanged between steps: tables stay resident, the graph holds the solve
-        b.capture_graph(sp, solve_only=True)
-        for _ in range(warmup):
+        b.prepare(sp)
+        b.capture_graph(sp, solve_only=True, steps=GSTEPS)
+        for _ in range(max(1, warmup // GSTEPS)):
             b.launch_graph(sp)
-        ms_s, per_s = timed(lambda: b.launch_graph(sp), steps)
+        ms_s, per_s = timed(lambda: b.launch_graph(sp), reps)
+        ms_s, per_s = ms_s / GSTEPS, per_s / GSTEPS
         b.sync(sp)
         for _ in range(2):
             b.run_plain(sp)
@@ -385,10 +396,13 @@ def bench_batch(torch, pf, steps: int, warmup: int) -> dict:
                         "fused step, one V/T/M set per degree",
             "complex_systems": int(systems),
             "batched_graph": {"value": systems / (ms_b * 1e-3), "unit": "systems/s", "ms_per_step": ms_b,
+                              "steps_per_graph": GSTEPS,
                               "p50_ms": float(np.percentile(per_b, 50)), "p99_ms": float(np.percentile(per_b, 99)),
                               "launches": int(info.launches)},
             "batched_eager_ms": ms_e,
+            "batched_graph_one_step_ms": ms_b1,
             "solve_only_graph": {"value": systems / (ms_s * 1e-3), "unit": "systems/s", "ms_per_step": ms_s,
+                                 "steps_per_graph": GSTEPS,
                                  "p50_ms": float(np.percentile(per_s, 50)), "p99_ms": float(np.percentile(per_s, 99)),
                                  "hbm_frac": info.algorithmic_bytes / (ms_s * 1e-3) / HBM_PEAK},
             "plain_per_size": {"value": systems / (ms_p * 1e-3), "unit": "systems/s", "ms_per_step": ms_p,

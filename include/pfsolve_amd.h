@@ -173,6 +173,9 @@ PfSolveResult pfsolve_batch_run_plain(pfsolve_batch* b, hipStream_t stream);
 PfSolveResult pfsolve_batch_capture_graph(pfsolve_batch* b, hipStream_t stream);
 /* solve_only = 1: the graph holds pfsolve_batch_solve (tables prepared beforehand) */
 PfSolveResult pfsolve_batch_capture_graph_mode(pfsolve_batch* b, hipStream_t stream, int solve_only);
+/* the same with `steps` (1..1024) consecutive steps in the one graph: a replay costs one
+ * graph launch (measured about 7 us more than a stream launch on ROCm 7) for all of them */
+PfSolveResult pfsolve_batch_capture_graph_steps(pfsolve_batch* b, hipStream_t stream, int solve_only, int steps);
 PfSolveResult pfsolve_batch_launch_graph(pfsolve_batch* b, hipStream_t stream);
 /* Waits for `stream` and reports a raised device error flag (a bounded wait
  * inside the compute / store split kernel ran out) as

@@ -199,8 +199,8 @@ class Batch:
     def run_plain(self, stream: int = 0) -> None:
         check(lib().pfsolve_batch_run_plain(self.handle, stream), "pfsolve_batch_run_plain")
 
-    def capture_graph(self, stream: int, solve_only: bool = False) -> None:
-        check(lib().pfsolve_batch_capture_graph_mode(self.handle, stream, int(solve_only)),
+    def capture_graph(self, stream: int, solve_only: bool = False, steps: int = 1) -> None:
+        check(lib().pfsolve_batch_capture_graph_steps(self.handle, stream, int(solve_only), int(steps)),
               "pfsolve_batch_capture_graph")
 
     def launch_graph(self, stream: int) -> None:

@@ -409,6 +409,8 @@ def declare_ext(lib: C.CDLL) -> None:
         f.restype = C.c_int
     lib.pfsolve_batch_capture_graph_mode.argtypes = [vp, vp, C.c_int]
     lib.pfsolve_batch_capture_graph_mode.restype = C.c_int
+    lib.pfsolve_batch_capture_graph_steps.argtypes = [vp, vp, C.c_int, C.c_int]
+    lib.pfsolve_batch_capture_graph_steps.restype = C.c_int
     lib.pfsolve_batch_tune.argtypes = [vp, vp, C.c_int, P(PfSolveBatchTuning)]
     lib.pfsolve_batch_tune.restype = C.c_int
     lib.pfsolve_batch_set_tuning.argtypes = [vp, P(PfSolveBatchTuning)]

@@ -606,13 +606,21 @@ PfSolveResult pfsolve_batch_capture_graph(pfsolve_batch* b, hipStream_t stream)
 
 PfSolveResult pfsolve_batch_capture_graph_mode(pfsolve_batch* b, hipStream_t stream, int solve_only)
 {
+    return pfsolve_batch_capture_graph_steps(b, stream, solve_only, 1);
+}
+
+PfSolveResult pfsolve_batch_capture_graph_steps(pfsolve_batch* b, hipStream_t stream, int solve_only, int steps)
+{
     if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     if (!stream) return PFSOLVE_ERROR_INVALID_QUEUE;
+    if (steps < 1 || steps > 1024) return PFSOLVE_ERROR_EMPTY_size;
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = NULL; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = NULL; }
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) != hipSuccess)
         return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
-    PfSolveResult r = solve_only ? pfsolve_batch_solve(b, stream) : launch_step(b, b->d_items, stream, -1, 1);
+    PfSolveResult r = PFSOLVE_SUCCESS;
+    for (int i = 0; i < steps && r == PFSOLVE_SUCCESS; i++)
+        r = solve_only ? pfsolve_batch_solve(b, stream) : launch_step(b, b->d_items, stream, -1, 1);
     hipGraph_t g = NULL;
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (r != PFSOLVE_SUCCESS) { if (g) hipGraphDestroy(g); return r; }
@@ -665,7 +673,9 @@ PfSolveResult pfsolve_batch_get_info(const pfsolve_batch* b, pfsolve_batch_info*
     int64_t alg = 0;
     for (int64_t s = 0; s < b->nsets; s++) alg += 16 * b->sets[s].N * b->sets[s].nsys + 80 * b->sets[s].N;
     info->algorithmic_bytes = alg;
-    int launches = 1;
+    /* per step: the table build launch (when some class reads tables from global memory)
+     * and one queue launch per class */
+    int launches = b->n_gjobs > 0 ? 1 : 0;
     for (int g = 0; g < NCLS; g++)
         if (b->unit_count[g]) { info->classes_used |= 1 << g; launches++; }
     info->launches = launches;

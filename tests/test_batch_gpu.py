@@ -297,6 +297,42 @@ def test_batch_harmonic_degrees_full_size():
     assert O.relerr_norm(y12, a + 2.0 * y2) <= 1e-13
 
 
+@pytest.mark.parametrize("kern", [0, 4])
+@pytest.mark.parametrize("solve_only", [0, 1])
+def test_batch_graph_of_several_steps(kern, solve_only):
+    """three steps captured in one graph: one replay equals three eager steps, bit for bit,
+    and a second replay continues from there"""
+    t = torch()
+    d = BatchData(SMALL_SETS, seed=9)
+    stream = t.cuda.current_stream().cuda_stream
+    tun = tuning(kernel=kern) if kern else None
+    d.reset()
+    with pf.Batch(d.sets, tun) as b:
+        b.run(stream)
+        b.sync(stream)
+        one = d.result()
+        b.run(stream)
+        b.run(stream)
+        b.sync(stream)
+        three = d.result()
+        for _ in range(3):
+            b.run(stream)
+        b.sync(stream)
+        six = d.result()
+        d.reset()
+        if solve_only:
+            b.prepare(stream)
+        s = t.cuda.Stream()
+        b.capture_graph(s.cuda_stream, solve_only=bool(solve_only), steps=3)
+        b.launch_graph(s.cuda_stream)
+        s.synchronize()
+        np.testing.assert_array_equal(d.result(), three)
+        b.launch_graph(s.cuda_stream)
+        s.synchronize()
+        np.testing.assert_array_equal(d.result(), six)
+    assert not np.array_equal(one, three)
+
+
 @pytest.mark.parametrize("kern", [0, 4, [4, 1, 4, 2, 1]])
 def test_batch_prepare_then_solve_matches_run(kern):
     """tables prepared once and kept resident: every solve equals a full run (row classes
