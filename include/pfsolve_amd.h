@@ -188,8 +188,13 @@ PfSolveResult pfsolve_batch_set_wait_polls(pfsolve_batch* b, int32_t polls);
  * lane, queue order) on scratch right-hand sides of the batch's shapes (the
  * caller's systems are not touched), keeps the fastest per class, applies it
  * to the batch and returns it in *best (may be NULL).  `repeats` timed runs
- * per candidate (>= 1).  Synchronises the stream. */
+ * per candidate (>= 1).  Synchronises the stream.  Candidates are timed as
+ * whole steps (pfsolve_batch_run); with solve_only = 1 as solves on tables
+ * prepared beforehand (pfsolve_batch_solve), for callers whose matrices do not
+ * change between steps.  The process-wide table keeps the last result. */
 PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repeats, pfsolve_batch_tuning* best);
+PfSolveResult pfsolve_batch_tune_mode(pfsolve_batch* b, hipStream_t stream, int repeats, int solve_only,
+                                      pfsolve_batch_tuning* best);
 PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tuning* t);
 PfSolveResult pfsolve_batch_get_info(const pfsolve_batch* b, pfsolve_batch_info* info);
 void pfsolve_batch_destroy(pfsolve_batch* b);

@@ -206,9 +206,10 @@ class Batch:
     def launch_graph(self, stream: int) -> None:
         check(lib().pfsolve_batch_launch_graph(self.handle, stream), "pfsolve_batch_launch_graph")
 
-    def tune(self, stream: int, repeats: int = 3) -> _abi.PfSolveBatchTuning:
+    def tune(self, stream: int, repeats: int = 3, solve_only: bool = False) -> _abi.PfSolveBatchTuning:
         t = _abi.PfSolveBatchTuning()
-        check(lib().pfsolve_batch_tune(self.handle, stream, repeats, C.byref(t)), "pfsolve_batch_tune")
+        check(lib().pfsolve_batch_tune_mode(self.handle, stream, repeats, int(solve_only), C.byref(t)),
+              "pfsolve_batch_tune")
         return t
 
     def set_tuning(self, t: _abi.PfSolveBatchTuning) -> None:

@@ -413,6 +413,8 @@ def declare_ext(lib: C.CDLL) -> None:
     lib.pfsolve_batch_capture_graph_steps.restype = C.c_int
     lib.pfsolve_batch_tune.argtypes = [vp, vp, C.c_int, P(PfSolveBatchTuning)]
     lib.pfsolve_batch_tune.restype = C.c_int
+    lib.pfsolve_batch_tune_mode.argtypes = [vp, vp, C.c_int, C.c_int, P(PfSolveBatchTuning)]
+    lib.pfsolve_batch_tune_mode.restype = C.c_int
     lib.pfsolve_batch_set_tuning.argtypes = [vp, P(PfSolveBatchTuning)]
     lib.pfsolve_batch_set_tuning.restype = C.c_int
     lib.pfsolve_batch_set_wait_polls.argtypes = [vp, C.c_int32]

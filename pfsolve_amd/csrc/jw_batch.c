@@ -685,7 +685,7 @@ PfSolveResult pfsolve_batch_get_info(const pfsolve_batch* b, pfsolve_batch_info*
 
 /* ---- autotuner ---- */
 static int time_group(pfsolve_batch* b, const jw_batch_item* d_items, hipStream_t stream, int g, int repeats,
-                      float* ms_out)
+                      int solve_only, float* ms_out)
 {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return -1;
@@ -695,11 +695,11 @@ static int time_group(pfsolve_batch* b, const jw_batch_item* d_items, hipStream_
     /* warm-up (also writes the tables) */
     if (launch_step(b, d_items, stream, g, 1) != PFSOLVE_SUCCESS) rc = -1;
     for (int k = 0; rc == 0 && k < repeats; k++) {
-        /* the whole step: the tables built in global memory (and the counters reset) or
-         * only the counters reset when every class builds its tables in LDS, then the
-         * queue launches */
+        /* the whole step (the global tables' build, then the queue launches), or with
+         * solve_only the queue launches alone on tables built beforehand */
+        if (solve_only && run_prologue(b, stream) != 0) { rc = -1; break; }
         hipEventRecord(e0, stream);
-        if (launch_step(b, d_items, stream, g, 1) != PFSOLVE_SUCCESS) { rc = -1; break; }
+        if (launch_step(b, d_items, stream, g, !solve_only) != PFSOLVE_SUCCESS) { rc = -1; break; }
         hipEventRecord(e1, stream);
         if (hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
         float ms = 0.f;
@@ -715,6 +715,12 @@ static int time_group(pfsolve_batch* b, const jw_batch_item* d_items, hipStream_
 }
 
 PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repeats, pfsolve_batch_tuning* best_out)
+{
+    return pfsolve_batch_tune_mode(b, stream, repeats, 0, best_out);
+}
+
+PfSolveResult pfsolve_batch_tune_mode(pfsolve_batch* b, hipStream_t stream, int repeats, int solve_only,
+                                      pfsolve_batch_tuning* best_out)
 {
     if (!b) return PFSOLVE_ERROR_PLAN_NOT_INITIALIZED;
     if (repeats < 1) repeats = 1;
@@ -806,7 +812,7 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
                                 res = upload_queue(b);
                                 if (res != PFSOLVE_SUCCESS) break;
                                 float ms = 0.f;
-                                if (time_group(b, d_items, stream, -1, repeats, &ms) != 0) {
+                                if (time_group(b, d_items, stream, -1, repeats, solve_only, &ms) != 0) {
                                     res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
                                     break;
                                 }
@@ -828,7 +834,8 @@ PfSolveResult pfsolve_batch_tune(pfsolve_batch* b, hipStream_t stream, int repea
                 b->tuning = t;
                 res = upload_queue(b);
                 if (res != PFSOLVE_SUCCESS) break;
-                if (time_group(b, d_items, stream, -1, repeats, &ms_order[o]) != 0) res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+                if (time_group(b, d_items, stream, -1, repeats, solve_only, &ms_order[o]) != 0)
+                    res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
             }
             cur.order = ms_order[1] < ms_order[0] ? 1 : 0;
         }

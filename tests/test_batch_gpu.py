@@ -228,13 +228,14 @@ def test_batch_graph_replays(kern):
             assert junk.item() == float(1 << 20)
 
 
-def test_batch_tuner_keeps_caller_data_and_results():
+@pytest.mark.parametrize("solve_only", [False, True])
+def test_batch_tuner_keeps_caller_data_and_results(solve_only):
     t = torch()
     d = BatchData(SMALL_SETS, seed=4)
     d.reset()
     stream = t.cuda.current_stream().cuda_stream
     with pf.Batch(d.sets) as b:
-        best = b.tune(stream, repeats=1)
+        best = b.tune(stream, repeats=1, solve_only=solve_only)
         # the tuner runs on scratch right-hand sides
         np.testing.assert_array_equal(d.result(), d.x)
         info = b.info()
