@@ -453,6 +453,62 @@ def batch_traffic_probe(torch, pf) -> dict:
             "calibration": {"read_bytes": 8 * n, "write_bytes": 8 * n}}
 
 
+def batch_graph_probe(torch, pf) -> dict:
+    """For kernel traces of hipGraph against stream launches: the 1024-degree batch on queue
+    kernel 1 (table build launch + queue launch per step) and on kernel 4 (one launch per
+    step), each as 10 eager steps, 10 replays of a one-step graph and one replay of a 10-step
+    graph, the phases separated by small torch fill kernels so a trace can be cut into them."""
+    lib = pf.lib()
+    sets, mats, nrhs = harmonic_batch_sets(lib, 0, 0)
+    dmat = torch.from_numpy(mats).cuda()
+    x = torch.rand(nrhs, dtype=torch.float64, device="cuda") * 2 - 1
+    for st in sets:
+        st["matrices"] = dmat.data_ptr()
+        st["rhs"] += x.data_ptr()
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    out = {"probe": "batch graph"}
+
+    def timed(fn, k):
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(k + 1)]
+        with torch.cuda.stream(stream):
+            for i in range(k):
+                evs[i].record(stream)
+                fn()
+            evs[k].record(stream)
+        stream.synchronize()
+        return float(evs[0].elapsed_time(evs[k]) / k) * 1e3
+
+    def marker(v):
+        torch.cuda.synchronize()
+        torch.full((1,), float(v), device="cuda")
+        torch.cuda.synchronize()
+
+    for kern in (1, 4):
+        t = pf._abi.PfSolveBatchTuning()
+        for c in range(pf._abi.BATCH_CLASSES):
+            t.kernel[c] = kern
+            t.waves[c] = 5 if c == 3 else 0
+        with pf.Batch(sets, t) as b:
+            b.capture_graph(sp)
+            for _ in range(3):
+                b.run(sp)
+                b.launch_graph(sp)
+            res = {}
+            marker(1)
+            res["eager_us"] = timed(lambda: b.run(sp), 10)
+            marker(2)
+            res["graph_1_step_us"] = timed(lambda: b.launch_graph(sp), 10)
+            b.capture_graph(sp, steps=10)
+            b.launch_graph(sp)
+            marker(3)
+            res["graph_10_steps_us_per_step"] = timed(lambda: b.launch_graph(sp), 1) / 10
+            marker(4)
+            b.sync(sp)
+            out[f"kernel{kern}"] = res
+    return out
+
+
 def to_dd(q):
     """exact rational -> (hi, lo) double-double, round to nearest"""
     from fractions import Fraction
@@ -894,6 +950,8 @@ def main() -> int:
                     help="length of the serving stream (60 = the full stream; 0 skips it)")
     ap.add_argument("--no-stream-reference", action="store_true",
                     help="skip timing torch fill_ / sum / copy_ / in-place mul_ over 1 GiB (practical ceilings)")
+    ap.add_argument("--batch-graph-probe", action="store_true",
+                    help="kernel-trace probe: batch steps eager, as one-step and as ten-step graphs")
     ap.add_argument("--batch-traffic-probe", action="store_true",
                     help="only run the 1024-degree batch solve three times and 1 GiB torch read / write "
                          "kernels (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE), print one JSON line, exit")
@@ -913,6 +971,9 @@ def main() -> int:
         print("no GPU available", file=sys.stderr)
         return 1
     torch.cuda.init()
+    if args.batch_graph_probe:
+        print(json.dumps(batch_graph_probe(torch, pf)))
+        return 0
     if args.batch_traffic_probe:
         print(json.dumps(batch_traffic_probe(torch, pf)))
         return 0
