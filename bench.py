@@ -119,6 +119,44 @@ def time_graph_replays(torch, stream, warmup: int, k: int, step, per_graph: int 
     return e0.elapsed_time(e1) / (k * per_graph)
 
 
+def time_hip_graph_replays(torch, stream, warmup: int, k: int, step, per_graph: int = 1) -> float:
+    """As time_graph_replays, with the graph captured, instantiated and launched through
+    HIP's own API (hipStreamBeginCapture / hipGraphInstantiate / hipGraphLaunch, what a C
+    caller does) rather than torch's graph wrapper: ms per step."""
+    hip = C.CDLL("libamdhip64.so")
+    vp = C.c_void_p
+    hip.hipStreamBeginCapture.argtypes = [vp, C.c_int]
+    hip.hipStreamEndCapture.argtypes = [vp, C.POINTER(vp)]
+    hip.hipGraphInstantiate.argtypes = [C.POINTER(vp), vp, vp, vp, C.c_size_t]
+    hip.hipGraphLaunch.argtypes = [vp, vp]
+    hip.hipGraphExecDestroy.argtypes = [vp]
+    hip.hipGraphDestroy.argtypes = [vp]
+    s = vp(stream.cuda_stream)
+    stream.synchronize()
+    assert hip.hipStreamBeginCapture(s, 2) == 0   # relaxed capture mode
+    for _ in range(per_graph):
+        step()
+    graph, exe = vp(), vp()
+    assert hip.hipStreamEndCapture(s, C.byref(graph)) == 0
+    assert hip.hipGraphInstantiate(C.byref(exe), graph, None, None, 0) == 0
+    try:
+        for _ in range(warmup):
+            assert hip.hipGraphLaunch(exe, s) == 0
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            e0.record(stream)
+            for _ in range(k):
+                assert hip.hipGraphLaunch(exe, s) == 0
+            e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / (k * per_graph)
+    finally:
+        hip.hipGraphExecDestroy(exe)
+        hip.hipGraphDestroy(graph)
+
+
 def kernel_window_ms(torch, lib, plan, stream, k: int, launch) -> float:
     """Average duration of the plan's main kernel over k back-to-back launches on
     `stream`: the library records one HIP event just before the first kernel and
@@ -1015,8 +1053,11 @@ def main() -> int:
     ms_step, steps_ms, wall, kavg = time_plan_steps(torch, lib, plan, stream, args.warmup, K,
                                                     lambda: plan.append(**launch))
     assert torch.isfinite(y).all().item(), "non-finite output"
-    graph_ms = time_graph_replays(torch, stream, args.warmup, K, lambda: plan.append(**launch))
-    graph10_ms = time_graph_replays(torch, stream, 2, max(2, K // 10), lambda: plan.append(**launch), per_graph=10)
+    graph_ms = time_hip_graph_replays(torch, stream, args.warmup, K, lambda: plan.append(**launch))
+    graph10_ms = time_hip_graph_replays(torch, stream, 2, max(2, K // 10), lambda: plan.append(**launch),
+                                        per_graph=10)
+    torch_graph10_ms = time_graph_replays(torch, stream, 2, max(2, K // 10), lambda: plan.append(**launch),
+                                          per_graph=10)
     plan.close()
     rows_read = ZP[1] - ZP[0]
     alg = algorithmic_bytes(N, nreal, rows_read, N)
@@ -1074,6 +1115,7 @@ def main() -> int:
         "p99_ms": float(np.percentile(steps_ms, 99)),
         "graph_ms_per_step": graph_ms,
         "graph_ms_per_step_10_per_graph": graph10_ms,
+        "torch_graph_ms_per_step_10_per_graph": torch_graph10_ms,
         "device_warm_s": args.warm_seconds,
         "wall_s": wall,
         "roofline": {
