@@ -138,7 +138,10 @@ typedef struct pfsolve_batch_tuning {
                                                    >= 4 wavefronts; no table build launch and
                                                    no table through HBM) */
     int32_t order;            /* 0: largest N first, 1: sets as given */
-    int32_t pad;
+    int32_t systems_per_wave; /* R = 2 class on queue kernel 1: systems one wavefront solves
+                                 per pass, 2 (a pair; 0 = 2), 4 (two groups of 32 lanes, for
+                                 sets with N <= 64) or 8 (four groups of 16 lanes, N <= 32;
+                                 sets with 32 < N <= 64 then use 4) */
 } pfsolve_batch_tuning;
 
 typedef struct pfsolve_batch_info {

@@ -312,7 +312,7 @@ class PfSolveBatchTuning(C.Structure):
         ("rows_per_lane_shift", C.c_int32 * BATCH_CLASSES),
         ("kernel", C.c_int32 * BATCH_CLASSES),
         ("order", C.c_int32),
-        ("pad", C.c_int32),
+        ("systems_per_wave", C.c_int32),
     ]
 
 

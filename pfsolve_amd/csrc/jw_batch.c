@@ -55,7 +55,7 @@ struct pfsolve_batch {
 /* ---- tuning table (process-wide) ---- */
 typedef struct tune_entry {
     int cls, bucket;
-    int waves, unit_pairs, grid_per_cu, shift, order, kernel;
+    int waves, unit_pairs, grid_per_cu, shift, order, kernel, spw;
 } tune_entry;
 static tune_entry* g_tune;
 static int g_ntune;
@@ -67,6 +67,9 @@ static int pairs_bucket(int64_t pairs)
     return b;
 }
 
+/* systems per wavefront pass of the R = 2 class: 0 (default), 2, 4 or 8 */
+static int spw_ok(int spw) { return spw == 0 || spw == 2 || spw == 4 || spw == 8; }
+
 static void tune_put(int cls, int bucket, const pfsolve_batch_tuning* t)
 {
     for (int i = 0; i < g_ntune; i++) {
@@ -77,6 +80,7 @@ static void tune_put(int cls, int bucket, const pfsolve_batch_tuning* t)
             g_tune[i].shift = t->rows_per_lane_shift[cls];
             g_tune[i].order = t->order;
             g_tune[i].kernel = t->kernel[cls];
+            g_tune[i].spw = t->systems_per_wave;
             return;
         }
     }
@@ -84,7 +88,7 @@ static void tune_put(int cls, int bucket, const pfsolve_batch_tuning* t)
     if (!n) return;
     g_tune = n;
     tune_entry e = {cls, bucket, t->waves[cls], t->unit_pairs[cls], t->grid_per_cu[cls], t->rows_per_lane_shift[cls],
-                    t->order, t->kernel[cls]};
+                    t->order, t->kernel[cls], t->systems_per_wave};
     g_tune[g_ntune++] = e;
 }
 
@@ -113,11 +117,11 @@ PfSolveResult pfsolve_tuning_save(const char* path)
     FILE* f = fopen(path, "w");
     if (!f) return PFSOLVE_ERROR_EMPTY_FILE;
     fprintf(f, "# pfsolve_amd batch tuning: class rows_per_lane pairs_log2 waves unit_pairs grid_per_cu shift order "
-               "kernel\n");
+               "kernel systems_per_wave\n");
     for (int i = 0; i < g_ntune; i++)
-        fprintf(f, "%d %d %d %d %d %d %d %d %d\n", g_tune[i].cls, 2 << g_tune[i].cls, g_tune[i].bucket,
+        fprintf(f, "%d %d %d %d %d %d %d %d %d %d\n", g_tune[i].cls, 2 << g_tune[i].cls, g_tune[i].bucket,
                 g_tune[i].waves, g_tune[i].unit_pairs, g_tune[i].grid_per_cu, g_tune[i].shift, g_tune[i].order,
-                g_tune[i].kernel);
+                g_tune[i].kernel, g_tune[i].spw);
     fclose(f);
     return PFSOLVE_SUCCESS;
 }
@@ -131,11 +135,11 @@ PfSolveResult pfsolve_tuning_load(const char* path)
     int bad = 0;
     while (fgets(line, sizeof(line), f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
-        /* the kernel column is optional (0: default) */
-        int c, r, b, w, u, g, s, o, k = 0;
-        const int nf = sscanf(line, "%d %d %d %d %d %d %d %d %d", &c, &r, &b, &w, &u, &g, &s, &o, &k);
-        if ((nf != 8 && nf != 9) || c < 0 || c >= NCLS || r != (2 << c) || w < 0 || u < 0 || g < 0 || s < 0 ||
-            s > 1 || o < 0 || o > 1 || k < 0 || k > NKERNELS) {
+        /* the kernel and systems-per-wavefront columns are optional (0: default) */
+        int c, r, b, w, u, g, s, o, k = 0, spw = 0;
+        const int nf = sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &c, &r, &b, &w, &u, &g, &s, &o, &k, &spw);
+        if ((nf < 8 || nf > 10) || c < 0 || c >= NCLS || r != (2 << c) || w < 0 || u < 0 || g < 0 || s < 0 ||
+            s > 1 || o < 0 || o > 1 || k < 0 || k > NKERNELS || !spw_ok(spw)) {
             bad = 1;
             continue;
         }
@@ -147,6 +151,7 @@ PfSolveResult pfsolve_tuning_load(const char* path)
         t.rows_per_lane_shift[c] = s;
         t.order = o;
         t.kernel[c] = k;
+        t.systems_per_wave = spw;
         tune_put(c, b, &t);
     }
     fclose(f);
@@ -227,7 +232,12 @@ static int group_waves_default(int g)
  * direct I/O measured slower (harmonic batch, solve only: 256-295 us against
  * 213-227 us: a lane's 16-byte accesses 128 bytes apart reach memory less well than
  * the staged, contiguous ones), so it is a tuner candidate only. */
-static int group_kernel_default(int g) { return g <= 2 ? 2 : 1; }
+static int group_kernel_default(int g) { return g == 0 ? 1 : (g <= 2 ? 2 : 1); }
+
+/* systems per wavefront pass of the R = 2 class on queue kernel 1 when not tuned: four
+ * 16-lane groups for sets with N <= 32, two 32-lane groups for N <= 64 (measured on small-N
+ * batches: 1.5 - 2 x the pair kernel's rate) */
+static int spw_effective(const pfsolve_batch* b) { return b->tuning.systems_per_wave ? b->tuning.systems_per_wave : 8; }
 
 static int group_kernel_ok(int g, int k) { return k == 1 || ((k == 2 || k == 3 || k == 4) && g <= 3); }
 
@@ -244,6 +254,16 @@ static int cmp_n_desc(const void* a, const void* c)
     return x->idx < y->idx ? -1 : (x->idx > y->idx);
 }
 
+/* passes per queue unit of set s (`pairs` passes in all): the tuned size, else `defup` when
+ * set (and smaller than the set), else the whole set */
+static int64_t unit_passes(const pfsolve_batch* b, int64_t s, int64_t pairs, int64_t defup)
+{
+    const int64_t tuned = b->tuning.unit_pairs[b->cls[s]];
+    if (tuned > 0) return tuned;
+    if (defup > 0 && defup < pairs) return defup;
+    return pairs > 0 ? pairs : 1;
+}
+
 /* fills the units (host, grouped by effective row class, queue order inside a
  * group) and the launch shapes from b->tuning */
 static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
@@ -256,14 +276,27 @@ static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
     if (b->tuning.order == 0 && S > 1) qsort(keys, (size_t)S, sizeof(n_index), cmp_n_desc);
     for (int64_t i = 0; i < S; i++) order[i] = keys[i].idx;
     free(keys);
+    /* untuned unit size: whole sets, unless a group has fewer sets than 4 units per CU; then
+     * units of at least 16 passes that give each CU about 4 of them */
+    int64_t nsets_g[NCLS] = {0}, passes_g[NCLS] = {0}, defup[NCLS] = {0};
+    for (int64_t s = 0; s < S; s++) {
+        nsets_g[eff_group(b, s)]++;
+        passes_g[eff_group(b, s)] += set_passes(b, s);
+    }
+    for (int g = 0; g < NCLS; g++) {
+        const int64_t target = 4 * (int64_t)(b->cu_count > 0 ? b->cu_count : 1);
+        if (nsets_g[g] < target) {
+            defup[g] = (passes_g[g] + target - 1) / target;
+            if (defup[g] < 16) defup[g] = 16;
+        }
+    }
     /* count units per group */
     int64_t cnt[NCLS] = {0};
     for (int64_t k = 0; k < S; k++) {
         const int64_t s = order[k];
         const int g = eff_group(b, s);
         const int64_t pairs = set_passes(b, s);
-        int64_t up = b->tuning.unit_pairs[b->cls[s]];
-        if (up <= 0) up = pairs > 0 ? pairs : 1;
+        const int64_t up = unit_passes(b, s, pairs, defup[g]);
         cnt[g] += (pairs + up - 1) / up;
     }
     int64_t total = 0;
@@ -281,8 +314,7 @@ static PfSolveResult build_queue(pfsolve_batch* b, jw_batch_unit** out_units)
         const int64_t s = order[k];
         const int g = eff_group(b, s);
         const int64_t pairs = set_passes(b, s);
-        int64_t up = b->tuning.unit_pairs[b->cls[s]];
-        if (up <= 0) up = pairs > 0 ? pairs : 1;
+        const int64_t up = unit_passes(b, s, pairs, defup[g]);
         for (int64_t p = 0; p < pairs; p += up) {
             jw_batch_unit* x = &u[fill[g]++];
             x->item = (int32_t)s;
@@ -483,6 +515,7 @@ PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set*
             b->tuning.rows_per_lane_shift[c] = e->shift;
             b->tuning.kernel[c] = e->kernel;
             b->tuning.order = e->order;
+            if (c == 0) b->tuning.systems_per_wave = e->spw;
         }
     }
     for (int c = NSHIFT; c < NCLS; c++) b->tuning.rows_per_lane_shift[c] = 0;
@@ -500,7 +533,7 @@ PfSolveResult pfsolve_batch_set_tuning(pfsolve_batch* b, const pfsolve_batch_tun
         if (t->waves[c] < 0 || t->unit_pairs[c] < 0 || t->grid_per_cu[c] < 0 || t->rows_per_lane_shift[c] < 0 ||
             t->rows_per_lane_shift[c] > 1 || t->kernel[c] < 0 || t->kernel[c] > NKERNELS)
             return PFSOLVE_ERROR_EMPTY_size;
-    if (t->order < 0 || t->order > 1) return PFSOLVE_ERROR_EMPTY_size;
+    if (t->order < 0 || t->order > 1 || !spw_ok(t->systems_per_wave)) return PFSOLVE_ERROR_EMPTY_size;
     b->tuning = *t;
     for (int c = NSHIFT; c < NCLS; c++) b->tuning.rows_per_lane_shift[c] = 0;
     return upload_queue(b);
@@ -533,7 +566,8 @@ static PfSolveResult launch_step(pfsolve_batch* b, const jw_batch_item* d_items,
                        : k == 3 ? jwdev_batch64d(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
                                                  b->d_counters + 2 * g, 2 << g, b->group_waves[g], b->group_grid[g], stream)
                                 : jwdev_batch64(d_items, b->d_units + b->unit_first[g], (int)b->unit_count[g],
-                                                b->d_counters + 2 * g, 2 << g, b->group_waves[g], b->group_grid[g], stream);
+                                                b->d_counters + 2 * g, 2 << g, b->group_waves[g], b->group_grid[g],
+                                                spw_effective(b), stream);
         if (rc != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     }
     return PFSOLVE_SUCCESS;
@@ -824,6 +858,32 @@ PfSolveResult pfsolve_batch_tune_mode(pfsolve_batch* b, hipStream_t stream, int 
                 }
             }
             cur = best;
+        }
+        if (res == PFSOLVE_SUCCESS) {
+            /* systems per wavefront pass of the R = 2 class on queue kernel 1 (sub-wavefront
+             * groups for its sets with N <= 64) */
+            b->tuning = cur;
+            res = upload_queue(b);
+            int small = 0;
+            for (int64_t s = 0; s < b->nsets && res == PFSOLVE_SUCCESS; s++)
+                if (b->sets[s].N <= 64 && b->sets[s].nsys > 0 && eff_group(b, s) == 0) small = 1;
+            if (res == PFSOLVE_SUCCESS && small && b->group_kernel[0] == 1) {
+                static const int spw_try[] = {2, 4, 8};
+                float best_ms = 1e30f;
+                int best_spw = cur.systems_per_wave;
+                for (int i = 0; i < 3 && res == PFSOLVE_SUCCESS; i++) {
+                    pfsolve_batch_tuning t = cur;
+                    t.systems_per_wave = spw_try[i];
+                    b->tuning = t;
+                    float ms = 0.f;
+                    if (time_group(b, d_items, stream, -1, repeats, solve_only, &ms) != 0) {
+                        res = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+                        break;
+                    }
+                    if (ms < best_ms) { best_ms = ms; best_spw = spw_try[i]; }
+                }
+                cur.systems_per_wave = best_spw;
+            }
         }
         if (res == PFSOLVE_SUCCESS) {
             /* queue order: largest N first vs as given, on the whole step */

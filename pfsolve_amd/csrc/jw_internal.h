@@ -203,8 +203,10 @@ int jwdev_main64s(const jw_run_desc* r, const jw_source_desc* src, const void* m
 /* persistent uneven-batch kernel over a unit queue; *d_counter must be 0 at launch */
 int jwdev_batch64_occupancy(int R, int waves);
 int jwdev_zero_counters(int* d_counters, int n, void* stream);
+/* spw (R = 2 only): systems per wavefront pass, 2 (pairs), 4 (two 32-lane groups for sets
+ * with N <= 64) or 8 (four 16-lane groups for N <= 32, two groups for 32 < N <= 64) */
 int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
-                  int waves, int grid, void* stream);
+                  int waves, int grid, int spw, void* stream);
 /* the same queue with direct I/O (R <= 16): LDS holds the table only, every wavefront
  * loads one pass ahead and stores from registers; up to 8 wavefronts per workgroup */
 int jwdev_batch64d_occupancy(int R, int waves);

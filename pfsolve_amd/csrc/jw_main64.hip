@@ -237,11 +237,24 @@ __device__ __forceinline__ void read_edges(int lane, const jw_pass64& g, int64_t
 
 // banded matvec, the three recurrences and row-0 scaleC, in registers.
 // ALL3: all three recurrences present (the fused step), decided at compile time.
-template <int R, bool ALL3>
+// G > 1 (R = 2 only): sub-wavefront groups.  The 64 lanes form G groups of LG = 64 / G
+// lanes, each solving its own pair of systems of N <= 2 LG rows; lane t is lane t mod LG of
+// its group for the table (the set's 64-lane table, whose lanes from N / 2 on are zero
+// rows).  What crosses lanes is cut at the group boundaries: the matvec's neighbour rows
+// and the recurrences' carries are 0 at a group's ends, and the scans' cross-row stages
+// run inside a group only (G = 2: the row_bcast15 stage and the readlane of the group's
+// second row start; G = 4: none).  The stages left out are the ones that, for a single
+// system of N <= 2 LG rows, only move its zero rows, so a group computes what the pair
+// kernel computes for the same pair.
+template <int R, bool ALL3, int G = 1>
 __device__ __forceinline__ void compute_pair(const double* tab, int lane, const jw_pass64& g, double (&x0)[R],
                                              double (&x1)[R])
 {
+    static_assert(G == 1 || (R == 2 && (G == 2 || G == 4)), "sub-wavefront groups: R = 2, G = 2 or 4");
     constexpr int SC = 8 * R * 64;            // start of the scan multipliers
+    constexpr int LG = 64 / G;
+    const int tl = G == 1 ? lane : (lane & (LG - 1));   // lane within the group: the table's lane
+    const bool gfirst = G > 1 && tl == 0, glast = G > 1 && tl == LG - 1;
     // ---- 2. banded matvec (coefficients carry the folded row scaling) ----
     {
         double m1a = 0.0, m2a = 0.0, p1a = 0.0, p2a = 0.0;
@@ -250,11 +263,13 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
         if (g.kl >= 2) { m2a = dpp_d<DPP_WAVE_SHR1>(x0[R - 2]); m2b = dpp_d<DPP_WAVE_SHR1>(x1[R - 2]); }
         if (g.ku >= 1) { p1a = dpp_d<DPP_WAVE_SHL1>(x0[0]); p1b = dpp_d<DPP_WAVE_SHL1>(x1[0]); }
         if (g.ku >= 2) { p2a = dpp_d<DPP_WAVE_SHL1>(x0[1]); p2b = dpp_d<DPP_WAVE_SHL1>(x1[1]); }
+        if (gfirst) { m1a = 0.0; m1b = 0.0; m2a = 0.0; m2b = 0.0; }
+        if (glast) { p1a = 0.0; p1b = 0.0; p2a = 0.0; p2b = 0.0; }
 #pragma unroll
         for (int i = 0; i < R; i++) {
-            const double2 c0 = *reinterpret_cast<const double2*>(tab + tab_coef64(0, i, lane, R));
-            const double2 c1 = *reinterpret_cast<const double2*>(tab + tab_coef64(1, i, lane, R));
-            const double vp2 = tab[tab_coef64(2, i, lane, R)];
+            const double2 c0 = *reinterpret_cast<const double2*>(tab + tab_coef64(0, i, tl, R));
+            const double2 c1 = *reinterpret_cast<const double2*>(tab + tab_coef64(1, i, tl, R));
+            const double vp2 = tab[tab_coef64(2, i, tl, R)];
             const double ia = x0[i], ib = x1[i];
             const double n1a = (i + 1 < R) ? x0[(i + 1) % R] : p1a;
             const double n1b = (i + 1 < R) ? x1[(i + 1) % R] : p1b;
@@ -284,7 +299,7 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
     if (ALL3 || g.fwd) {
         double lr[R];
 #pragma unroll
-        for (int i = 0; i < R; i++) lr[i] = tab[tab_coef64(2, i, lane, R) + 1];
+        for (int i = 0; i < R; i++) lr[i] = tab[tab_coef64(2, i, tl, R) + 1];
         double a = x0[0], b = x1[0];
 #pragma unroll
         for (int i = 1; i < R; i++) {
@@ -293,7 +308,7 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const double f = tab[SC + k * 64 + lane];
+            const double f = tab[SC + k * 64 + tl];
             double sa, sb;
             switch (k) {
             case 0: sa = dpp_d<DPP_ROW_SHR(1)>(a); sb = dpp_d<DPP_ROW_SHR(1)>(b); break;
@@ -304,17 +319,21 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
             a = __builtin_fma(f, sa, a);
             b = __builtin_fma(f, sb, b);
         }
-        {
+        if (G <= 2) {
             // rows 1 and 3 take the ends of rows 0 and 2; rows 2 and 3 then take lane 31
-            const double fp = tab[SC + 4 * 64 + lane], fp2 = tab[SC + 5 * 64 + lane];
+            const double fp = tab[SC + 4 * 64 + tl];
             const double sa = dpp_rm<DPP_ROW_BCAST15, 0xa>(a), sb = dpp_rm<DPP_ROW_BCAST15, 0xa>(b);
             a = __builtin_fma(fp, sa, a);
             b = __builtin_fma(fp, sb, b);
-            const double ta = dpp_rm<DPP_ROW_BCAST31, 0xc>(a), tb = dpp_rm<DPP_ROW_BCAST31, 0xc>(b);
-            a = __builtin_fma(fp2, ta, a);
-            b = __builtin_fma(fp2, tb, b);
+            if (G == 1) {
+                const double fp2 = tab[SC + 5 * 64 + tl];
+                const double ta = dpp_rm<DPP_ROW_BCAST31, 0xc>(a), tb = dpp_rm<DPP_ROW_BCAST31, 0xc>(b);
+                a = __builtin_fma(fp2, ta, a);
+                b = __builtin_fma(fp2, tb, b);
+            }
         }
         double wa = dpp_d<DPP_WAVE_SHR1>(a), wb = dpp_d<DPP_WAVE_SHR1>(b);
+        if (gfirst) { wa = 0.0; wb = 0.0; }
 #pragma unroll
         for (int i = 0; i < R; i++) {
             wa = __builtin_fma(-lr[i], wa, x0[i]);
@@ -333,7 +352,7 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
         if (!ALL3 && stage == 1 && !g.b2) continue;
         double hh[R];
 #pragma unroll
-        for (int i = 0; i < R; i++) hh[i] = tab[tab_coef64(3, i, lane, R) + stage];
+        for (int i = 0; i < R; i++) hh[i] = tab[tab_coef64(3, i, tl, R) + stage];
         const int j = 1 + stage;
         double a = x0[R - 1], b = x1[R - 1];
 #pragma unroll
@@ -343,7 +362,7 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const double f = tab[SC + (j * 8 + k) * 64 + lane];
+            const double f = tab[SC + (j * 8 + k) * 64 + tl];
             double sa, sb;
             switch (k) {
             case 0: sa = dpp_d<DPP_ROW_SHL(1)>(a); sb = dpp_d<DPP_ROW_SHL(1)>(b); break;
@@ -354,7 +373,7 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
             a = __builtin_fma(f, sa, a);
             b = __builtin_fma(f, sb, b);
         }
-        {
+        if (G == 1) {
             // every row adds the in-row results of the later row starts (lanes 16, 32, 48),
             // weighted by matrix-only multipliers (zero for rows at or after that start)
             const double c1 = tab[SC + (j * 8 + 4) * 64 + lane];
@@ -368,8 +387,17 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
             b = __builtin_fma(c2, b2, b);
             a = __builtin_fma(c3, a3, a);
             b = __builtin_fma(c3, b3, b);
+        } else if (G == 2) {
+            // the group's second row start: lane 16 of group 0, lane 48 of group 1
+            const double c1 = tab[SC + (j * 8 + 4) * 64 + tl];
+            const double a16 = readlane_d(a, 16), a48 = readlane_d(a, 48);
+            const double b16 = readlane_d(b, 16), b48 = readlane_d(b, 48);
+            const bool g0 = lane < 32;
+            a = __builtin_fma(c1, g0 ? a16 : a48, a);
+            b = __builtin_fma(c1, g0 ? b16 : b48, b);
         }
         double za = dpp_d<DPP_WAVE_SHL1>(a), zb = dpp_d<DPP_WAVE_SHL1>(b);
+        if (glast) { za = 0.0; zb = 0.0; }
 #pragma unroll
         for (int i = R - 1; i >= 0; i--) {
             za = __builtin_fma(-hh[i], za, x0[i]);
@@ -485,6 +513,78 @@ __device__ __forceinline__ void jw_pair64(const double* tab, double2* stg, int l
 }
 
 // A set's table (NTAB doubles, whole 1 KiB blocks) from global memory into LDS at `smem`
+// One pass of sub-wavefront groups (R = 2): the 2G consecutive systems sys0 .. sys0+2G-1
+// (those below sys_end) in one wavefront, group gi solving systems sys0 + 2 gi and
+// sys0 + 2 gi + 1.  Staging: system slot k (system sys0 + k) is chunks [k LG, (k+1) LG)
+// of the wavefront's 128 chunks (chunk m = rows 2m, 2m+1).  Every chunk is written each
+// pass (by the input DMA or as zeros), so the pass needs no zeroed staging.
+template <int G>
+__device__ __forceinline__ void jw_groups64(const double* tab, double2* stg, int lane, const jw_pass64& g,
+                                            int64_t sys0, int64_t sys_end)
+{
+    constexpr int LG = 64 / G;
+    const int tl = lane & (LG - 1), gi = lane / LG;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int c = 64 * j + lane;
+        const int64_t sys = sys0 + c / LG;
+        const int m = c & (LG - 1);
+        if (sys < sys_end && m >= g.icl && m < g.ich)
+            __builtin_amdgcn_global_load_lds(g.rhs + sys * g.stride + g.offSol + 2 * m,
+                                             (__attribute__((address_space(3))) void*)(stg + 64 * j), 16, 0, 0);
+        else
+            stg[c] = make_double2(0.0, 0.0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);   // the DMA and the zero writes have landed
+    compiler_fence();
+    const int64_t s0 = sys0 + 2 * gi, s1 = s0 + 1;
+    double x0[2], x1[2];
+    {
+        const double2 v0 = stg[2 * gi * LG + tl], v1 = stg[(2 * gi + 1) * LG + tl];
+        x0[0] = v0.x;
+        x0[1] = v0.y;
+        x1[0] = v1.x;
+        x1[1] = v1.y;
+    }
+    // the odd ends of the input keep-range, by their owning lanes
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const bool on = e == 0 ? g.in_edge_lo : g.in_edge_hi;
+        const int r = e == 0 ? g.in_lo : g.in_hi - 1;
+        if (on && tl == r / 2) {
+            const double v0 = s0 < sys_end ? g.rhs[s0 * g.stride + g.offSol + r] : 0.0;
+            const double v1 = s1 < sys_end ? g.rhs[s1 * g.stride + g.offSol + r] : 0.0;
+            if (r & 1) { x0[1] = v0; x1[1] = v1; } else { x0[0] = v0; x1[0] = v1; }
+        }
+    }
+    compute_pair<2, true, G>(tab, lane, g, x0, x1);
+    compiler_fence();
+    stg[2 * gi * LG + tl] = make_double2(x0[0], x0[1]);
+    stg[(2 * gi + 1) * LG + tl] = make_double2(x1[0], x1[1]);
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the results are in the staging
+    compiler_fence();
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int c = 64 * j + lane;
+        const int64_t sys = sys0 + c / LG;
+        const int m = c & (LG - 1);
+        const double2 v = stg[c];
+        if (sys < sys_end && m >= g.ocl && m < g.och)
+            store_stream(reinterpret_cast<double2*>(g.rhs + sys * g.stride + g.offSol) + m, v);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const bool on = e == 0 ? g.out_edge_lo : g.out_edge_hi;
+        const int r = e == 0 ? g.out_lo : g.out_hi - 1;
+        if (on && tl == r / 2) {
+            if (s0 < sys_end) g.rhs[s0 * g.stride + g.offSol + r] = (r & 1) ? x0[1] : x0[0];
+            if (s1 < sys_end) g.rhs[s1 * g.stride + g.offSol + r] = (r & 1) ? x1[1] : x1[0];
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): the staging is read before the next pass writes it
+    compiler_fence();
+}
+
 template <int NTAB>
 __device__ __forceinline__ void copy_table_lds(unsigned char* smem, const void* table, int wave, int nwaves, int lane)
 {
@@ -1063,7 +1163,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
 template <int R>
 __global__ __launch_bounds__(max_threads<R>()) void jw_batch64_kernel(const jw_batch_item* items,
                                                                       const jw_batch_unit* units, int nunits,
-                                                                      int* counter)
+                                                                      int* counter, int spw)
 {
     constexpr int NTAB = JW_TAB64_ELEMS(R);
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1087,6 +1187,22 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_batch64_kernel(const jw_b
             zero_staging<R>(stg, lane);   // the new set's keep-range may be narrower
             __syncthreads();
             cur = un.item;
+        }
+        if constexpr (R == 2) {
+            // sub-wavefront groups: G pairs (2G systems) per wavefront pass
+            const int G = spw >= 8 && it.N <= 32 ? 4 : (spw >= 4 && it.N <= 64 ? 2 : 1);
+            if (G > 1) {
+                const int tl = lane & (64 / G - 1);
+                const jw_pass64 g = make_pass64(reinterpret_cast<double*>(it.rhs), it.nsys, it.stride, it.offSol, 0,
+                                                it.N, 0, it.N, it.kl, it.ku, true, true, true, it.apply_scale != 0,
+                                                it.scale, tl);
+                const int64_t sys_end = it.nsys < 2 * (int64_t)un.pair1 ? it.nsys : 2 * (int64_t)un.pair1;
+                for (int q = un.pair0 + wave * G; q < un.pair1; q += nwaves * G) {
+                    if (G == 2) jw_groups64<2>(tab, stg, lane, g, 2 * (int64_t)q, sys_end);
+                    else jw_groups64<4>(tab, stg, lane, g, 2 * (int64_t)q, sys_end);
+                }
+                continue;
+            }
         }
         for (int q = un.pair0 + wave; q < un.pair1; q += nwaves) {
             const jw_pass64 g = make_pass64(reinterpret_cast<double*>(it.rhs), it.nsys, it.stride, it.offSol, 0,
@@ -1258,7 +1374,7 @@ __global__ __launch_bounds__(kBatchDirectThreads) void jw_batch64d_kernel(const 
 // The same queue for long systems (R = 32): one system per wavefront pass, table
 // read through a buffer resource; the unit ranges count systems, not pairs.
 __global__ __launch_bounds__(256) void jw_batch64g_kernel(const jw_batch_item* items, const jw_batch_unit* units,
-                                                          int nunits, int* counter)
+                                                          int nunits, int* counter, int /* systems per pass: 1 */)
 {
     constexpr int R = 32;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1645,7 +1761,7 @@ int jwdev_batch64_occupancy(int R, int waves)
 }
 
 int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, int nunits, int* d_counter, int R,
-                  int waves, int grid, void* stream)
+                  int waves, int grid, int spw, void* stream)
 {
     const void* f = pick_batch64(R);
     if (!f) { snprintf(g_err64, sizeof(g_err64), "no batch kernel for R=%d", R); return -1; }
@@ -1655,7 +1771,7 @@ int jwdev_batch64(const jw_batch_item* d_items, const jw_batch_unit* d_units, in
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -2;
     void* args[] = {const_cast<jw_batch_item**>(&d_items), const_cast<jw_batch_unit**>(&d_units), &nunits,
-                    &d_counter};
+                    &d_counter, &spw};
     e = hipLaunchKernel(f, dim3(grid), dim3(64 * waves), args, lds, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : -3;
 }

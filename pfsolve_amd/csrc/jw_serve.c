@@ -807,8 +807,10 @@ PfSolveResult pfsolve_serve(const pfsolve_serve_config* cfg, pfsolve_serve_reque
                     for (int c = 0; c < NCLS && !err; c++) {
                         if (!cls_n[c]) continue;
                         int64_t grid = cls_n[c] < grid_max[c] ? cls_n[c] : grid_max[c];
+                        /* R = 2: sub-wavefront groups for requests with N <= 64 (8 systems
+                         * per wavefront pass for N <= 32, 4 for N <= 64) */
                         if (jwdev_batch64(x->d_items, x->d_units + cls_first[c], (int)cls_n[c], x->d_counters + 2 * c,
-                                          2 << c, waves[c], (int)grid, x->stream) != 0)
+                                          2 << c, waves[c], (int)grid, 8, x->stream) != 0)
                             err = PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
                     }
                 }

@@ -167,6 +167,12 @@ def tuning(**kw):
     dict(kernel=[3, 3, 3, 3, 1], waves=4, unit_pairs=2),
     dict(kernel=3, waves=2, order=1, unit_pairs=1),
     dict(kernel=[3, 1, 3, 2, 1], rows_per_lane_shift=[1, 0, 1, 0, 0]),
+    dict(systems_per_wave=4),
+    dict(systems_per_wave=8),
+    dict(systems_per_wave=8, unit_pairs=1),
+    dict(systems_per_wave=4, unit_pairs=3, order=1),
+    dict(systems_per_wave=8, waves=2, kernel=[1, 2, 2, 2, 1]),
+    dict(systems_per_wave=4, kernel=4),
     dict(kernel=4),
     dict(kernel=[4, 4, 4, 4, 1], waves=4, unit_pairs=2),
     dict(kernel=4, waves=2, order=1, unit_pairs=1),
@@ -182,6 +188,25 @@ def test_batch_tunings(kw):
     d.check(got)
     if "rows_per_lane_shift" not in kw:
         np.testing.assert_array_equal(got, ref)
+
+
+# small systems for the sub-wavefront groups: N around the 16- and 32-lane group limits,
+# odd and even system counts, one set with more systems than a pass of every wavefront holds
+GROUP_SETS = [(2, 7, 1, 1.0), (5, 1, 0, 0.5), (16, 8, 1, 1.0), (31, 9, 1, 2.0), (32, 301, 1, 1.0),
+              (33, 15, 0, 1.0), (47, 16, 1, 0.75), (63, 17, 1, 1.0), (64, 33, 1, -1.0), (65, 3, 1, 1.0),
+              (128, 5, 1, 1.0), (300, 2, 1, 1.0)]
+
+
+@pytest.mark.parametrize("spw", [4, 8])
+@pytest.mark.parametrize("unit_pairs", [0, 1, 5])
+def test_batch_sub_wavefront_groups(spw, unit_pairs):
+    """2 or 4 groups of lanes per wavefront (4 or 8 systems per pass) on the R = 2 class:
+    the oracle's results, and bit for bit the pair kernel's"""
+    d = BatchData(GROUP_SETS, seed=11)
+    ref = run_batch(d, tuning(unit_pairs=unit_pairs))
+    got = run_batch(d, tuning(systems_per_wave=spw, unit_pairs=unit_pairs))
+    d.check(got)
+    np.testing.assert_array_equal(got, ref)
 
 
 def test_batch_split_kernel_time_out_is_reported_and_recovers():

@@ -54,14 +54,17 @@ def test_tuning_file_round_trip(tmp_path):
     L = pf.lib()
     L.pfsolve_tuning_clear()
     p = tmp_path / "t.txt"
-    # the kernel column (last) is optional: 0 when absent
-    p.write_text("# comment\n0 2 5 8 4 1 1 0\n3 16 10 5 0 1 0 1 2\n1 4 6 8 0 1 0 0 3\n2 8 7 12 0 1 0 0 4\n")
+    # the kernel and systems-per-wavefront columns (last two) are optional: 0 when absent
+    p.write_text("# comment\n0 2 5 8 4 1 1 0\n3 16 10 5 0 1 0 1 2\n1 4 6 8 0 1 0 0 3\n2 8 7 12 0 1 0 0 4\n"
+                 "0 2 9 16 0 1 0 0 1 8\n")
     assert L.pfsolve_tuning_load(str(p).encode()) == 0
     q = tmp_path / "u.txt"
     assert L.pfsolve_tuning_save(str(q).encode()) == 0
     lines = [l.split() for l in q.read_text().splitlines() if not l.startswith("#")]
-    assert lines == [["0", "2", "5", "8", "4", "1", "1", "0", "0"], ["3", "16", "10", "5", "0", "1", "0", "1", "2"],
-                     ["1", "4", "6", "8", "0", "1", "0", "0", "3"], ["2", "8", "7", "12", "0", "1", "0", "0", "4"]]
+    assert lines == [["0", "2", "5", "8", "4", "1", "1", "0", "0", "0"],
+                     ["3", "16", "10", "5", "0", "1", "0", "1", "2", "0"],
+                     ["1", "4", "6", "8", "0", "1", "0", "0", "3", "0"], ["2", "8", "7", "12", "0", "1", "0", "0", "4", "0"],
+                     ["0", "2", "9", "16", "0", "1", "0", "0", "1", "8"]]
     L.pfsolve_tuning_clear()
 
 
@@ -69,7 +72,8 @@ def test_tuning_file_rejects_bad_lines(tmp_path):
     L = pf.lib()
     L.pfsolve_tuning_clear()
     p = tmp_path / "bad.txt"
-    p.write_text("1 8 3 4 4 1 0 0\n0 2 1 1 1 1 2 0\n2 8 1 1 1 1 0 0 5\n")   # wrong R for class 1; shift 2; kernel 5
+    # wrong R for class 1; shift 2; kernel 5; 3 systems per wavefront
+    p.write_text("1 8 3 4 4 1 0 0\n0 2 1 1 1 1 2 0\n2 8 1 1 1 1 0 0 5\n0 2 1 1 1 1 0 0 1 3\n")
     assert L.pfsolve_tuning_load(str(p).encode()) == R_("PFSOLVE_ERROR_EMPTY_FILE")
     assert L.pfsolve_tuning_load(str(tmp_path / "missing.txt").encode()) == R_("PFSOLVE_ERROR_EMPTY_FILE")
     L.pfsolve_tuning_clear()
