@@ -168,6 +168,18 @@ template <typename T> __device__ __forceinline__ T mem_load(const void* base, in
     if (mem_f64) return Ar<T>::from(reinterpret_cast<const double*>(base)[idx], 0.0);
     return reinterpret_cast<const T*>(base)[idx];
 }
+// double-double without a branch on the storage: the two doubles of element idx (DD
+// storage), or the double at idx twice with the low word then 0 (FP64 storage).  A caller's
+// loads then stay in flight together (a branch around each load makes the compiler wait for
+// it before the join).
+template <> __device__ __forceinline__ dd mem_load<dd>(const void* base, int64_t idx, int mem_f64)
+{
+    const double* p = reinterpret_cast<const double*>(base);
+    const int64_t i = mem_f64 ? idx : 2 * idx;
+    const double h = p[i];
+    const double l = p[mem_f64 ? i : i + 1];
+    return dd_make(h, mem_f64 ? 0.0 : l);
+}
 template <> __device__ __forceinline__ double mem_load<double>(const void* base, int64_t idx, int)
 {
     return reinterpret_cast<const double*>(base)[idx];

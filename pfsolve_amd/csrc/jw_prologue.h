@@ -68,7 +68,11 @@ __host__ __device__ inline size_t prologue_lds_bytes(int RL, size_t elt)
 }
 
 // the band coefficients v_{-2..2} of row r of the matvec (zero outside the
-// matrix), P = storage position of row r in the permuted diagonals
+// matrix), P = storage position of row r in the permuted diagonals.  Every load is
+// issued before any value is used: positions outside the matrix read an in-bounds
+// neighbour (a clamped column, or the row's own position) and are zeroed by a select
+// afterwards, so no per-row branch separates the loads (the compiler waits for a load
+// before the join of any branch around it).
 template <typename T>
 __device__ __forceinline__ void row_band(const jw_source_desc& d, const void* mat, int64_t r, int64_t N, int64_t P,
                                          T (&v)[5])
@@ -83,22 +87,36 @@ __device__ __forceinline__ void row_band(const jw_source_desc& d, const void* ma
     case JW_BAND_DIAG:
         v[2] = mem_load<T>(mat, d.mdA + P, mf);
         break;
-    case JW_BAND_UPPER:
+    case JW_BAND_UPPER: {
+        const T u = mem_load<T>(mat, d.udA + P, mf);
         v[2] = d.unit_diag ? A::one() : mem_load<T>(mat, d.mdA + P, mf);
-        if (r + 1 < ncols) v[3] = mem_load<T>(mat, d.udA + P, mf);
+        v[3] = A::sel(r + 1 < ncols, u, A::zero());
         break;
-    case JW_BAND_LOWER:
+    }
+    case JW_BAND_LOWER: {
+        const T l = mem_load<T>(mat, d.ldA + P, mf);
         v[2] = d.unit_diag ? A::one() : mem_load<T>(mat, d.mdA + P, mf);
-        if (r >= 1 && r - 1 < ncols) v[1] = mem_load<T>(mat, d.ldA + P, mf);
+        v[1] = A::sel(r >= 1 && r - 1 < ncols, l, A::zero());
         break;
-    case JW_BAND_GB:
+    }
+    case JW_BAND_GB: {
+        T raw[5];
 #pragma unroll
         for (int off = -2; off <= 2; off++) {
             const int64_t col = r + off;
-            if (off < -d.KL || off > d.KU || col < 0 || col >= N) continue;
-            v[2 + off] = mem_load<T>(mat, d.offV + (int64_t)(d.KU - off) * N + col, mf);
+            const int64_t cc = col < 0 ? 0 : (col >= N ? N - 1 : col);
+            // band rows outside [-KL, KU] are not in the storage: read the main one
+            const int j = (off < -d.KL || off > d.KU) ? d.KU : d.KU - off;
+            raw[2 + off] = mem_load<T>(mat, d.offV + (int64_t)j * N + cc, mf);
+        }
+#pragma unroll
+        for (int off = -2; off <= 2; off++) {
+            const int64_t col = r + off;
+            const bool ok = off >= -d.KL && off <= d.KU && col >= 0 && col < N;
+            v[2 + off] = A::sel(ok, raw[2 + off], A::zero());
         }
         break;
+    }
     default:
         v[2] = A::one();
         break;
@@ -171,15 +189,20 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         for (int k = 0; k < 5; k++) vb[j][k] = A::zero();
         if (r < RL && gr < N) {
             const int64_t P = source_pos(d, gr, N);
-            row_band<T>(d, mat, gr, N, P, vb[j]);
-            if (has_f && gr >= 1) ldv = mem_load<T>(mat, d.ldS + P, mf);
-            if (has_b1 && gr + 1 < N) udv = mem_load<T>(mat, d.udS + P, mf);
+            // the solve diagonals' loads first (the band's products need its own loads),
+            // all at the row's position, zeroed by selects where the row has no entry
+            T ldr = A::zero(), udr = A::zero(), Dr = A::one(), Er = A::zero();
+            if (has_f) ldr = mem_load<T>(mat, d.ldS + P, mf);
+            if (has_b1) udr = mem_load<T>(mat, d.udS + P, mf);
             if (d.has_m) {
-                Dv = mem_load<T>(mat, d.mdM + P, mf);
-                if (gr + 1 < N) Ev = mem_load<T>(mat, d.udM + P, mf);
-            } else {
-                Dv = A::one();
+                Dr = mem_load<T>(mat, d.mdM + P, mf);
+                Er = mem_load<T>(mat, d.udM + P, mf);
             }
+            row_band<T>(d, mat, gr, N, P, vb[j]);
+            ldv = A::sel(gr >= 1, ldr, A::zero());
+            udv = A::sel(gr + 1 < N, udr, A::zero());
+            Dv = Dr;
+            Ev = A::sel(gr + 1 < N, Er, A::zero());
         }
         if (r < RL) {
             sl[r] = ldv;
