@@ -152,11 +152,13 @@ template <> struct Ar<dd> {
     __device__ static __forceinline__ dd from(double h, double l) { return dd_make(h, l); }
     __device__ static __forceinline__ dd add(dd a, dd b) { return dd_add(a, b); }
     __device__ static __forceinline__ dd sub(dd a, dd b) { return dd_sub(a, b); }
-    __device__ static __forceinline__ dd mul(dd a, dd b) { return dd_mul(a, b); }
+    // products and multiply-adds in the reference's forms (dd_mul_r, dd_madd_r); sums,
+    // differences and quotients in the accurate forms
+    __device__ static __forceinline__ dd mul(dd a, dd b) { return dd_mul_r(a, b); }
     __device__ static __forceinline__ dd div(dd a, dd b) { return dd_div(a, b); }
     __device__ static __forceinline__ dd neg(dd a) { return dd_neg(a); }
-    __device__ static __forceinline__ dd madd(dd a, dd b, dd c) { return dd_add(a, dd_mul(b, c)); }
-    __device__ static __forceinline__ dd msub(dd a, dd b, dd c) { return dd_sub(a, dd_mul(b, c)); }
+    __device__ static __forceinline__ dd madd(dd a, dd b, dd c) { return dd_madd_r(a, b, c); }
+    __device__ static __forceinline__ dd msub(dd a, dd b, dd c) { return dd_msub_r(a, b, c); }
     __device__ static __forceinline__ double hi(dd a) { return a.hi; }
     __device__ static __forceinline__ dd ldexp_(dd a, int e) { return dd_make(ldexp(a.hi, e), ldexp(a.lo, e)); }
     __device__ static __forceinline__ dd sel(bool c, dd a, dd b) { return c ? a : b; }
