@@ -67,12 +67,86 @@ __host__ __device__ inline size_t prologue_lds_bytes(int RL, size_t elt)
     return (4 * (size_t)RL + (RL > 16 ? (size_t)RL : 16)) * elt;
 }
 
-// the band coefficients v_{-2..2} of row r of the matvec (zero outside the
-// matrix), P = storage position of row r in the permuted diagonals.  Every load is
-// issued before any value is used: positions outside the matrix read an in-bounds
-// neighbour (a clamped column, or the row's own position) and are zeroed by a select
-// afterwards, so no per-row branch separates the loads (the compiler waits for a load
-// before the join of any branch around it).
+// The band coefficients v_{-2..2} of row r of the matvec (zero outside the matrix),
+// P = storage position of row r in the permuted diagonals, in two parts: row_band_load
+// issues the row's loads (positions outside the matrix read an in-bounds neighbour: a
+// clamped column, or the row's own position), row_band_finish zeroes what lies outside by
+// selects and applies the pre-scale.  No per-row branch separates the loads (the compiler
+// waits for a load before the join of any branch around it), and a caller can issue the
+// loads of several rows before finishing any.
+template <typename T>
+__device__ __forceinline__ void row_band_load(const jw_source_desc& d, const void* mat, int64_t r, int64_t N,
+                                              int64_t P, T (&raw)[6])
+{
+    using A = Ar<T>;
+    const int mf = d.mem_f64;
+#pragma unroll
+    for (int k = 0; k < 5; k++) raw[k] = A::zero();
+    switch (d.band_kind) {
+    case JW_BAND_DIAG:
+        raw[2] = mem_load<T>(mat, d.mdA + P, mf);
+        break;
+    case JW_BAND_UPPER:
+        raw[3] = mem_load<T>(mat, d.udA + P, mf);
+        raw[2] = d.unit_diag ? A::one() : mem_load<T>(mat, d.mdA + P, mf);
+        break;
+    case JW_BAND_LOWER:
+        raw[1] = mem_load<T>(mat, d.ldA + P, mf);
+        raw[2] = d.unit_diag ? A::one() : mem_load<T>(mat, d.mdA + P, mf);
+        break;
+    case JW_BAND_GB:
+#pragma unroll
+        for (int off = -2; off <= 2; off++) {
+            const int64_t col = r + off;
+            const int64_t cc = col < 0 ? 0 : (col >= N ? N - 1 : col);
+            // band rows outside [-KL, KU] are not in the storage: read the main one
+            const int j = (off < -d.KL || off > d.KU) ? d.KU : d.KU - off;
+            raw[2 + off] = mem_load<T>(mat, d.offV + (int64_t)j * N + cc, mf);
+        }
+        break;
+    default:
+        raw[2] = A::one();
+        break;
+    }
+    raw[5] = d.mdS >= 0 ? mem_load<T>(mat, d.mdS + P, mf) : A::one();
+}
+
+template <typename T>
+__device__ __forceinline__ void row_band_finish(const jw_source_desc& d, int64_t r, int64_t N, const T (&raw)[6],
+                                                T (&v)[5])
+{
+    using A = Ar<T>;
+    // columns of the matvec: N, or in a chained step the previous iteration's rows
+    const int64_t ncols = d.Ncols > 0 ? d.Ncols : N;
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[k] = raw[k];
+    switch (d.band_kind) {
+    case JW_BAND_UPPER:
+        v[3] = A::sel(r + 1 < ncols, raw[3], A::zero());
+        break;
+    case JW_BAND_LOWER:
+        v[1] = A::sel(r >= 1 && r - 1 < ncols, raw[1], A::zero());
+        break;
+    case JW_BAND_GB:
+#pragma unroll
+        for (int off = -2; off <= 2; off++) {
+            const int64_t col = r + off;
+            const bool ok = off >= -d.KL && off <= d.KU && col >= 0 && col < N;
+            v[2 + off] = A::sel(ok, raw[2 + off], A::zero());
+        }
+        break;
+    default:
+        break;
+    }
+    if (d.mdS >= 0) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) v[k] = A::mul(v[k], raw[5]);
+    }
+}
+
+// The same in one piece (loads, then selects and the pre-scale), for builds that finish
+// each row before loading the next (double-double: with the two-part form the compiler
+// keeps the rows' band arrays in scratch memory, measured 3x slower loads).
 template <typename T>
 __device__ __forceinline__ void row_band(const jw_source_desc& d, const void* mat, int64_t r, int64_t N, int64_t P,
                                          T (&v)[5])
@@ -180,6 +254,27 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     // band coefficients into registers (format 1: straight into the table); row
     // r = tid + j * nthr of thread tid
     T vb[kProRows][5];
+    // FP64: every row's loads first (rows outside the matrix read row 0's positions and
+    // are discarded), then the selects, products and LDS stores, so that a thread's rows
+    // cost one round of memory latency.  Double-double loads and finishes row by row (the
+    // rows' double-double loads at once need more registers than they save time).
+    constexpr bool PRE = sizeof(T) == 8;
+    T raw[PRE ? kProRows : 1][6], sr[PRE ? kProRows : 1][4];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int j = 0; j < kProRows; j++) {
+            const int r = tid + j * nthr;
+            const int64_t gr = rb + r;
+            if (j * nthr >= RL) continue;   // no thread has a row here (workgroup-uniform)
+            const int64_t gq = (r < RL && gr < N) ? gr : 0;
+            const int64_t P = source_pos(d, gq, N);
+            sr[j][0] = has_f ? mem_load<T>(mat, d.ldS + P, mf) : A::zero();
+            sr[j][1] = has_b1 ? mem_load<T>(mat, d.udS + P, mf) : A::zero();
+            sr[j][2] = d.has_m ? mem_load<T>(mat, d.mdM + P, mf) : A::one();
+            sr[j][3] = d.has_m ? mem_load<T>(mat, d.udM + P, mf) : A::zero();
+            row_band_load<T>(d, mat, gq, N, P, raw[j]);
+        }
+    }
 #pragma unroll
     for (int j = 0; j < kProRows; j++) {
         const int r = tid + j * nthr;
@@ -188,21 +283,30 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
 #pragma unroll
         for (int k = 0; k < 5; k++) vb[j][k] = A::zero();
         if (r < RL && gr < N) {
-            const int64_t P = source_pos(d, gr, N);
-            // the solve diagonals' loads first (the band's products need its own loads),
-            // all at the row's position, zeroed by selects where the row has no entry
-            T ldr = A::zero(), udr = A::zero(), Dr = A::one(), Er = A::zero();
-            if (has_f) ldr = mem_load<T>(mat, d.ldS + P, mf);
-            if (has_b1) udr = mem_load<T>(mat, d.udS + P, mf);
-            if (d.has_m) {
-                Dr = mem_load<T>(mat, d.mdM + P, mf);
-                Er = mem_load<T>(mat, d.udM + P, mf);
+            if constexpr (PRE) {
+                row_band_finish<T>(d, gr, N, raw[j], vb[j]);
+                ldv = A::sel(gr >= 1, sr[j][0], A::zero());
+                udv = A::sel(gr + 1 < N, sr[j][1], A::zero());
+                Dv = sr[j][2];
+                Ev = A::sel(gr + 1 < N, sr[j][3], A::zero());
+            } else {
+                const int64_t P = source_pos(d, gr, N);
+                // the solve diagonals' loads first (the band's products need its own
+                // loads), all at the row's position, zeroed by selects where the row has no
+                // entry
+                T ldr = A::zero(), udr = A::zero(), Dr = A::one(), Er = A::zero();
+                if (has_f) ldr = mem_load<T>(mat, d.ldS + P, mf);
+                if (has_b1) udr = mem_load<T>(mat, d.udS + P, mf);
+                if (d.has_m) {
+                    Dr = mem_load<T>(mat, d.mdM + P, mf);
+                    Er = mem_load<T>(mat, d.udM + P, mf);
+                }
+                row_band<T>(d, mat, gr, N, P, vb[j]);
+                ldv = A::sel(gr >= 1, ldr, A::zero());
+                udv = A::sel(gr + 1 < N, udr, A::zero());
+                Dv = Dr;
+                Ev = A::sel(gr + 1 < N, Er, A::zero());
             }
-            row_band<T>(d, mat, gr, N, P, vb[j]);
-            ldv = A::sel(gr >= 1, ldr, A::zero());
-            udv = A::sel(gr + 1 < N, udr, A::zero());
-            Dv = Dr;
-            Ev = A::sel(gr + 1 < N, Er, A::zero());
         }
         if (r < RL) {
             sl[r] = ldv;
