@@ -157,7 +157,9 @@ typedef struct pfsolve_batch pfsolve_batch;
 
 /* Validates the sets, allocates the tables and the work queue (device memory
  * owned by the batch) and uploads the queue; synchronous.  tuning: NULL uses
- * the defaults, or the entry of a table loaded with pfsolve_tuning_load. */
+ * the defaults, or the entry of a table loaded with pfsolve_tuning_load.
+ * A batch's steps share its work queues and tables: steps of one batch must not
+ * overlap on the device (issue them on one stream, or order the streams). */
 PfSolveResult pfsolve_batch_create(pfsolve_batch** out, const pfsolve_batch_set* sets, int64_t nsets,
                                    const pfsolve_batch_tuning* tuning);
 PfSolveResult pfsolve_batch_run(pfsolve_batch* b, hipStream_t stream);
