@@ -4,6 +4,8 @@ one-plan-per-set form (same kernels, bit for bit), across tunings, through
 hipGraph replays and at the full 1024-degree workload."""
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -207,6 +209,27 @@ def test_batch_sub_wavefront_groups(spw, unit_pairs):
     got = run_batch(d, tuning(systems_per_wave=spw, unit_pairs=unit_pairs))
     d.check(got)
     np.testing.assert_array_equal(got, ref)
+
+
+def test_batch_refuses_bad_systems_per_wave_and_graph_steps():
+    """systems_per_wave must be 0, 2, 4 or 8 and a captured graph holds 1..1024 steps; a refused
+    call leaves the batch usable"""
+    t = torch()
+    d = BatchData(SMALL_SETS[:4], seed=12)
+    stream = t.cuda.current_stream().cuda_stream
+    L = pf.lib()
+    with pf.Batch(d.sets) as b:
+        for spw in (1, 3, 6, 16, -2):
+            assert L.pfsolve_batch_set_tuning(b.handle, C.byref(tuning(systems_per_wave=spw))) == \
+                _abi.PFSOLVE_ERROR_EMPTY_size
+        s = t.cuda.Stream()
+        for steps in (0, -1, 1025):
+            assert L.pfsolve_batch_capture_graph_steps(b.handle, s.cuda_stream, 0, steps) == \
+                _abi.PFSOLVE_ERROR_EMPTY_size
+        d.reset()
+        b.run(stream)
+        b.sync(stream)
+        d.check(d.result())
 
 
 def test_batch_split_kernel_time_out_is_reported_and_recovers():
