@@ -251,6 +251,17 @@ int jwdev_long_prologue(const jw_source_desc* src, const void* matrices, void* t
                         int precision, int* sync, void* gval, int* err, void* stream);
 int jwdev_long_chunks(const jw_long_desc* d, int precision, int grid_x, void* stream);
 int jwdev_long_reduced(const jw_long_desc* d, int precision, void* stream);
+/* long FP64 systems whose rows are 16-byte aligned: the chunk modes on the pair machinery
+ * (format-2 tables of JW_TAB64_ELEMS(8) elements per chunk, built by jwdev_long_prologue
+ * when given that table size); a persistent grid of cu_count * occupancy workgroups at most */
+int64_t jwdev_long64_lds_bytes(void);
+int jwdev_long64_occupancy(void);
+int jwdev_long64(const jw_long_desc* d, int cu_count, int occupancy, void* stream);
+/* the front of such a step in one launch: every chunk's table (format 2), COEF and REDUCE of
+ * the slice in d; sync holds 32 + builds ints and gval one 8-byte slot per build (both set
+ * by the call), err the device error flag (a bounded wait that ran out) */
+int jwdev_long64_front(const jw_source_desc* src, const void* matrices, const jw_long_desc* d, int* sync, void* gval,
+                       int* err, int cu_count, int occupancy, void* stream);
 int jwdev_block(const jw_block_desc* d, void* stream);
 int jwdev_gbmv(const jw_gbmv_desc* d, void* stream);
 int jwdev_device_info(int* cu_count, int* lds_per_block, int* arch_major, int* arch_minor);

@@ -71,50 +71,18 @@ constexpr int64_t long_tab_elems()
 // wait is bounded: when it runs out the device error flag is raised (PfSolveSync reports
 // it) and the build goes on with a unit pivot, so the grid always drains.
 // ------------------------------------------------------------------------
-constexpr int kLongMaxPolls = 1 << 22;
-
-template <typename T>
-struct ChainHooks {
-    T* carry;        // LDS: carry[0] = g entering the build
-    int* flags;      // per build: 1 once its last pivot is in gval
-    T* gval;         // per build: g of its last row
-    int* err;
-    int64_t build;
-    int chained;
-    __device__ void wait_carry(int tid) const
-    {
-        if (!chained || build == 0 || tid != 0) return;
-        int polls = 0;
-        bool ok = true;
-        while (__hip_atomic_load(&flags[build - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-            if (++polls > kLongMaxPolls) {
-                ok = false;
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        carry[0] = ok ? gval[build - 1] : Ar<T>::one();
-    }
-    __device__ void publish(int tid, const T* sg, int RL) const
-    {
-        if (!chained || tid != 0) return;
-        gval[build] = sg[RL - 1];
-        __hip_atomic_store(&flags[build], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-};
-
 template <typename T>
 __global__ __launch_bounds__(kPrologueThreads) void jw_long_prologue_kernel(jw_source_desc d, const void* mat,
                                                                           T* table, int chained, int* sync, T* gval,
-                                                                          int* err)
+                                                                          int* err, int format)
 {
     extern __shared__ __align__(16) unsigned char pro_smem[];
     __shared__ int s_build;
     constexpr int R = LongGeom<T>::R;
     constexpr int NSUB = LongGeom<T>::NSUB;
     constexpr int SR = NSUB * R * kLongL;
-    constexpr int64_t TE = long_tab_elems<T>();
+    // format 1 for the chunk kernel here, format 2 (FP64) for the pair kernel's chunks
+    const int64_t TE = format == 2 ? (int64_t)JW_TAB64_ELEMS(R) : long_tab_elems<T>();
     // the pivot carry follows the build's five row arrays
     T* carry = reinterpret_cast<T*>(pro_smem + prologue_lds_bytes(SR, sizeof(T)));
     int64_t build = blockIdx.x;
@@ -129,7 +97,7 @@ __global__ __launch_bounds__(kPrologueThreads) void jw_long_prologue_kernel(jw_s
     if (chained && threadIdx.x == 0)
         carry[1] = rb > 0 ? mem_load<T>(mat, d.udS + source_pos(d, rb - 1, d.N), d.mem_f64) : Ar<T>::zero();
     const ChainHooks<T> hooks{carry, sync + 1, gval, err, build, chained};
-    prologue_body<T, ChainHooks<T>>(d, mat, table + build * NSUB * TE, kLongL, R, 1, pro_smem, rb,
+    prologue_body<T, ChainHooks<T>>(d, mat, table + build * NSUB * TE, kLongL, R, format, pro_smem, rb,
                                     chained ? carry : nullptr, NSUB, hooks);
 }
 
@@ -615,7 +583,12 @@ int jwdev_long_prologue(const jw_source_desc* src, const void* matrices, void* t
 {
     if (!matrices || !table) return fail("long prologue: no matrices or table");
     const bool f64 = precision == JW_PREC_F64;
-    if (tab_elems != (f64 ? long_tab_elems<double>() : long_tab_elems<dd>()))
+    // the table size names the format: format 1 (the chunk kernel), or FP64 format 2 (the
+    // pair kernel's chunks, jw_long64_kernel)
+    int format = 1;
+    if (f64 && tab_elems == (int64_t)JW_TAB64_ELEMS(LongGeom<double>::R))
+        format = 2;
+    else if (tab_elems != (f64 ? long_tab_elems<double>() : long_tab_elems<dd>()))
         return fail("long prologue: table size mismatch");
     const int SR = f64 ? LongGeom<double>::NSUB * LongGeom<double>::R * kLongL
                        : LongGeom<dd>::NSUB * LongGeom<dd>::R * kLongL;
@@ -640,10 +613,11 @@ int jwdev_long_prologue(const jw_source_desc* src, const void* matrices, void* t
     const dim3 grid((unsigned)nit);
     if (f64)
         hipLaunchKernelGGL(jw_long_prologue_kernel<double>, grid, dim3(kPrologueThreads), lds, s, *src, matrices,
-                           reinterpret_cast<double*>(table), chained, sync, reinterpret_cast<double*>(gval), err);
+                           reinterpret_cast<double*>(table), chained, sync, reinterpret_cast<double*>(gval), err,
+                           format);
     else
         hipLaunchKernelGGL(jw_long_prologue_kernel<dd>, grid, dim3(kPrologueThreads), lds, s, *src, matrices,
-                           reinterpret_cast<dd*>(table), chained, sync, reinterpret_cast<dd*>(gval), err);
+                           reinterpret_cast<dd*>(table), chained, sync, reinterpret_cast<dd*>(gval), err, format);
     e = hipGetLastError();
     if (e != hipSuccess) return fail("long prologue launch", e);
     return 0;

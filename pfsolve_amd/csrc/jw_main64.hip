@@ -29,6 +29,8 @@
 #include "jw_common.h"
 #include "jw_prologue.h"
 
+extern "C" void jwdev_set_error(const char* msg);
+
 namespace {
 
 constexpr int kLdsBytesMax = 160 * 1024;
@@ -246,10 +248,22 @@ __device__ __forceinline__ void read_edges(int lane, const jw_pass64& g, int64_t
 // second row start; G = 4: none).  The stages left out are the ones that, for a single
 // system of N <= 2 LG rows, only move its zero rows, so a group computes what the pair
 // kernel computes for the same pair.
-template <int R, bool ALL3, int G = 1>
+// What a chunk of a long system (64 R of its rows, a pass of its own) takes from outside
+// and hands out, per system of the pair.  In: lane 0 holds the two rows before the chunk
+// (h1 = row -1, h2 = row -2) and the forward carry cw (W' of row -1); lane 63 holds the two
+// rows after it (h1 = row C, h2 = row C + 1) and the backward carries cz, cy (Z' and y of
+// row C); every other lane holds zeros.  Out: wl = W' of the lane's last row (the chunk's
+// last row on lane 63), zf = Z' of the lane's first row (the chunk's first row on lane 0).
+struct jw_edge64 {
+    double h1[2], h2[2], cw[2], cz[2], cy[2];
+    double wl[2], zf[2];
+};
+
+template <int R, bool ALL3, int G = 1, bool EDGE = false>
 __device__ __forceinline__ void compute_pair(const double* tab, int lane, const jw_pass64& g, double (&x0)[R],
-                                             double (&x1)[R])
+                                             double (&x1)[R], jw_edge64* e = nullptr)
 {
+    static_assert(!EDGE || G == 1, "chunk edges: whole wavefronts only");
     static_assert(G == 1 || (R == 2 && (G == 2 || G == 4)), "sub-wavefront groups: R = 2, G = 2 or 4");
     constexpr int SC = 8 * R * 64;            // start of the scan multipliers
     constexpr int LG = 64 / G;
@@ -265,6 +279,17 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
         if (g.ku >= 2) { p2a = dpp_d<DPP_WAVE_SHL1>(x0[1]); p2b = dpp_d<DPP_WAVE_SHL1>(x1[1]); }
         if (gfirst) { m1a = 0.0; m1b = 0.0; m2a = 0.0; m2b = 0.0; }
         if (glast) { p1a = 0.0; p1b = 0.0; p2a = 0.0; p2b = 0.0; }
+        if constexpr (EDGE) {
+            // the rows next to the chunk enter where the wave shifts bring zeros
+            if (lane == 0) {
+                if (g.kl >= 1) { m1a = e->h1[0]; m1b = e->h1[1]; }
+                if (g.kl >= 2) { m2a = e->h2[0]; m2b = e->h2[1]; }
+            }
+            if (lane == 63) {
+                if (g.ku >= 1) { p1a = e->h1[0]; p1b = e->h1[1]; }
+                if (g.ku >= 2) { p2a = e->h2[0]; p2b = e->h2[1]; }
+            }
+        }
 #pragma unroll
         for (int i = 0; i < R; i++) {
             const double2 c0 = *reinterpret_cast<const double2*>(tab + tab_coef64(0, i, tl, R));
@@ -300,6 +325,13 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
         double lr[R];
 #pragma unroll
         for (int i = 0; i < R; i++) lr[i] = tab[tab_coef64(2, i, tl, R) + 1];
+        if constexpr (EDGE) {
+            // W' entering the chunk: its first row becomes u'_0 - l'_0 cw
+            if (lane == 0) {
+                x0[0] = __builtin_fma(-lr[0], e->cw[0], x0[0]);
+                x1[0] = __builtin_fma(-lr[0], e->cw[1], x1[0]);
+            }
+        }
         double a = x0[0], b = x1[0];
 #pragma unroll
         for (int i = 1; i < R; i++) {
@@ -348,11 +380,23 @@ __device__ __forceinline__ void compute_pair(const double* tab, int lane, const 
 #pragma unroll
     for (int stage = 0; stage < 2; stage++) {
         if (R >= 16 && stage == 1) phase_barrier();
-            if (!ALL3 && stage == 0 && !g.b1) continue;
+        if constexpr (EDGE) {
+            if (stage == 0) { e->wl[0] = x0[R - 1]; e->wl[1] = x1[R - 1]; }
+            if (stage == 1) { e->zf[0] = x0[0]; e->zf[1] = x1[0]; }
+        }
+        if (!ALL3 && stage == 0 && !g.b1) continue;
         if (!ALL3 && stage == 1 && !g.b2) continue;
         double hh[R];
 #pragma unroll
         for (int i = 0; i < R; i++) hh[i] = tab[tab_coef64(3, i, tl, R) + stage];
+        if constexpr (EDGE) {
+            // Z' (stage 0) or y (stage 1) entering the chunk from the row after it
+            if (lane == 63) {
+                const double ca = stage == 0 ? e->cz[0] : e->cy[0], cb = stage == 0 ? e->cz[1] : e->cy[1];
+                x0[R - 1] = __builtin_fma(-hh[R - 1], ca, x0[R - 1]);
+                x1[R - 1] = __builtin_fma(-hh[R - 1], cb, x1[R - 1]);
+            }
+        }
         const int j = 1 + stage;
         double a = x0[R - 1], b = x1[R - 1];
 #pragma unroll
@@ -613,6 +657,348 @@ __global__ __launch_bounds__(max_threads<R>()) void jw_main64_kernel(jw_run_desc
                                         (int)d.in_lo, (int)d.in_hi, (int)d.out_lo, (int)d.out_hi, d.kl, d.ku,
                                         d.fwd, d.b1, d.b2, d.apply_scale, d.scale_hi, lane);
         jw_pair64<R, ALL3>(tab, stg, lane, g, 2 * q);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Long FP64 systems on the pair machinery (16-byte aligned systems).  Chunk c of a long
+// system (rows c C .. c C + C - 1, C = 512, R = 8) is one wavefront pass over a pair of
+// systems like a whole 512-row system, with the chunk's format-2 table (the scalings
+// continue across chunks) and the edges of jw_edge64.  The modes of the chunk
+// decomposition (jw_long.hip: COEF, REDUCE, FINISH, the reduced solve between REDUCE and
+// FINISH) run the same arithmetic, so the carries the reduced solve composes from COEF's
+// responses are the ones FINISH applies.
+// ---------------------------------------------------------------------------
+constexpr int kLong64R = 8;
+
+// reduced data of chunk c, system s: array a (the layout of jw_long_desc)
+__device__ __forceinline__ int64_t long_at(const jw_long_desc& d, int a, int64_t c, int64_t s)
+{
+    return ((int64_t)a * d.red_sys + s) * d.nchunks + c;
+}
+
+template <int MODE, bool ALL3>
+__device__ __forceinline__ void long64_pass(const double* tab, double2* stg, int lane, const jw_pass64& g,
+                                            const jw_long_desc& d, int64_t c, int64_t sys0)
+{
+    constexpr int R = kLong64R;
+    constexpr int C = 64 * R;
+    const jw_run_desc& rd = d.run;
+    const double* rhs = reinterpret_cast<const double*>(rd.rhs);
+    double* red = reinterpret_cast<double*>(d.red);
+    const int64_t rb = c * C;
+    jw_edge64 e;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        e.h1[s] = 0.0;
+        e.h2[s] = 0.0;
+        e.cw[s] = 0.0;
+        e.cz[s] = 0.0;
+        e.cy[s] = 0.0;
+    }
+    if constexpr (MODE != JW_LONG_COEF) dma_pair<R>(stg, lane, g, sys0);
+    // the edges, loaded while the DMA is in flight
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int64_t sys = sys0 + s;
+        if constexpr (MODE == JW_LONG_COEF) {
+            // unit carries, zero right-hand sides: system 0 takes w, 1 takes z, 2 takes y
+            if (lane == 0 && sys == 0) e.cw[s] = 1.0;
+            if (lane == 63 && sys == 1) e.cz[s] = 1.0;
+            if (lane == 63 && sys == 2) e.cy[s] = 1.0;
+        } else if (sys < g.nsys) {
+            const int64_t base = sys * rd.stride + rd.offSol;
+            if (rd.band) {
+                if constexpr (MODE == JW_LONG_REDUCE) {
+                    // rows next to the chunk, before any chunk is written: lane 0 rows rb-1,
+                    // rb-2, lane 63 rows rb+C, rb+C+1 (inside the input keep-range)
+                    const int64_t g1 = lane == 0 ? rb - 1 : rb + C;
+                    const int64_t g2 = lane == 0 ? rb - 2 : rb + C + 1;
+                    const bool edge = lane == 0 || lane == 63;
+                    if (edge && g1 >= rd.in_lo && g1 < rd.in_hi) e.h1[s] = rhs[base + g1];
+                    if (edge && g2 >= rd.in_lo && g2 < rd.in_hi) e.h2[s] = rhs[base + g2];
+                } else {
+                    // the same rows as REDUCE read them (the step runs in place)
+                    const bool lo = lane == 0 && c > 0, hi = lane == 63 && c + 1 < d.nchunks;
+                    if (lo || hi) {
+                        e.h1[s] = red[lo ? long_at(d, 6, c - 1, sys) : long_at(d, 3, c + 1, sys)];
+                        e.h2[s] = red[lo ? long_at(d, 5, c - 1, sys) : long_at(d, 4, c + 1, sys)];
+                    }
+                }
+            }
+            if constexpr (MODE == JW_LONG_FINISH) {
+                if (lane == 0) e.cw[s] = red[long_at(d, 7, c, sys)];
+                if (lane == 63) {
+                    e.cz[s] = red[long_at(d, 8, c, sys)];
+                    e.cy[s] = red[long_at(d, 9, c, sys)];
+                }
+            }
+        }
+    }
+    double x0[R], x1[R];
+    if constexpr (MODE == JW_LONG_COEF) {
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            x0[i] = 0.0;
+            x1[i] = 0.0;
+        }
+    } else {
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the DMA writes have landed in LDS
+        compiler_fence();
+        read_pair<R>(stg, lane, g, sys0, x0, x1);
+    }
+    if constexpr (MODE == JW_LONG_REDUCE) {
+        // the rows the neighbouring chunks' banded matvec needs, as read
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int64_t sys = sys0 + s;
+            if (sys >= g.nsys) continue;
+            if (lane == 0) {
+                red[long_at(d, 3, c, sys)] = s == 0 ? x0[0] : x1[0];
+                red[long_at(d, 4, c, sys)] = s == 0 ? x0[1] : x1[1];
+            }
+            if (lane == 63) {
+                red[long_at(d, 5, c, sys)] = s == 0 ? x0[R - 2] : x1[R - 2];
+                red[long_at(d, 6, c, sys)] = s == 0 ? x0[R - 1] : x1[R - 1];
+            }
+        }
+    }
+    compute_pair<R, ALL3, 1, true>(tab, lane, g, x0, x1, &e);
+    if constexpr (MODE == JW_LONG_FINISH) {
+        write_blocked<R>(stg, lane, x0, x1);
+        store_pair<R>(stg, lane, g, sys0);
+        rezero_input<R>(stg, lane, g);
+        __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): LDS traffic done before the next DMA lands
+        compiler_fence();
+    } else {
+        // boundary values: W' at the chunk's last row, Z' and y at its first
+        double* coef = red + (int64_t)JW_LONG_ARRAYS * d.nchunks * d.red_sys;
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int64_t sys = sys0 + s;
+            if (sys >= g.nsys) continue;
+            const double y0 = s == 0 ? x0[0] : x1[0];
+            if constexpr (MODE == JW_LONG_COEF) {
+                if (lane == 63) coef[9 * c + 3 * sys + 0] = e.wl[s];
+                if (lane == 0) {
+                    coef[9 * c + 3 * sys + 1] = e.zf[s];
+                    coef[9 * c + 3 * sys + 2] = y0;
+                }
+            } else {
+                if (lane == 63) red[long_at(d, 0, c, sys)] = e.wl[s];
+                if (lane == 0) {
+                    red[long_at(d, 1, c, sys)] = e.zf[s];
+                    red[long_at(d, 2, c, sys)] = y0;
+                }
+            }
+        }
+    }
+}
+
+// A persistent grid: workgroup b takes the units [b per_wg, (b + 1) per_wg) of the
+// (chunk, pair of systems) space in chunk order, stages each chunk's table in LDS once and
+// spreads the chunk's pairs over its wavefronts.
+template <int MODE, bool ALL3>
+__global__ __launch_bounds__(max_threads<kLong64R>()) void jw_long64_kernel(jw_long_desc d, int64_t per_wg)
+{
+    constexpr int R = kLong64R;
+    constexpr int C = 64 * R;
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
+    extern __shared__ __align__(16) unsigned char smem[];
+    const double* tab = reinterpret_cast<const double*>(smem);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* stg = reinterpret_cast<double2*>(smem + NTAB * 8) + (size_t)wave * 64 * R;
+    const jw_run_desc& rd = d.run;
+    const int64_t nc = d.nchunks;
+    const int64_t nsys = MODE == JW_LONG_COEF ? 3 : rd.nsys;
+    const int64_t ppc = (nsys + 1) >> 1;
+    const int64_t u0 = (int64_t)blockIdx.x * per_wg;
+    const int64_t u1 = u0 + per_wg < nc * ppc ? u0 + per_wg : nc * ppc;
+    for (int64_t c = u0 / ppc; c < nc && c * ppc < u1; c++) {
+        __syncthreads();   // the previous chunk's passes are done: its table may go
+        copy_table_lds<NTAB>(smem, reinterpret_cast<const double*>(d.table) + c * d.tab_elems, wave, nwaves, lane);
+        zero_staging<R>(stg, lane);   // this chunk's keep-range may be narrower
+        __syncthreads();
+        // keep-ranges relative to the chunk, inside its rows of the system
+        const int64_t rb = c * C;
+        const int64_t cn = (int64_t)rd.N - rb < C ? (int64_t)rd.N - rb : C;
+        auto clip = [&](int64_t v) -> int {
+            const int64_t w = v - rb;
+            return (int)(w < 0 ? 0 : (w > cn ? cn : w));
+        };
+        const int ilo = clip(rd.in_lo), olo = clip(rd.out_lo);
+        const int ihi = clip(rd.in_hi) > ilo ? clip(rd.in_hi) : ilo;
+        const int ohi = clip(rd.out_hi) > olo ? clip(rd.out_hi) : olo;
+        const int64_t q0 = (u0 > c * ppc ? u0 : c * ppc) - c * ppc;
+        const int64_t q1 = (u1 < (c + 1) * ppc ? u1 : (c + 1) * ppc) - c * ppc;
+        for (int64_t q = q0 + wave; q < q1; q += nwaves) {
+            const jw_pass64 g = make_pass64(reinterpret_cast<double*>(rd.rhs), nsys, rd.stride, rd.offSol + rb, ilo,
+                                            ihi, olo, ohi, rd.kl, rd.ku, rd.fwd, rd.b1, rd.b2,
+                                            MODE == JW_LONG_FINISH && c == 0 && rd.apply_scale, rd.scale_hi, lane);
+            long64_pass<MODE, ALL3>(tab, stg, lane, g, d, c, 2 * q);
+        }
+    }
+}
+
+// The front of a long FP64 step in one launch: the chunks' table builds, then COEF and
+// REDUCE of the first slice of systems behind them.  Workgroups take tickets in start
+// order.  The first nbuild build the tables, NSUB chunks each (pivots handed from build to
+// build), mark the build's tables written (tflags) and leave.  The others drain a queue
+// of units in chunk order (unit 0 of a chunk: COEF on two wavefronts; unit j > 0: REDUCE of
+// pairs [(j-1) P, j P), two passes per wavefront), waiting for each chunk's tables before
+// staging them, so that the reduction runs while later tables are still being built.  A
+// worker's ticket comes after every build's, so every build it waits for has started, and
+// builds wait only for earlier builds: the launch drains whatever the residency.  All
+// waits are bounded (the device error flag is raised when one runs out).
+// sync: [0] ticket, [1] queue counter, [kLong64Tflags + b] build b's tables written;
+// slots: build b's last pivot (below); both set by jw_long64_clear_kernel before the launch.
+constexpr int kLong64Nsub = 4;
+constexpr int kLong64PairsPerUnit = 2 * (max_threads<kLong64R>() / 64);
+constexpr int kLong64Tflags = 32;   // the table flags start one 128-byte line after the counters
+
+// Pivot hand-over between the builds through one 64-bit slot per build that holds the bits
+// of the build's last pivot, all ones (a NaN pattern never published: a NaN pivot is
+// published as the canonical NaN) until then.  The value carries its own readiness, so
+// relaxed atomics suffice (the next build needs nothing else the previous one wrote): one
+// store and the polls that see it, no flag, no release or acquire.  The wait is bounded:
+// when it runs out the device error flag is raised and the build goes on with a unit pivot.
+constexpr unsigned long long kPivotEmpty = ~0ull;
+
+struct SlotHooks {
+    double* carry;                  // LDS: carry[0] = g entering the build
+    unsigned long long* slots;
+    int* err;
+    int64_t build;
+    int chained;
+    __device__ void wait_carry(int tid) const
+    {
+        if (!chained || build == 0 || tid != 0) return;
+        int polls = 0;
+        unsigned long long v;
+        while ((v = __hip_atomic_load(&slots[build - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kPivotEmpty) {
+            if (++polls > kLongMaxPolls) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                v = (unsigned long long)__double_as_longlong(1.0);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        carry[0] = __longlong_as_double((long long)v);
+    }
+    __device__ void publish(int tid, const double* sg, int RL) const
+    {
+        if (!chained || tid != 0) return;
+        const double g = sg[RL - 1];
+        const unsigned long long v = g != g ? 0x7FF8000000000000ull : (unsigned long long)__double_as_longlong(g);
+        __hip_atomic_store(&slots[build], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+__device__ __forceinline__ void wait_table(int* flag, int* err)
+{
+    int polls = 0;
+    // relaxed polls (no cache invalidation per poll), one acquire once the flag is seen
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (++polls > kLongMaxPolls) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(16);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+template <bool ALL3>
+__global__ __launch_bounds__(max_threads<kLong64R>()) void jw_long64_front_kernel(jw_source_desc src, const void* mat,
+                                                                                 jw_long_desc d, int* sync,
+                                                                                 unsigned long long* slots, int* err,
+                                                                                 int nbuild, int chained)
+{
+    constexpr int R = kLong64R;
+    constexpr int C = 64 * R;
+    constexpr int SR = kLong64Nsub * C;
+    constexpr int NTAB = JW_TAB64_ELEMS(R);
+    constexpr int P = kLong64PairsPerUnit;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_id;
+    int* queue = sync + 1;
+    int* tflags = sync + kLong64Tflags;
+    double* table = reinterpret_cast<double*>(const_cast<void*>(d.table));
+    if (threadIdx.x == 0) s_id = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int tk = s_id;
+    if (tk < nbuild) {
+        const int64_t rb = (int64_t)tk * SR;
+        // the pivot carry follows the build's five row arrays; thread 0 reads the raw
+        // super-diagonal entry of the row before the build (the build's first barrier
+        // publishes it)
+        double* carry = reinterpret_cast<double*>(smem + prologue_lds_bytes(SR, sizeof(double)));
+        if (chained && threadIdx.x == 0)
+            carry[1] = rb > 0 ? mem_load<double>(mat, src.udS + source_pos(src, rb - 1, src.N), src.mem_f64) : 0.0;
+        const SlotHooks hooks{carry, slots, err, tk, chained};
+        prologue_body<double, SlotHooks>(src, mat, table + (int64_t)tk * kLong64Nsub * NTAB, 64, R, 2, smem, rb,
+                                         chained ? carry : nullptr, kLong64Nsub, hooks);
+        // every thread's table stores complete at agent scope, then the flag
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(&tflags[tk], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const double* tab = reinterpret_cast<const double*>(smem);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    double2* stg = reinterpret_cast<double2*>(smem + NTAB * 8) + (size_t)wave * 64 * R;
+    const jw_run_desc& rd = d.run;
+    const int64_t nc = d.nchunks;
+    const int64_t ppc = (rd.nsys + 1) >> 1;
+    const int upc = 1 + (int)((ppc + P - 1) / P);
+    const int nunits = (int)nc * upc;
+    int64_t cur = -1;
+    for (;;) {
+        __syncthreads();   // the previous unit is finished: s_id and the table may change
+        if (threadIdx.x == 0) s_id = __hip_atomic_fetch_add(queue, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_id);
+        if (u >= nunits) break;
+        const int64_t c = u / upc;
+        const int j = u - (int)c * upc;
+        if (c != cur) {
+            if (threadIdx.x == 0) wait_table(&tflags[c / kLong64Nsub], err);
+            __syncthreads();
+            copy_table_lds<NTAB>(smem, table + c * NTAB, wave, nwaves, lane);
+            zero_staging<R>(stg, lane);   // this chunk's keep-range may be narrower
+            __syncthreads();
+            cur = c;
+        }
+        const int64_t rb = c * C;
+        const int64_t cn = (int64_t)rd.N - rb < C ? (int64_t)rd.N - rb : C;
+        auto clip = [&](int64_t v) -> int {
+            const int64_t w = v - rb;
+            return (int)(w < 0 ? 0 : (w > cn ? cn : w));
+        };
+        const int ilo = clip(rd.in_lo), olo = clip(rd.out_lo);
+        const int ihi = clip(rd.in_hi) > ilo ? clip(rd.in_hi) : ilo;
+        const int ohi = clip(rd.out_hi) > olo ? clip(rd.out_hi) : olo;
+        if (j == 0) {
+            // COEF: the 3 unit-carry systems, passes 0 and 1 on wavefronts 0 and 1
+            if (wave < 2) {
+                const jw_pass64 g = make_pass64(reinterpret_cast<double*>(rd.rhs), 3, rd.stride, rd.offSol + rb, ilo,
+                                                ihi, olo, ohi, rd.kl, rd.ku, rd.fwd, rd.b1, rd.b2, false, 1.0, lane);
+                long64_pass<JW_LONG_COEF, ALL3>(tab, stg, lane, g, d, c, 2 * (int64_t)wave);
+            }
+        } else {
+            const int64_t q0 = (int64_t)(j - 1) * P;
+            const int64_t q1 = q0 + P < ppc ? q0 + P : ppc;
+            for (int64_t q = q0 + wave; q < q1; q += nwaves) {
+                const jw_pass64 g = make_pass64(reinterpret_cast<double*>(rd.rhs), rd.nsys, rd.stride, rd.offSol + rb,
+                                                ilo, ihi, olo, ohi, rd.kl, rd.ku, rd.fwd, rd.b1, rd.b2, false, 1.0,
+                                                lane);
+                long64_pass<JW_LONG_REDUCE, ALL3>(tab, stg, lane, g, d, c, 2 * q);
+            }
+        }
     }
 }
 
@@ -1557,6 +1943,38 @@ const void* pick64(int R, bool all3 = true)
     }
 }
 
+template <int MODE>
+const void* long64_ptr(bool all3)
+{
+    return all3 ? reinterpret_cast<const void*>(&jw_long64_kernel<MODE, true>)
+                : reinterpret_cast<const void*>(&jw_long64_kernel<MODE, false>);
+}
+
+const void* pick_long64(int mode, bool all3)
+{
+    switch (mode) {
+    case JW_LONG_COEF: return long64_ptr<JW_LONG_COEF>(all3);
+    case JW_LONG_REDUCE: return long64_ptr<JW_LONG_REDUCE>(all3);
+    case JW_LONG_FINISH: return long64_ptr<JW_LONG_FINISH>(all3);
+    default: return nullptr;
+    }
+}
+
+constexpr int kLong64Waves = max_threads<kLong64R>() / 64;
+
+const void* pick_long64_front(bool all3)
+{
+    return all3 ? reinterpret_cast<const void*>(&jw_long64_front_kernel<true>)
+                : reinterpret_cast<const void*>(&jw_long64_front_kernel<false>);
+}
+
+// zeroes the counters and table flags, empties the pivot slots
+__global__ void jw_long64_clear_kernel(int* p, int n, unsigned long long* slots, int nslots)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) slots[i] = kPivotEmpty;
+}
+
 // clears the work-queue counters (a kernel, so that it can sit in a captured graph)
 __global__ void jw_zero_counters_kernel(int* p, int n)
 {
@@ -1971,6 +2389,144 @@ int jwdev_main64s(const jw_run_desc* r, const jw_source_desc* src, const void* m
     void* args[] = {const_cast<jw_run_desc*>(r), &c, &d_err, const_cast<jw_source_desc*>(src), &matrices};
     e = hipLaunchKernel(f, dim3(grid), dim3(64 * (c + st)), args, lds, reinterpret_cast<hipStream_t>(stream));
     return e == hipSuccess ? 0 : -3;
+}
+
+/* Long FP64 systems (16-byte aligned) on the pair machinery: one workgroup of 12
+ * wavefronts per CU (the chunk's table and a staging of 2 x 512 rows per wavefront) */
+int64_t jwdev_long64_lds_bytes(void)
+{
+    return (int64_t)JW_TAB64_ELEMS(kLong64R) * 8 + 2LL * 64 * kLong64R * 8 * kLong64Waves;
+}
+
+int jwdev_long64_occupancy(void)
+{
+    int best = -1;
+    for (int m = JW_LONG_COEF; m <= JW_LONG_FINISH + 1; m++) {
+        for (int v = 0; v < 2; v++) {
+            const void* f = m <= JW_LONG_FINISH ? pick_long64(m, v == 0) : pick_long64_front(v == 0);
+            const int lds = (int)jwdev_long64_lds_bytes();
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess) return 0;
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * kLong64Waves, lds) != hipSuccess) return 0;
+            if (best < 0 || nb < best) best = nb;
+        }
+    }
+    return best < 0 ? 0 : best;
+}
+
+/* the front of a step (jw_long64_front_kernel): the table builds of every chunk, COEF and
+ * REDUCE of the slice in d (its first systems) in one launch, after a fill kernel that
+ * zeroes the 32 + nbuild ints of sync and empties the nbuild 8-byte pivot slots at gval */
+int jwdev_long64_front(const jw_source_desc* src, const void* matrices, const jw_long_desc* d, int* sync, void* gval,
+                       int* err, int cu_count, int occupancy, void* stream)
+{
+    const int C = 64 * kLong64R;
+    const int64_t SR = (int64_t)kLong64Nsub * C;
+    if (!matrices || !d->table || !d->red || !d->run.rhs || !sync || !gval || !err) {
+        jwdev_set_error("long64 front: no matrices, table, reduced data, rhs or hand-over scratch");
+        return -1;
+    }
+    const int64_t nbuild = (src->N + SR - 1) / SR;
+    if (src->N < 1 || nbuild > (1 << 20) || d->nchunks < 1 || (int64_t)d->nchunks * C < d->run.N ||
+        d->nchunks > nbuild * kLong64Nsub || d->tab_elems != JW_TAB64_ELEMS(kLong64R) || cu_count < 1 ||
+        occupancy < 1 || d->run.nsys < 1 || d->run.nsys > d->red_sys) {
+        jwdev_set_error("long64 front: bad size, chunk count, table size, slice or grid");
+        return -1;
+    }
+    if (((uintptr_t)d->run.rhs % 16) != 0 || (d->run.stride % 2) != 0 || (d->run.offSol % 2) != 0) {
+        jwdev_set_error("long64 front: systems not 16-byte aligned");
+        return -1;
+    }
+    const int64_t ppc = (d->run.nsys + 1) / 2;
+    const int64_t units = (int64_t)d->nchunks * (1 + (ppc + kLong64PairsPerUnit - 1) / kLong64PairsPerUnit);
+    if (units > (1LL << 30)) {
+        jwdev_set_error("long64 front: too many work units");
+        return -1;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int nclear = (int)(kLong64Tflags + nbuild);
+    unsigned long long* slots = reinterpret_cast<unsigned long long*>(gval);
+    hipLaunchKernelGGL(jw_long64_clear_kernel, dim3((unsigned)((nclear + 255) / 256)), dim3(256), 0, s, sync, nclear,
+                       slots, (int)nbuild);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        jwdev_set_error("long64 front: clear launch failed");
+        return -3;
+    }
+    const void* f = pick_long64_front(d->run.fwd && d->run.b1 && d->run.b2);
+    const size_t lds = (size_t)jwdev_long64_lds_bytes();
+    if (prologue_lds_bytes((int)SR, sizeof(double)) + 2 * sizeof(double) > lds) {
+        jwdev_set_error("long64 front: table build does not fit");
+        return -1;
+    }
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+        jwdev_set_error("long64 front: hipFuncSetAttribute failed");
+        return -2;
+    }
+    /* every build's workgroup, then a full GPU of workers */
+    const int64_t grid = nbuild + (int64_t)cu_count * occupancy;
+    jw_source_desc sd = *src;
+    const void* mat = matrices;
+    jw_long_desc dd = *d;
+    int nb = (int)nbuild;
+    int chained = src->solve_kind == JW_SOLVE_TRI;
+    void* args[] = {&sd, (void*)&mat, &dd, &sync, &slots, &err, &nb, &chained};
+    e = hipLaunchKernel(f, dim3((unsigned)grid), dim3(64 * kLong64Waves), args, lds, s);
+    if (e != hipSuccess) {
+        jwdev_set_error("long64 front: launch failed");
+        return -3;
+    }
+    return 0;
+}
+
+int jwdev_long64(const jw_long_desc* d, int cu_count, int occupancy, void* stream)
+{
+    const int C = 64 * kLong64R;
+    if (!d->table || !d->red || (d->mode != JW_LONG_COEF && !d->run.rhs)) {
+        jwdev_set_error("long64: no table, reduced data or rhs");
+        return -1;
+    }
+    if (d->mode < JW_LONG_COEF || d->mode > JW_LONG_FINISH || d->nchunks < 1 ||
+        (int64_t)d->nchunks * C < d->run.N || d->tab_elems != JW_TAB64_ELEMS(kLong64R) || cu_count < 1 ||
+        occupancy < 1) {
+        jwdev_set_error("long64: bad mode, chunk count, table size or grid");
+        return -1;
+    }
+    if (d->mode != JW_LONG_COEF && d->run.nsys > d->red_sys) {
+        jwdev_set_error("long64: slice exceeds reduced data");
+        return -1;
+    }
+    /* the DMA moves 16-byte row pairs: aligned systems only */
+    if (d->mode != JW_LONG_COEF &&
+        (((uintptr_t)d->run.rhs % 16) != 0 || (d->run.stride % 2) != 0 || (d->run.offSol % 2) != 0)) {
+        jwdev_set_error("long64: systems not 16-byte aligned");
+        return -1;
+    }
+    const int64_t nsys = d->mode == JW_LONG_COEF ? 3 : d->run.nsys;
+    if (nsys < 1) return 0;
+    const int64_t total = (int64_t)d->nchunks * ((nsys + 1) / 2);
+    int64_t grid = (int64_t)cu_count * occupancy;
+    const int64_t need = (total + kLong64Waves - 1) / kLong64Waves;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    int64_t per_wg = (total + grid - 1) / grid;
+    grid = (total + per_wg - 1) / per_wg;
+    const void* f = pick_long64(d->mode, d->run.fwd && d->run.b1 && d->run.b2);
+    const size_t lds = (size_t)jwdev_long64_lds_bytes();
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+        jwdev_set_error("long64: hipFuncSetAttribute failed");
+        return -2;
+    }
+    void* args[] = {const_cast<jw_long_desc*>(d), &per_wg};
+    e = hipLaunchKernel(f, dim3((unsigned)grid), dim3(64 * kLong64Waves), args, lds,
+                        reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        jwdev_set_error("long64: launch failed");
+        return -3;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -45,8 +45,10 @@ struct pfsolve_plan {
     /* long systems cut into chunks of 64 long_R rows (long_R = 0: not available; long_only:
      * the only path of this plan).  A slot's area holds, from long_tab_off, long_ntab
      * format-1 tables of long_tab_elems elements, and from long_red_off the reduced data
-     * for slices of long_red_sys systems; long_occ chunk workgroups fit one CU */
-    int long_R, long_only, long_occ;
+     * for slices of long_red_sys systems; long_occ chunk workgroups fit one CU.  FP64 plans
+     * run 16-byte aligned systems on the pair machinery instead (format-2 tables in the same
+     * space, long64_occ of its workgroups per CU; 0: not available) */
+    int long_R, long_only, long_occ, long64_occ;
     int64_t long_nc, long_ntab, long_tab_elems, long_red_sys;
     size_t long_tab_off, long_red_off;
     /* pivot hand-over between table builds: ticket and per-build flags (ints), then the

@@ -216,10 +216,12 @@ struct NoPivotHooks {
 };
 
 // One table build.  Local row r of the build is row rb + r of the system.  A build of a
-// long system's later rows (rb > 0, format 1) takes from `carry` the pivot of the row
-// before its first (carry[0] = g_{rb-1}) and that row's raw super-diagonal entry
-// (carry[1] = ud_{rb-1}), so that its pivots continue the exact recurrence; it writes
-// nsub format-1 tables of R * L rows each, one after the other (table_elems apart).
+// long system's later rows (rb > 0) takes from `carry` the pivot of the row before its
+// first (carry[0] = g_{rb-1}) and that row's raw super-diagonal entry (carry[1] =
+// ud_{rb-1}), so that its pivots continue the exact recurrence; it writes nsub tables of
+// R * L rows each, one after the other (format 1: table_elems apart; format 2, L = 64:
+// JW_TAB64_ELEMS(R) apart, the scalings folded across the build's ends too, so that the
+// tables of consecutive builds continue each other's scaled recurrences).
 // Hooks of a build that hands its pivots on (tridiagonal solves only): wait_carry(tid),
 // called by every thread just before the barrier after which carry[0] is first read, and
 // publish(tid, sg, RL), called by every thread once the g of every row is final in sg.
@@ -274,6 +276,13 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
             sr[j][3] = d.has_m ? mem_load<T>(mat, d.udM + P, mf) : A::zero();
             row_band_load<T>(d, mat, gq, N, P, raw[j]);
         }
+    }
+    // format 2 folds D and g of the rows next to the build into its first and last rows:
+    // a build of a long system's rows takes D_{rb-1} and D_{rb+RL} from the matrix
+    T dprev = A::one(), dnext = A::one();
+    if (format == 2 && d.has_m) {
+        if (tid == 0 && rb >= 1 && rb - 1 < N) dprev = mem_load<T>(mat, d.mdM + source_pos(d, rb - 1, N), mf);
+        if (tid == (RL - 1) % nthr && rb + RL < N) dnext = mem_load<T>(mat, d.mdM + source_pos(d, rb + RL, N), mf);
     }
 #pragma unroll
     for (int j = 0; j < kProRows; j++) {
@@ -568,38 +577,48 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     __syncthreads();
 
     if (format == 2) {
+        // nsub tables of 64 R rows each, JW_TAB64_ELEMS(R) apart; in a build of a long
+        // system's rows the first row's l' and the last row's h' take a and D of the rows
+        // next to the build (g_{rb-1} from the carry of a tridiagonal solve, else 1)
+        const int64_t te2 = JW_TAB64_ELEMS(R);
+        const T aprev = A::mul(dprev, has_carry ? carry[0] : A::one());
         // fold the scalings (row-parallel); l' and h' replace l and h in LDS (a row
         // reads only its own l and h)
 #pragma unroll
         for (int j = 0; j < kProRows; j++) {
             const int r = tid + j * nthr;
             if (r >= RL) continue;
-            const int t = r / R, i = r % R;
+            const int kt = r / RLs, rr = r - kt * RLs;
+            const int t = rr / R, i = rr % R;
             const T ar = A::mul(sD[r], sg[r]);
             T cf[8];
 #pragma unroll
             for (int k = 0; k < 5; k++) cf[k] = A::mul(ar, vb[j][k]);
             T lp = A::zero(), hp = A::zero();
-            if (r >= 1 && r < N) lp = A::div(A::mul(ar, sl[r]), A::mul(sD[r - 1], sg[r - 1]));
-            if (r + 1 < N) hp = A::div(A::mul(sD[r], sh[r]), sD[r + 1]);
+            if (rb + r >= 1 && rb + r < N)
+                lp = A::div(A::mul(ar, sl[r]), r >= 1 ? A::mul(sD[r - 1], sg[r - 1]) : aprev);
+            if (rb + r + 1 < N) hp = A::div(A::mul(sD[r], sh[r]), r + 1 < RL ? sD[r + 1] : dnext);
             cf[5] = lp;
             cf[6] = hp;
             cf[7] = sE[r];
+            T* tk = table + kt * te2;
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                table[tab_coef64(p, i, t, R)] = cf[2 * p];
-                table[tab_coef64(p, i, t, R) + 1] = cf[2 * p + 1];
+                tk[tab_coef64(p, i, t, R)] = cf[2 * p];
+                tk[tab_coef64(p, i, t, R) + 1] = cf[2 * p + 1];
             }
             sl[r] = lp;
             sh[r] = hp;
         }
         __syncthreads();
-        // scan multipliers: wavefront j (wave-uniform) takes recurrence j (0: forward
-        // through l', 1: backward through h', 2: backward through E)
-        if (tid < 192) {
-            const int j = tid >> 6;
+        // scan multipliers: wavefront item (wave-uniform) takes recurrence j = item % 3
+        // (0: forward through l', 1: backward through h', 2: backward through E) of
+        // table item / 3
+        for (int item = tid >> 6; item < 3 * nsub; item += nthr >> 6) {
+            const int kt = item / 3, j = item - 3 * kt;
             const int t = tid & 63;
-            const T* src = j == 0 ? sl : (j == 1 ? sh : sE);
+            T* const tabk = table + kt * te2;
+            const T* src = (j == 0 ? sl : (j == 1 ? sh : sE)) + kt * RLs;
             T v = A::one();
             for (int i = 0; i < R; i++) v = A::mul(v, A::neg(src[t * R + i]));
             // in-row Hillis-Steele: before step k the value covers the 2^k lanes a
@@ -609,7 +628,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int s = 1 << k;
-                table[SC + (int64_t)(j * 8 + k) * 64 + t] = v;
+                tabk[SC + (int64_t)(j * 8 + k) * 64 + t] = v;
                 if (j == 0) {
                     const T up = lane_fetch(v, tr >= s ? t - s : t);
                     if (tr >= s) v = A::mul(v, up);
@@ -627,10 +646,10 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 T m2 = A::zero();
                 if (rw == 2) m2 = v;
                 if (rw == 3) m2 = A::mul(pre47, v);
-                table[SC + (int64_t)4 * 64 + t] = v;
-                table[SC + (int64_t)5 * 64 + t] = m2;
-                table[SC + (int64_t)6 * 64 + t] = A::zero();
-                table[SC + (int64_t)7 * 64 + t] = A::zero();
+                tabk[SC + (int64_t)4 * 64 + t] = v;
+                tabk[SC + (int64_t)5 * 64 + t] = m2;
+                tabk[SC + (int64_t)6 * 64 + t] = A::zero();
+                tabk[SC + (int64_t)7 * 64 + t] = A::zero();
             } else {
                 // backward cross-row combine from the in-row results S_1..S_3 at the
                 // row starts (lanes 16, 32, 48): a lane of row r adds
@@ -641,10 +660,10 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 if (rw == 0) { c1 = v; c2 = A::mul(v, P1); c3 = A::mul(c2, P2); }
                 if (rw == 1) { c2 = v; c3 = A::mul(v, P2); }
                 if (rw == 2) c3 = v;
-                table[SC + (int64_t)(j * 8 + 4) * 64 + t] = c1;
-                table[SC + (int64_t)(j * 8 + 5) * 64 + t] = c2;
-                table[SC + (int64_t)(j * 8 + 6) * 64 + t] = c3;
-                table[SC + (int64_t)(j * 8 + 7) * 64 + t] = A::zero();
+                tabk[SC + (int64_t)(j * 8 + 4) * 64 + t] = c1;
+                tabk[SC + (int64_t)(j * 8 + 5) * 64 + t] = c2;
+                tabk[SC + (int64_t)(j * 8 + 6) * 64 + t] = c3;
+                tabk[SC + (int64_t)(j * 8 + 7) * 64 + t] = A::zero();
             }
         }
         return;
@@ -697,5 +716,42 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         }
     }
 }
+
+// Pivot hand-over between the table builds of a long system (tridiagonal solves): build b
+// waits for build b-1's last pivot (flags[b-1], acquire) and publishes its own (release).
+// The wait is bounded: when it runs out the device error flag is raised and the build goes
+// on with a unit pivot, so the grid always drains.
+constexpr int kLongMaxPolls = 1 << 22;
+
+template <typename T>
+struct ChainHooks {
+    T* carry;        // LDS: carry[0] = g entering the build
+    int* flags;      // per build: 1 once its last pivot is in gval
+    T* gval;         // per build: g of its last row
+    int* err;
+    int64_t build;
+    int chained;
+    __device__ void wait_carry(int tid) const
+    {
+        if (!chained || build == 0 || tid != 0) return;
+        int polls = 0;
+        bool ok = true;
+        while (__hip_atomic_load(&flags[build - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            if (++polls > kLongMaxPolls) {
+                ok = false;
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        carry[0] = ok ? gval[build - 1] : Ar<T>::one();
+    }
+    __device__ void publish(int tid, const T* sg, int RL) const
+    {
+        if (!chained || tid != 0) return;
+        gval[build] = sg[RL - 1];
+        __hip_atomic_store(&flags[build], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
 
 #endif  // PFSOLVE_JW_PROLOGUE_H

@@ -127,6 +127,9 @@ static PfSolveResult long_setup(PfSolveApplication* app, struct pfsolve_plan* p)
     p->long_tab_elems = jwdev_table_elems(64, R);
     p->long_occ = jwdev_long_occupancy(kp);
     if (p->long_occ < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
+    /* format-2 tables are smaller than format-1 ones: the same space holds either */
+    p->long64_occ = (kp == JW_PREC_F64 && R == 8 && jwdev_table64_elems(8) <= p->long_tab_elems)
+                        ? jwdev_long64_occupancy() : 0;
     /* systems per slice: every system of the plan, up to 64 MiB of reduced data */
     int64_t nsys = (int64_t)(cfg->size[1] * cfg->size[2]);
     int64_t cap = ((int64_t)64 << 20) / (JW_LONG_ARRAYS * p->long_nc * (int64_t)elt);
@@ -139,7 +142,8 @@ static PfSolveResult long_setup(PfSolveApplication* app, struct pfsolve_plan* p)
     const int64_t nbuild = p->long_ntab / jwdev_long_nsub(kp);
     p->long_sync_off = p->long_red_off +
                        ((size_t)(JW_LONG_ARRAYS * p->long_nc * cap + 9 * p->long_nc) * elt + 255) / 256 * 256;
-    p->long_gval_off = p->long_sync_off + ((size_t)(nbuild + 1) * sizeof(int) + 255) / 256 * 256;
+    /* ticket and pivot flags (chunk kernel); counters and table flags (pair path: 32 + nbuild) */
+    p->long_gval_off = p->long_sync_off + ((size_t)(nbuild + 32) * sizeof(int) + 255) / 256 * 256;
     p->table_bytes = p->long_gval_off + (size_t)nbuild * elt;
     p->table_stride = (p->table_bytes + 255) / 256 * 256;
     return PFSOLVE_SUCCESS;
@@ -552,8 +556,31 @@ static PfSolveResult append_long(PfSolveApplication* app, const jw_source_desc* 
     d.run.vec2 = p->precision == JW_PREC_F64 && ((uintptr_t)run->rhs % 16) == 0 && (run->stride % 2) == 0 &&
                  (run->offSol % 2) == 0;
     if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
-    if (jwdev_long_prologue(src, cfg->buffer[0], (void*)d.table, d.tab_elems, p->precision,
-                            (int*)((char*)slot_area + p->long_sync_off), (char*)slot_area + p->long_gval_off, p->d_err,
+    int* sync = (int*)((char*)slot_area + p->long_sync_off);
+    void* gval = (char*)slot_area + p->long_gval_off;
+    if (d.run.vec2 && p->long64_occ > 0) {
+        /* aligned FP64 systems: the chunks on the pair machinery (format-2 tables); the
+         * table builds, COEF and the first slice's REDUCE in one launch */
+        d.tab_elems = jwdev_table64_elems(8);
+        for (int64_t s0 = 0; s0 < run->nsys; s0 += p->long_red_sys) {
+            const int64_t n = run->nsys - s0 < p->long_red_sys ? run->nsys - s0 : p->long_red_sys;
+            d.run.nsys = n;
+            d.run.offSol = run->offSol + s0 * run->stride;
+            d.mode = JW_LONG_REDUCE;
+            if (s0 == 0) {
+                if (jwdev_long64_front(src, cfg->buffer[0], &d, sync, gval, p->d_err, p->cu_count, p->long64_occ, s) != 0)
+                    return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            } else if (jwdev_long64(&d, p->cu_count, p->long64_occ, s) != 0) {
+                return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            }
+            if (jwdev_long_reduced(&d, p->precision, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+            d.mode = JW_LONG_FINISH;
+            if (jwdev_long64(&d, p->cu_count, p->long64_occ, s) != 0) return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
+        }
+        if (p->ev_stop && hipEventRecord(p->ev_stop, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
+        return PFSOLVE_SUCCESS;
+    }
+    if (jwdev_long_prologue(src, cfg->buffer[0], (void*)d.table, d.tab_elems, p->precision, sync, gval, p->d_err,
                             s) != 0)
         return PFSOLVE_ERROR_FAILED_TO_LAUNCH_KERNEL;
     d.mode = JW_LONG_COEF;

@@ -20,18 +20,19 @@ LONG_SIZES = [2049, 3000, 3841, 10000]
 
 @pytest.mark.parametrize("jw_type,ub", MODES)
 @pytest.mark.parametrize("N", LONG_SIZES)
-@pytest.mark.parametrize("rt", [0, 1])
+@pytest.mark.parametrize("rt", [0, 1, 2])
 def test_long_reference_modes(jw_type, ub, N, rt):
     """every reference mode; rt: runtime matrix / solution offsets, keep-ranges that cut
-    chunks, scaleC and an odd system stride"""
+    chunks and scaleC, with an odd solution offset (rt = 1: the chunk kernel) or with even
+    strides and offsets (rt = 2: 16-byte aligned systems, the pair kernel's chunks)"""
     print(f"long modes jw{jw_type} ub{ub} N={N} rt={rt}", flush=True)
     jw = jw_type % 10
     B = 5
     offM, offV = (13, 3 * N + 29) if rt else (0, 0)
     total = max(3 * N, offV + N, offM + 2 * N) + 8
     buf = mode_buffer(jw, ub, N, rt, offM, total, 3000 + N + jw)
-    S = N + 7 if rt else N
-    offS = 5 if rt else 0
+    S = (N + 7 if rt == 1 else N + 8 - N % 2) if rt else N
+    offS = (5 if rt == 1 else 6) if rt else 0
     izp = (N // 4 + 1, N - 3) if rt else (0, 0)
     ozp = (515, N - N // 8) if rt else (0, 0)
     x = O.uniform(91 + N, -1, 1, B * S + offS + 9)
@@ -71,11 +72,12 @@ def test_long_fused_step(N, permuted):
         assert O.relerr_norm(got[:N], y * np.r_[0.5, np.ones(N - 1)]) <= TOL64   # row-0 scaleC
 
 
-def test_long_fused_runtime_keep_ranges_and_custom_split():
+@pytest.mark.parametrize("pad", [3, 4])
+def test_long_fused_runtime_keep_ranges_and_custom_split(pad):
     """caller-chosen Msplit0 (1000-row storage blocks), runtime keep-ranges whose ends fall
-    inside chunks, odd stride, runtime scaleC"""
+    inside chunks, runtime scaleC; odd stride (the chunk kernel) or even (the pair kernel)"""
     N, B = 5000, 5
-    S = N + 3
+    S = N + pad
     raw = O.raw_matrices(77, N)
     offV, offT, offM = 0, 5 * N + 3, 8 * N + 11
     V, Ta, Tb, Tc, Md, Me = raw
