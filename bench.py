@@ -766,8 +766,10 @@ def bench_cfg0(torch, pf, steps: int, warmup: int, N: int = 64, B: int = 1024) -
 def bench_long(torch, pf, steps: int, warmup: int, N: int = 100000, ncx: int = 256) -> dict:
     """Long systems: the fused step y = M^-1 T^-1 V x at N = 100000 rows (matrices in the
     reference's Msplit0 = 704 block storage) on 256 complex systems (XXII), FP64.  The step is
-    the chunk path: per-chunk tables with the pivots handed from build to build, the chunks'
-    responses to unit carries, the reduction, the reduced solve and the finishing pass."""
+    the chunk path on the pair machinery (16-byte aligned systems): one launch builds the
+    per-chunk tables (pivots handed from build to build) while its other workgroups compute
+    the chunks' responses to unit carries and reduce the chunks behind the builds, then the
+    reduced solve and the finishing pass."""
     lib = pf.lib()
     nreal = 2 * ncx
     buf, (offV, offT, offM) = gen_fused_matrices(N, 3, lib.pfsolve_reorder_i0)
@@ -788,8 +790,9 @@ def bench_long(torch, pf, steps: int, warmup: int, N: int = 100000, ncx: int = 2
                     f"systems (XXII), KL=KU=2, FP64",
         "value": ncx / (ms * 1e-3), "unit": "complex systems/s", "ms_per_step": ms,
         "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
-        "chunk_rows": 64 * geo.rows_per_lane, "kernels": "jw_long_prologue_kernel, jw_long_chunk_kernel x3, "
-                                                         "jw_long_reduced_kernel",
+        "chunk_rows": 64 * geo.rows_per_lane,
+        "kernels": "jw_long64_clear_kernel, jw_long64_front_kernel (table builds + COEF + REDUCE), "
+                   "jw_long_reduced_kernel, jw_long64_kernel (FINISH)",
         "kernels_ms": kms, "algorithmic_bytes": alg, "hbm_frac": alg / (kms * 1e-3) / HBM_PEAK,
     }
 

@@ -254,6 +254,7 @@ int jwdev_long_reduced(const jw_long_desc* d, int precision, void* stream);
 /* long FP64 systems whose rows are 16-byte aligned: the chunk modes on the pair machinery
  * (format-2 tables of JW_TAB64_ELEMS(8) elements per chunk, built by jwdev_long_prologue
  * when given that table size); a persistent grid of cu_count * occupancy workgroups at most */
+int jwdev_long64_nsub(void);   /* chunks per table build */
 int64_t jwdev_long64_lds_bytes(void);
 int jwdev_long64_occupancy(void);
 int jwdev_long64(const jw_long_desc* d, int cu_count, int occupancy, void* stream);

@@ -854,7 +854,7 @@ __global__ __launch_bounds__(max_threads<kLong64R>()) void jw_long64_kernel(jw_l
 // waits are bounded (the device error flag is raised when one runs out).
 // sync: [0] ticket, [1] queue counter, [kLong64Tflags + b] build b's tables written;
 // slots: build b's last pivot (below); both set by jw_long64_clear_kernel before the launch.
-constexpr int kLong64Nsub = 4;
+constexpr int kLong64Nsub = 6;   // chunks per table build (3072 rows: 384 threads in the pivot scan)
 constexpr int kLong64PairsPerUnit = 2 * (max_threads<kLong64R>() / 64);
 constexpr int kLong64Tflags = 32;   // the table flags start one 128-byte line after the counters
 
@@ -2393,6 +2393,8 @@ int jwdev_main64s(const jw_run_desc* r, const jw_source_desc* src, const void* m
 
 /* Long FP64 systems (16-byte aligned) on the pair machinery: one workgroup of 12
  * wavefronts per CU (the chunk's table and a staging of 2 x 512 rows per wavefront) */
+int jwdev_long64_nsub(void) { return kLong64Nsub; }
+
 int64_t jwdev_long64_lds_bytes(void)
 {
     return (int64_t)JW_TAB64_ELEMS(kLong64R) * 8 + 2LL * 64 * kLong64R * 8 * kLong64Waves;

@@ -56,8 +56,8 @@ __host__ __device__ __forceinline__ int ilog2_rt(int v)
 // wavefront 0.
 // ------------------------------------------------------------------------
 constexpr int kPrologueThreads = 512;
-constexpr int kProRows = 4;   // rows per thread: R * L <= 2048
-constexpr int kMaxRT = 8;     // rows per thread of the pivot scan (256 threads): R * L <= 2048
+constexpr int kProRows = 4;   // rows per thread: a build's rows <= 4 x its threads
+constexpr int kMaxRT = 8;     // rows per thread of the pivot scan: a build's rows <= 8 x NP
 
 // LDS of prologue_body: 5 arrays of R*L elements (ld -> l -> l', ud -> h -> h', D, E, g)
 // five arrays of RL values; the last (g) is at least 16 values long, since the
@@ -254,7 +254,17 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
 
     // phase 1: raw solve coefficients per row (zeros outside the matrix) into LDS,
     // band coefficients into registers (format 1: straight into the table); row
-    // r = tid + j * nthr of thread tid
+    // row_of(j) of thread tid.  Format 1: tid + j nthr.  Format 2 goes lane-major: lane t
+    // of wavefront w takes row t R + i of table k, (k, i) the (w + j nwaves)-th pair, so
+    // that one table store of the fold (a coefficient pair of row-in-lane i over the 64
+    // lanes) writes 1 KiB of contiguous memory.  A row index of RL or more: no row.
+    const int nwv = nthr >> 6;
+    auto row_of = [&](int j) -> int {
+        if (format != 2) return tid + j * nthr;
+        const int c = (tid >> 6) + nwv * j;
+        const int k = c / R;
+        return k * RLs + (tid & 63) * R + (c - k * R);
+    };
     T vb[kProRows][5];
     // FP64: every row's loads first (rows outside the matrix read row 0's positions and
     // are discarded), then the selects, products and LDS stores, so that a thread's rows
@@ -265,9 +275,10 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     if constexpr (PRE) {
 #pragma unroll
         for (int j = 0; j < kProRows; j++) {
-            const int r = tid + j * nthr;
+            const int r = row_of(j);
             const int64_t gr = rb + r;
-            if (j * nthr >= RL) continue;   // no thread has a row here (workgroup-uniform)
+            // no thread of the wavefront has a row here (wavefront-uniform)
+            if (format != 2 ? j * nthr >= RL : (tid >> 6) + nwv * j >= RL / 64) continue;
             const int64_t gq = (r < RL && gr < N) ? gr : 0;
             const int64_t P = source_pos(d, gq, N);
             sr[j][0] = has_f ? mem_load<T>(mat, d.ldS + P, mf) : A::zero();
@@ -279,14 +290,15 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     }
     // format 2 folds D and g of the rows next to the build into its first and last rows:
     // a build of a long system's rows takes D_{rb-1} and D_{rb+RL} from the matrix
+    // (every thread reads them: one address each, workgroup-uniform conditions)
     T dprev = A::one(), dnext = A::one();
     if (format == 2 && d.has_m) {
-        if (tid == 0 && rb >= 1 && rb - 1 < N) dprev = mem_load<T>(mat, d.mdM + source_pos(d, rb - 1, N), mf);
-        if (tid == (RL - 1) % nthr && rb + RL < N) dnext = mem_load<T>(mat, d.mdM + source_pos(d, rb + RL, N), mf);
+        if (rb >= 1 && rb - 1 < N) dprev = mem_load<T>(mat, d.mdM + source_pos(d, rb - 1, N), mf);
+        if (rb + RL < N) dnext = mem_load<T>(mat, d.mdM + source_pos(d, rb + RL, N), mf);
     }
 #pragma unroll
     for (int j = 0; j < kProRows; j++) {
-        const int r = tid + j * nthr;
+        const int r = row_of(j);
         const int64_t gr = rb + r;
         T ldv = A::zero(), udv = A::zero(), Dv = A::zero(), Ev = A::zero();
 #pragma unroll
@@ -340,13 +352,15 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     // each wavefront, then the wavefront totals through LDS) gives every thread
     // the map of all rows before its own, applied to (1,1) the pivot of the row
     // before its first; each thread then runs the exact recurrence over its rows.
-    // NP is fixed, so the rounding does not depend on the workgroup size (every
-    // kernel that builds a table gives the same table, bit for bit).
+    // NP depends on the build's rows only (256 up to 2048 rows, one wavefront more per
+    // further 512), so the rounding does not depend on the workgroup size (every kernel
+    // that builds a table gives the same table, bit for bit).
     if (d.solve_kind == JW_SOLVE_TRI) {
-        constexpr int NP = 256;      // participating threads (workgroups have >= 256)
-        // the four wavefront totals live in g's array, which nothing reads or writes
-        // until the per-row pass below (a barrier keeps that pass after every read):
-        // no static LDS, so four 256-thread builds of 1024 rows fit one CU
+        // participating threads (the workgroup has at least NP)
+        const int NP = RL <= 64 * 4 * kMaxRT ? 256 : (RL + 64 * kMaxRT - 1) / (64 * kMaxRT) * 64;
+        // the wavefront totals live in g's array, which nothing reads or writes until the
+        // per-row pass below (a barrier keeps that pass after every read): no static LDS,
+        // so four 256-thread builds of 1024 rows fit one CU
         T* s_tot = sg;
         const int RT = (RL + NP - 1) / NP;
         const int r0 = tid < NP ? tid * RT : RL;   // threads >= NP own no rows
@@ -586,7 +600,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         // reads only its own l and h)
 #pragma unroll
         for (int j = 0; j < kProRows; j++) {
-            const int r = tid + j * nthr;
+            const int r = row_of(j);
             if (r >= RL) continue;
             const int kt = r / RLs, rr = r - kt * RLs;
             const int t = rr / R, i = rr % R;

@@ -127,9 +127,8 @@ static PfSolveResult long_setup(PfSolveApplication* app, struct pfsolve_plan* p)
     p->long_tab_elems = jwdev_table_elems(64, R);
     p->long_occ = jwdev_long_occupancy(kp);
     if (p->long_occ < 1) return PFSOLVE_ERROR_FAILED_TO_SET_DYNAMIC_SHARED_MEMORY;
-    /* format-2 tables are smaller than format-1 ones: the same space holds either */
-    p->long64_occ = (kp == JW_PREC_F64 && R == 8 && jwdev_table64_elems(8) <= p->long_tab_elems)
-                        ? jwdev_long64_occupancy() : 0;
+    /* FP64: 16-byte aligned systems may take the pair path (its tables share the space below) */
+    p->long64_occ = (kp == JW_PREC_F64 && R == 8) ? jwdev_long64_occupancy() : 0;
     /* systems per slice: every system of the plan, up to 64 MiB of reduced data */
     int64_t nsys = (int64_t)(cfg->size[1] * cfg->size[2]);
     int64_t cap = ((int64_t)64 << 20) / (JW_LONG_ARRAYS * p->long_nc * (int64_t)elt);
@@ -138,7 +137,14 @@ static PfSolveResult long_setup(PfSolveApplication* app, struct pfsolve_plan* p)
     cap = (cap + 1) / 2 * 2;
     p->long_red_sys = cap;
     p->long_tab_off = (p->table_bytes + 255) / 256 * 256;
-    p->long_red_off = p->long_tab_off + ((size_t)p->long_ntab * p->long_tab_elems * elt + 255) / 256 * 256;
+    /* the pair path's format-2 tables (builds of jwdev_long64_nsub chunks) in the same space */
+    size_t tab_bytes = (size_t)p->long_ntab * p->long_tab_elems * elt;
+    if (kp == JW_PREC_F64) {
+        const int64_t b64 = rows * jwdev_long64_nsub();
+        const size_t t64 = (size_t)((p->N + b64 - 1) / b64 * jwdev_long64_nsub()) * jwdev_table64_elems(8) * elt;
+        if (t64 > tab_bytes) tab_bytes = t64;
+    }
+    p->long_red_off = p->long_tab_off + (tab_bytes + 255) / 256 * 256;
     const int64_t nbuild = p->long_ntab / jwdev_long_nsub(kp);
     p->long_sync_off = p->long_red_off +
                        ((size_t)(JW_LONG_ARRAYS * p->long_nc * cap + 9 * p->long_nc) * elt + 255) / 256 * 256;
