@@ -940,8 +940,11 @@ __global__ __launch_bounds__(max_threads<kLong64R>()) void jw_long64_front_kerne
         const SlotHooks hooks{carry, slots, err, tk, chained};
         prologue_body<double, SlotHooks>(src, mat, table + (int64_t)tk * kLong64Nsub * NTAB, 64, R, 2, smem, rb,
                                          chained ? carry : nullptr, kLong64Nsub, hooks);
-        // every thread's table stores complete at agent scope, then the flag
-        __threadfence();
+        // every wavefront waits for its own table stores (written into the L2), then one
+        // agent-scope release by thread 0 writes the L2 back and sets the flag (one L2
+        // writeback per build: a fence in every wavefront measured 3x longer under the
+        // workers' load, and slowed them)
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (threadIdx.x == 0) __hip_atomic_store(&tflags[tk], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         return;
@@ -1446,7 +1449,7 @@ __global__ __launch_bounds__(split_threads<R>()) void jw_main64s_kernel(jw_run_d
     // every workgroup builds the coefficient table in its own LDS from the
     // caller's matrices (no separate prologue launch); the staging area, unused
     // until the barrier below, is the build's scratch
-    prologue_body<double>(src, mat, reinterpret_cast<double*>(smem), 64, R, 2, smem + NTAB * 8);
+    prologue_body<double, NoPivotHooks, true>(src, mat, reinterpret_cast<double*>(smem), 64, R, 2, smem + NTAB * 8);
     // block operator factors past the staging and the build's scratch
     double* fin = reinterpret_cast<double*>(
         smem + NTAB * 8 + max_i64(2LL * 64 * R * 8 * ncomp, (int64_t)prologue_lds_bytes(64 * R, 8)));
@@ -1616,8 +1619,8 @@ template <int R>
 __device__ __noinline__ void build_table_lds(const jw_prologue_job* job)
 {
     extern __shared__ __align__(16) unsigned char smem_build[];
-    prologue_body<double>(job->src, job->matrices, reinterpret_cast<double*>(smem_build), 64, R, 2,
-                          smem_build + JW_TAB64_ELEMS(R) * 8);
+    prologue_body<double, NoPivotHooks, true>(job->src, job->matrices, reinterpret_cast<double*>(smem_build), 64, R,
+                                              2, smem_build + JW_TAB64_ELEMS(R) * 8);
 }
 
 // The batch queue with the table built in LDS: as jw_batch64_kernel, but a workgroup

@@ -225,7 +225,8 @@ struct NoPivotHooks {
 // Hooks of a build that hands its pivots on (tridiagonal solves only): wait_carry(tid),
 // called by every thread just before the barrier after which carry[0] is first read, and
 // publish(tid, sg, RL), called by every thread once the g of every row is final in sg.
-template <typename T, typename H = NoPivotHooks>
+// LDS_TABLE: the table is in LDS (the row order of the build, below, follows from it).
+template <typename T, typename H = NoPivotHooks, bool LDS_TABLE = false>
 __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const void* mat, T* table, int L, int R, int format,
                               unsigned char* smem_raw, int64_t rb = 0, const T* carry = nullptr, int nsub = 1,
                               const H& hooks = H())
@@ -254,13 +255,15 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
 
     // phase 1: raw solve coefficients per row (zeros outside the matrix) into LDS,
     // band coefficients into registers (format 1: straight into the table); row
-    // row_of(j) of thread tid.  Format 1: tid + j nthr.  Format 2 goes lane-major: lane t
-    // of wavefront w takes row t R + i of table k, (k, i) the (w + j nwaves)-th pair, so
-    // that one table store of the fold (a coefficient pair of row-in-lane i over the 64
-    // lanes) writes 1 KiB of contiguous memory.  A row index of RL or more: no row.
+    // row_of(j) of thread tid: tid + j nthr (consecutive threads read consecutive rows),
+    // except for a format-2 table in LDS, which goes lane-major: lane t of wavefront w
+    // takes row t R + i of table k, (k, i) the (w + j nwaves)-th pair, so that a table
+    // store of the fold (a coefficient pair of row-in-lane i over the 64 lanes) writes
+    // 1 KiB of consecutive LDS without bank conflicts.  A row of RL or more: no row.
     const int nwv = nthr >> 6;
+    const bool lane_major = LDS_TABLE && format == 2;
     auto row_of = [&](int j) -> int {
-        if (format != 2) return tid + j * nthr;
+        if (!lane_major) return tid + j * nthr;
         const int c = (tid >> 6) + nwv * j;
         const int k = c / R;
         return k * RLs + (tid & 63) * R + (c - k * R);
@@ -278,7 +281,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
             const int r = row_of(j);
             const int64_t gr = rb + r;
             // no thread of the wavefront has a row here (wavefront-uniform)
-            if (format != 2 ? j * nthr >= RL : (tid >> 6) + nwv * j >= RL / 64) continue;
+            if (!lane_major ? j * nthr >= RL : (tid >> 6) + nwv * j >= RL / 64) continue;
             const int64_t gq = (r < RL && gr < N) ? gr : 0;
             const int64_t P = source_pos(d, gq, N);
             sr[j][0] = has_f ? mem_load<T>(mat, d.ldS + P, mf) : A::zero();
