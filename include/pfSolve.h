@@ -247,7 +247,7 @@ typedef struct {
     /* ---- extension fields (no reference counterpart), appended ---- */
     int64_t offsetT;             /* fused step (jw_type 14): tridiagonal block T */
     int64_t autotune;            /* 0: built-in table, 1: run the autotuner at plan time */
-    int64_t kernelVariant;       /* 0: automatic, 1: general kernel, 2: wave-per-system FP64 kernel */
+    int64_t kernelVariant;       /* 0: automatic, 1: general kernel (long systems: the chunk kernel), 2: wave-per-system FP64 kernel */
     /* Fused block operators around a JW step (block codes as in `block`: layout +
      * 20 for accumulate + 100/200/300/400 for ScaleC/ScaleD/SphLaplA/SphLaplB).
      * inputBlock (layout 4) reads complex (interleaved re, im) systems from

@@ -564,9 +564,10 @@ static PfSolveResult append_long(PfSolveApplication* app, const jw_source_desc* 
     if (p->ev_start && hipEventRecord(p->ev_start, s) != hipSuccess) return PFSOLVE_ERROR_FAILED_TO_EVENT_RECORD;
     int* sync = (int*)((char*)slot_area + p->long_sync_off);
     void* gval = (char*)slot_area + p->long_gval_off;
-    if (d.run.vec2 && p->long64_occ > 0) {
+    if (d.run.vec2 && p->long64_occ > 0 && cfg->kernelVariant != 1) {
         /* aligned FP64 systems: the chunks on the pair machinery (format-2 tables); the
-         * table builds, COEF and the first slice's REDUCE in one launch */
+         * table builds, COEF and the first slice's REDUCE in one launch (kernelVariant 1:
+         * the chunk kernel, as for every other long system) */
         d.tab_elems = jwdev_table64_elems(8);
         for (int64_t s0 = 0; s0 < run->nsys; s0 += p->long_red_sys) {
             const int64_t n = run->nsys - s0 < p->long_red_sys ? run->nsys - s0 : p->long_red_sys;

@@ -72,6 +72,28 @@ def test_long_fused_step(N, permuted):
         assert O.relerr_norm(got[:N], y * np.r_[0.5, np.ones(N - 1)]) <= TOL64   # row-0 scaleC
 
 
+@pytest.mark.parametrize("N", [3000, 10000])
+def test_long_chunk_kernel_on_aligned_systems(N):
+    """kernelVariant 1 keeps 16-byte aligned FP64 systems on the chunk kernel (format-1 tables,
+    separate launches); the default takes the pair path: both against the oracle, and the two
+    against each other"""
+    print(f"long chunk kernel aligned N={N}", flush=True)
+    B = 6
+    raw, buf, (offV, offT, offM) = fused_buffers(N, N + 2, 1)
+    x = O.uniform(23 + N, -1, 1, B * N)
+    outs = []
+    for variant in (1, 0):
+        cfg = dict(size=[N, B, 1], jw_type=14, doublePrecision=1, scaleC=0.5, KL=2, KU=2, kernelVariant=variant,
+                   offsetV=offV, offsetT=offT, offsetM=offM)
+        outs.append(run_plan(cfg, buf, x))
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=0.5,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    want = O.jw_f128(p, buf, x)
+    for got in outs:
+        check_systems(got, want, x, B, N, 0, N, 0, N, TOL64)
+    assert O.relerr_norm(outs[0], outs[1]) <= 2 * TOL64
+
+
 @pytest.mark.parametrize("pad", [3, 4])
 def test_long_fused_runtime_keep_ranges_and_custom_split(pad):
     """caller-chosen Msplit0 (1000-row storage blocks), runtime keep-ranges whose ends fall
