@@ -431,11 +431,10 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 xa = A::one(); xb = A::zero(); xc = A::zero(); xe = A::one();
             }
         }
-        hooks.wait_carry(tid);
-        __syncthreads();
-        T gprev = A::one();
+        __syncthreads();   // the wavefront totals are in LDS
         if (part) {
-            // the maps of all earlier wavefronts, then the in-wavefront part on top
+            // the maps of all earlier wavefronts, then the in-wavefront part on top (none of
+            // it depends on a carry, so a build that waits for one has it ready)
             T wa = A::one(), wb = A::zero(), wc = A::zero(), we = A::one();
             for (int w = 0; w < wv; w++) {
                 T ta = s_tot[4 * w + 0], tb = s_tot[4 * w + 1], tc = s_tot[4 * w + 2], te = s_tot[4 * w + 3];
@@ -443,6 +442,11 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 wa = ta; wb = tb; wc = tc; we = te;
             }
             compose(xa, xb, xc, xe, wa, wb, wc, we);
+        }
+        hooks.wait_carry(tid);
+        __syncthreads();   // the carry is in LDS, the totals are read: g's array is free
+        T gprev = A::one();
+        if (part) {
             // g_{r-1} = 1 / p_{r-1} of the row before this thread's first: the prefix map
             // applied to (1, 1) (the first row has no sub-diagonal, so any start works), or
             // after a carry to (1, g_in), i.e. p = 1 / g_in
@@ -453,7 +457,6 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
                 gprev = A::div(A::add(xc, xe), A::add(xa, xb));
             }
         }
-        __syncthreads();   // the totals are read: g's array is free
         // first pass: the exact recurrence over this thread's rows from the scanned pivot;
         // the rows' ld / ud stay in registers (nothing reads another thread's raw ld again)
         T pr[kMaxRT], ldv[kMaxRT], udv[kMaxRT];
@@ -505,6 +508,9 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         }
         __syncthreads();
         const bool refine = A::hi(sh[0]) != 0.0;
+        // not refined: every first-pass g is final (in sg since the barrier before the
+        // check), so the last one goes to the next build at once
+        if (!refine) hooks.publish(tid, sg, RL);
         T rho[kMaxRT], cc[kMaxRT];
         T sc = A::one(), ss = A::zero();      // this thread's map e -> sc e + ss
         if (refine && part) {
@@ -567,7 +573,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         // refined: the corrected g of every row (and of the row before) is in sg; not
         // refined: every thread has read the flag before thread 0 writes h_0 over it
         __syncthreads();
-        hooks.publish(tid, sg, RL);
+        if (refine) hooks.publish(tid, sg, RL);
         if (part) {
 #pragma unroll
             for (int i = 0; i < kMaxRT; i++) {
