@@ -1619,8 +1619,8 @@ template <int R>
 __device__ __noinline__ void build_table_lds(const jw_prologue_job* job)
 {
     extern __shared__ __align__(16) unsigned char smem_build[];
-    prologue_body<double, NoPivotHooks, true>(job->src, job->matrices, reinterpret_cast<double*>(smem_build), 64, R,
-                                              2, smem_build + JW_TAB64_ELEMS(R) * 8);
+    prologue_body<double>(job->src, job->matrices, reinterpret_cast<double*>(smem_build), 64, R, 2,
+                          smem_build + JW_TAB64_ELEMS(R) * 8);
 }
 
 // The batch queue with the table built in LDS: as jw_batch64_kernel, but a workgroup

@@ -225,8 +225,8 @@ struct NoPivotHooks {
 // Hooks of a build that hands its pivots on (tridiagonal solves only): wait_carry(tid),
 // called by every thread just before the barrier after which carry[0] is first read, and
 // publish(tid, sg, RL), called by every thread once the g of every row is final in sg.
-// LDS_TABLE: the table is in LDS (the row order of the build, below, follows from it).
-template <typename T, typename H = NoPivotHooks, bool LDS_TABLE = false>
+// LANE_MAJOR: the rows of a format-2 build go lane-major over the threads (below).
+template <typename T, typename H = NoPivotHooks, bool LANE_MAJOR = false>
 __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const void* mat, T* table, int L, int R, int format,
                               unsigned char* smem_raw, int64_t rb = 0, const T* carry = nullptr, int nsub = 1,
                               const H& hooks = H())
@@ -256,12 +256,15 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
     // phase 1: raw solve coefficients per row (zeros outside the matrix) into LDS,
     // band coefficients into registers (format 1: straight into the table); row
     // row_of(j) of thread tid: tid + j nthr (consecutive threads read consecutive rows),
-    // except for a format-2 table in LDS, which goes lane-major: lane t of wavefront w
-    // takes row t R + i of table k, (k, i) the (w + j nwaves)-th pair, so that a table
-    // store of the fold (a coefficient pair of row-in-lane i over the 64 lanes) writes
-    // 1 KiB of consecutive LDS without bank conflicts.  A row of RL or more: no row.
+    // or with LANE_MAJOR (format 2) lane t of wavefront w takes row t R + i of table k,
+    // (k, i) the (w + j nwaves)-th pair, so that a table store of the fold (a coefficient
+    // pair of row-in-lane i over the 64 lanes) writes 1 KiB of consecutive memory: without
+    // bank conflicts for a table in LDS, at the price of loads that stride R rows over the
+    // lanes.  Measured: faster for the split kernel's build (its matrices are read by every
+    // workgroup, from the L2), slower for a build that reads its own matrices from HBM
+    // (batch queue kernel 4, the long-system builds).  A row of RL or more: no row.
     const int nwv = nthr >> 6;
-    const bool lane_major = LDS_TABLE && format == 2;
+    const bool lane_major = LANE_MAJOR && format == 2;
     auto row_of = [&](int j) -> int {
         if (!lane_major) return tid + j * nthr;
         const int c = (tid >> 6) + nwv * j;
