@@ -36,12 +36,18 @@
 #include "jw_common.h"
 #include "jw_prologue.h"
 
-template <typename T>
+// SMALL: a build of at most kPrologueThreads rows (one row per thread, at most 2 per thread
+// in the pivot scan), whose per-thread arrays then fit in fewer registers (the double-double
+// build spills otherwise)
+template <typename T, bool SMALL = false>
 __global__ __launch_bounds__(kPrologueThreads) void jw_prologue_kernel(jw_source_desc d, const void* mat, T* table,
                                                                        int L, int R, int format)
 {
     extern __shared__ __align__(16) unsigned char pro_smem[];
-    prologue_body<T>(d, mat, table, L, R, format, pro_smem);
+    if constexpr (SMALL)
+        prologue_body<T, NoPivotHooks, false, 1, 2>(d, mat, table, L, R, format, pro_smem);
+    else
+        prologue_body<T>(d, mat, table, L, R, format, pro_smem);
 }
 
 // batched: one workgroup per matrix set; workgroup 0 also clears the work-queue counters.
@@ -538,17 +544,18 @@ int jwdev_prologue(const jw_source_desc* d, const void* matrices, void* table, i
     }
     if (format == 2 && L != 64) { snprintf(g_err, sizeof(g_err), "format 2 needs 64 lanes"); return -1; }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const void* fn = precision == JW_PREC_F64 ? reinterpret_cast<const void*>(&jw_prologue_kernel<double>)
-                                              : reinterpret_cast<const void*>(&jw_prologue_kernel<dd>);
+    // a build of at most one row per thread takes the small-array variant (same table)
+    const bool small = RL <= kPrologueThreads;
+    const void* fn = precision == JW_PREC_F64
+                         ? (small ? reinterpret_cast<const void*>(&jw_prologue_kernel<double, true>)
+                                  : reinterpret_cast<const void*>(&jw_prologue_kernel<double, false>))
+                         : (small ? reinterpret_cast<const void*>(&jw_prologue_kernel<dd, true>)
+                                  : reinterpret_cast<const void*>(&jw_prologue_kernel<dd, false>));
     hipError_t ea = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (ea != hipSuccess) { set_err("hipFuncSetAttribute(prologue)", ea); return -2; }
-    if (precision == JW_PREC_F64) {
-        hipLaunchKernelGGL(jw_prologue_kernel<double>, dim3(1), dim3(kPrologueThreads), lds, s, *d, matrices,
-                           reinterpret_cast<double*>(table), L, R, format);
-    } else {
-        hipLaunchKernelGGL(jw_prologue_kernel<dd>, dim3(1), dim3(kPrologueThreads), lds, s, *d, matrices,
-                           reinterpret_cast<dd*>(table), L, R, format);
-    }
+    void* args[] = {const_cast<jw_source_desc*>(d), const_cast<void**>(&matrices), &table, &L, &R, &format};
+    hipError_t el = hipLaunchKernel(fn, dim3(1), dim3(kPrologueThreads), args, lds, s);
+    if (el != hipSuccess) { set_err("prologue launch", el); return -1; }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { set_err("prologue launch", e); return -1; }
     return 0;

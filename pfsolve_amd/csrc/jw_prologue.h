@@ -225,8 +225,10 @@ struct NoPivotHooks {
 // Hooks of a build that hands its pivots on (tridiagonal solves only): wait_carry(tid),
 // called by every thread just before the barrier after which carry[0] is first read, and
 // publish(tid, sg, RL), called by every thread once the g of every row is final in sg.
-// LANE_MAJOR: the rows of a format-2 build go lane-major over the threads (below).
-template <typename T, typename H = NoPivotHooks, bool LANE_MAJOR = false>
+// LANE_MAJOR: the rows of a format-2 build go lane-major over the threads (below).  MAXJ,
+// MAXRT: rows per thread at most in the row phases (RL <= MAXJ nthr) and in the pivot scan
+// (RL <= MAXRT NP); smaller bounds keep a small build's per-thread arrays in fewer registers.
+template <typename T, typename H = NoPivotHooks, bool LANE_MAJOR = false, int MAXJ = kProRows, int MAXRT = kMaxRT>
 __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const void* mat, T* table, int L, int R, int format,
                               unsigned char* smem_raw, int64_t rb = 0, const T* carry = nullptr, int nsub = 1,
                               const H& hooks = H())
@@ -271,16 +273,16 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         const int k = c / R;
         return k * RLs + (tid & 63) * R + (c - k * R);
     };
-    T vb[kProRows][5];
+    T vb[MAXJ][5];
     // FP64: every row's loads first (rows outside the matrix read row 0's positions and
     // are discarded), then the selects, products and LDS stores, so that a thread's rows
     // cost one round of memory latency.  Double-double loads and finishes row by row (the
     // rows' double-double loads at once need more registers than they save time).
     constexpr bool PRE = sizeof(T) == 8;
-    T raw[PRE ? kProRows : 1][6], sr[PRE ? kProRows : 1][4];
+    T raw[PRE ? MAXJ : 1][6], sr[PRE ? MAXJ : 1][4];
     if constexpr (PRE) {
 #pragma unroll
-        for (int j = 0; j < kProRows; j++) {
+        for (int j = 0; j < MAXJ; j++) {
             const int r = row_of(j);
             const int64_t gr = rb + r;
             // no thread of the wavefront has a row here (wavefront-uniform)
@@ -303,7 +305,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         if (rb + RL < N) dnext = mem_load<T>(mat, d.mdM + source_pos(d, rb + RL, N), mf);
     }
 #pragma unroll
-    for (int j = 0; j < kProRows; j++) {
+    for (int j = 0; j < MAXJ; j++) {
         const int r = row_of(j);
         const int64_t gr = rb + r;
         T ldv = A::zero(), udv = A::zero(), Dv = A::zero(), Ev = A::zero();
@@ -462,11 +464,11 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         }
         // first pass: the exact recurrence over this thread's rows from the scanned pivot;
         // the rows' ld / ud stay in registers (nothing reads another thread's raw ld again)
-        T pr[kMaxRT], ldv[kMaxRT], udv[kMaxRT];
+        T pr[MAXRT], ldv[MAXRT], udv[MAXRT];
         const T udp0 = udprev;
         if (part) {
 #pragma unroll
-            for (int i = 0; i < kMaxRT; i++) {
+            for (int i = 0; i < MAXRT; i++) {
                 pr[i] = A::one();
                 ldv[i] = A::zero();
                 udv[i] = A::zero();
@@ -514,13 +516,13 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         // not refined: every first-pass g is final (in sg since the barrier before the
         // check), so the last one goes to the next build at once
         if (!refine) hooks.publish(tid, sg, RL);
-        T rho[kMaxRT], cc[kMaxRT];
+        T rho[MAXRT], cc[MAXRT];
         T sc = A::one(), ss = A::zero();      // this thread's map e -> sc e + ss
         if (refine && part) {
             T gm = (r0 >= 1 && r0 < RL) ? sg[r0 - 1] : ((r0 == 0 && has_carry) ? carry[0] : A::one());
             udprev = udp0;
 #pragma unroll
-            for (int i = 0; i < kMaxRT; i++) {
+            for (int i = 0; i < MAXRT; i++) {
                 rho[i] = A::zero();
                 cc[i] = A::zero();
                 const int r = r0 + i;
@@ -562,7 +564,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
             for (int w = 0; w < wv; w++) ein = A::madd(sl[2 * w + 1], sl[2 * w], ein);
             ein = A::madd(ss, sc, ein);
 #pragma unroll
-            for (int i = 0; i < kMaxRT; i++) {
+            for (int i = 0; i < MAXRT; i++) {
                 const int r = r0 + i;
                 if (i >= RT || rb + r >= N) continue;
                 ein = A::madd(rho[i], cc[i], ein);
@@ -579,7 +581,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         if (refine) hooks.publish(tid, sg, RL);
         if (part) {
 #pragma unroll
-            for (int i = 0; i < kMaxRT; i++) {
+            for (int i = 0; i < MAXRT; i++) {
                 const int r = r0 + i;
                 if (i >= RT || r >= RL) continue;
                 // l_r = ld_r g_{r-1},  h_r = ud_r g_r  (zero outside the matrix)
@@ -611,7 +613,7 @@ __device__ __forceinline__ void prologue_body(const jw_source_desc& d, const voi
         // fold the scalings (row-parallel); l' and h' replace l and h in LDS (a row
         // reads only its own l and h)
 #pragma unroll
-        for (int j = 0; j < kProRows; j++) {
+        for (int j = 0; j < MAXJ; j++) {
             const int r = row_of(j);
             if (r >= RL) continue;
             const int kt = r / RLs, rr = r - kt * RLs;
