@@ -59,6 +59,10 @@ constexpr int kPrologueThreads = 512;
 constexpr int kProRows = 4;   // rows per thread: a build's rows <= 4 x its threads
 constexpr int kMaxRT = 8;     // rows per thread of the pivot scan: a build's rows <= 8 x NP
 
+// rows per thread at most of a build of RL rows on at least 256 threads (row phases and
+// pivot scan alike), for the MAXJ / MAXRT bounds of a build whose size is known at compile time
+__host__ __device__ constexpr int pro_rows(int RL) { return RL <= 256 ? 1 : (RL + 255) / 256; }
+
 // LDS of prologue_body: 5 arrays of R*L elements (ld -> l -> l', ud -> h -> h', D, E, g)
 // five arrays of RL values; the last (g) is at least 16 values long, since the
 // pivot scan keeps its wavefront totals there before g is written
