@@ -781,6 +781,9 @@ def bench_long(torch, pf, steps: int, warmup: int, N: int = 100000, ncx: int = 2
                    doublePrecision=1, scaleC=1.0, KL=2, KU=2, offsetV=offV, offsetT=offT, offsetM=offM)
     geo = plan.geometry()
     ms, per, _, kms = time_plan_steps(torch, lib, plan, stream, warmup, steps, plan.append)
+    # the same step's launches (fill, front, reduced solve, finishing pass) replayed from a
+    # hipGraph of ten steps: no gap between the launches
+    graph10_ms = time_graph_replays(torch, stream, 2, max(2, steps // 10), plan.append, per_graph=10)
     plan.close()
     assert torch.isfinite(x).all().item(), "non-finite long-system output"
     # minimal traffic: every row of every system read and written once, the matrices once
@@ -790,6 +793,7 @@ def bench_long(torch, pf, steps: int, warmup: int, N: int = 100000, ncx: int = 2
                     f"systems (XXII), KL=KU=2, FP64",
         "value": ncx / (ms * 1e-3), "unit": "complex systems/s", "ms_per_step": ms,
         "p50_ms": float(np.percentile(per, 50)), "p99_ms": float(np.percentile(per, 99)),
+        "graph_ms_per_step_10_per_graph": graph10_ms,
         "chunk_rows": 64 * geo.rows_per_lane,
         "kernels": "jw_long64_clear_kernel, jw_long64_front_kernel (table builds + COEF + REDUCE), "
                    "jw_long_reduced_kernel, jw_long64_kernel (FINISH)",

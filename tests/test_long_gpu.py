@@ -10,7 +10,7 @@ import pytest
 from pfsolve_amd import _abi
 
 import _oracle as O
-from jwgpu import run_plan
+from jwgpu import run_plan, to_dev
 from test_jw_gpu import MODES, RT_ALL, TOL64, check_systems, dd_inputs, fused_buffers, mode_buffer
 
 pytestmark = pytest.mark.gpu
@@ -118,6 +118,47 @@ def test_long_fused_runtime_keep_ranges_and_custom_split(pad):
                   cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
     want = O.jw_f128(p, buf, x)
     check_systems(got, want, x, B, S, 0, N, 3, N - 513, TOL64)
+
+
+def test_long_step_replayed_from_a_graph():
+    """the pair path's launches (fill kernel, front, reduced solve, finishing pass) captured
+    once in a hipGraph and replayed on fresh right-hand sides, with eager copies and
+    synchronizations between the replays: every replay bit-identical to the eager step"""
+    import torch as t
+    import pfsolve_amd as pf
+    N, B = 10000, 6
+    raw, buf, (offV, offT, offM) = fused_buffers(N, 31, 1)
+    dmat = to_dev(buf)
+    xs = [O.uniform(40 + i, -1, 1, B * N) for i in range(3)]
+    drhs = to_dev(xs[0])
+    stream = t.cuda.Stream()
+    plan = pf.Plan(dmat.data_ptr(), drhs.data_ptr(), stream=stream.cuda_stream, size=[N, B, 1], jw_type=14,
+                   doublePrecision=1, scaleC=0.5, KL=2, KU=2, offsetV=offV, offsetT=offT, offsetM=offM)
+    try:
+        want = []
+        for x in xs:
+            drhs.copy_(t.from_numpy(x).cuda())
+            t.cuda.synchronize()
+            plan.append()
+            stream.synchronize()
+            want.append(drhs.cpu().numpy())
+        graph = t.cuda.CUDAGraph()
+        t.cuda.synchronize()
+        with t.cuda.graph(graph, stream=stream, capture_error_mode="relaxed"):
+            plan.append()
+        for i, x in enumerate(xs):
+            drhs.copy_(t.from_numpy(x).cuda())
+            t.cuda.synchronize()
+            graph.replay()
+            t.cuda.synchronize()
+            got = drhs.cpu().numpy()
+            assert np.array_equal(got, want[i]), f"replay {i} differs from the eager step"
+        plan.sync()
+    finally:
+        plan.close()
+    p = O.problem(jw_type=14, N=N, B=B, KL=2, KU=2, cfg_outStride=N, cfg_scaleC=0.5,
+                  cfg_offsetV=offV, cfg_offsetT=offT, cfg_offsetM=offM)
+    check_systems(want[2], O.jw_f128(p, buf, xs[2]), xs[2], B, N, 0, N, 0, N, TOL64)
 
 
 def test_long_slices_of_the_batch():
